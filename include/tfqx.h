@@ -1,0 +1,172 @@
+/* tfqx: C-ABI of the MI355X (gfx950) circuit-simulation op family.
+ *
+ * Drop-in for TFQ 0.7.7's native simulation ops.  Each op of the reference is
+ * a TensorFlow kernel loaded with load_op_library; each is replaced here by a
+ * plan (host: parse, resolve, fuse) followed by an upload and a launch on a
+ * caller-supplied HIP stream:
+ *
+ *   tfqx_plan_simulate_expectation          TfqSimulateExpectation
+ *       core/ops/tfq_simulate_expectation_op.cc:45-283 (REGISTER_OP :253-283)
+ *   tfqx_plan_simulate_sampled_expectation  TfqSimulateSampledExpectation
+ *       core/ops/tfq_simulate_sampled_expectation_op.cc:48-342 (:308-342)
+ *   tfqx_plan_simulate_samples              TfqSimulateSamples
+ *       core/ops/tfq_simulate_samples_op.cc:48-285 (:254-285)
+ *   tfqx_plan_simulate_state                TfqSimulateState
+ *       core/ops/tfq_simulate_state_op.cc:43-242 (:218-242)
+ *   tfqx_plan_inner_product                 TfqInnerProduct
+ *       core/ops/math_ops/tfq_inner_product.cc:45-323 (:294-323)
+ *   tfqx_plan_adjoint_gradient              TfqAdjointGradient
+ *       core/ops/tfq_adj_grad_op.cc:46-427 (:393-427)
+ *
+ * Inputs mirror the reference op inputs.  A string tensor of n elements is
+ * passed as (bytes, offsets[n+1]): element i is bytes[offsets[i]:offsets[i+1]].
+ * Dense inputs are host pointers in row-major order.  Outputs are
+ * caller-allocated device buffers with the reference's dtypes and shapes:
+ *
+ *   expectation / sampled expectation  float32   [batch, num_ops]
+ *   samples                            int8      [batch, num_samples, max_nq]
+ *   state                              complex64 [batch, 2^max_nq]
+ *   inner product                      complex64 [batch, num_other]
+ *   adjoint gradient                   float32   [batch, num_symbols]
+ *
+ * with the reference's sentinels: -2 expectation / 0 gradient / 1+0j inner
+ * product for empty programs, -2 padding of samples and states.
+ *
+ * Errors: every call returns a status (0 = OK) and writes a message into
+ * (err, err_len); messages keep the reference's wording (e.g. "Unparseable
+ * proto", "Could not find symbol in parameter map", "Found a Pauli sum
+ * operating on qubits not found in circuit").
+ *
+ * Memory: plans own host memory only.  Device memory is a caller-allocated
+ * workspace of tfqx_plan_workspace_bytes() bytes (256-byte aligned); the
+ * library performs no device allocation on the op path.
+ */
+#ifndef TFQX_H_
+#define TFQX_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  TFQX_OK = 0,
+  TFQX_INVALID_ARGUMENT = 3,
+  TFQX_UNIMPLEMENTED = 12,
+  TFQX_INTERNAL = 13,
+  TFQX_DEVICE_ERROR = 14
+};
+
+/* Adjoint derivative of a parameterised gate. */
+enum {
+  TFQX_GRAD_TFQ_FD = 0,   /* (G(v+eps)-G(v-eps))/(2 eps), eps=5e-3, float32: adj_util.cc:175-302 */
+  TFQX_GRAD_ANALYTIC = 1  /* exact dG/dv */
+};
+
+typedef struct tfqx_plan tfqx_plan;
+typedef struct tfqx_graph tfqx_graph;
+
+/* TfqSimulateExpectation(programs[B], symbol_names[S], symbol_values[B,S],
+ * pauli_sums[B,O]) -> float[B,O]. */
+int tfqx_plan_simulate_expectation(
+    const uint8_t* programs, const int64_t* program_offsets, int64_t batch,
+    const uint8_t* symbol_names, const int64_t* symbol_offsets, int64_t num_symbols,
+    const float* symbol_values,
+    const uint8_t* pauli_sums, const int64_t* pauli_offsets, int64_t num_ops,
+    tfqx_plan** plan, char* err, int64_t err_len);
+
+/* TfqSimulateSampledExpectation(..., num_samples int32[B,O]) -> float[B,O].
+ * The reference seeds from tensorflow::random::New64(); here the seed of the
+ * Philox4x32-10 stream is explicit. */
+int tfqx_plan_simulate_sampled_expectation(
+    const uint8_t* programs, const int64_t* program_offsets, int64_t batch,
+    const uint8_t* symbol_names, const int64_t* symbol_offsets, int64_t num_symbols,
+    const float* symbol_values,
+    const uint8_t* pauli_sums, const int64_t* pauli_offsets, int64_t num_ops,
+    const int32_t* num_samples, uint64_t seed,
+    tfqx_plan** plan, char* err, int64_t err_len);
+
+/* TfqSimulateSamples(programs[B], symbol_names[S], symbol_values[B,S],
+ * num_samples int32[1]) -> int8[B, num_samples, max_nq]. */
+int tfqx_plan_simulate_samples(
+    const uint8_t* programs, const int64_t* program_offsets, int64_t batch,
+    const uint8_t* symbol_names, const int64_t* symbol_offsets, int64_t num_symbols,
+    const float* symbol_values, int32_t num_samples, uint64_t seed,
+    tfqx_plan** plan, char* err, int64_t err_len);
+
+/* TfqSimulateState(programs[B], symbol_names[S], symbol_values[B,S])
+ * -> complex64[B, 2^max_nq]. */
+int tfqx_plan_simulate_state(
+    const uint8_t* programs, const int64_t* program_offsets, int64_t batch,
+    const uint8_t* symbol_names, const int64_t* symbol_offsets, int64_t num_symbols,
+    const float* symbol_values,
+    tfqx_plan** plan, char* err, int64_t err_len);
+
+/* TfqInnerProduct(programs[B], symbol_names[S], symbol_values[B,S],
+ * other_programs[B,K]) -> complex64[B,K]. */
+int tfqx_plan_inner_product(
+    const uint8_t* programs, const int64_t* program_offsets, int64_t batch,
+    const uint8_t* symbol_names, const int64_t* symbol_offsets, int64_t num_symbols,
+    const float* symbol_values,
+    const uint8_t* other_programs, const int64_t* other_offsets, int64_t num_other,
+    tfqx_plan** plan, char* err, int64_t err_len);
+
+/* TfqAdjointGradient(programs[B], symbol_names[S], symbol_values[B,S],
+ * pauli_sums[B,O], downstream_grads[B,O]) -> float[B,S]. */
+int tfqx_plan_adjoint_gradient(
+    const uint8_t* programs, const int64_t* program_offsets, int64_t batch,
+    const uint8_t* symbol_names, const int64_t* symbol_offsets, int64_t num_symbols,
+    const float* symbol_values,
+    const uint8_t* pauli_sums, const int64_t* pauli_offsets, int64_t num_ops,
+    const float* downstream_grads, int32_t derivative,
+    tfqx_plan** plan, char* err, int64_t err_len);
+
+/* Expectations and adjoint gradients of the same inputs from one launch
+ * (no counterpart op in the reference: TFQ evaluates them with two ops,
+ * TfqSimulateExpectation and TfqAdjointGradient).  Output: float32
+ * expectations [batch, num_ops] followed by gradients [batch, num_symbols]. */
+int tfqx_plan_expectation_and_gradient(
+    const uint8_t* programs, const int64_t* program_offsets, int64_t batch,
+    const uint8_t* symbol_names, const int64_t* symbol_offsets, int64_t num_symbols,
+    const float* symbol_values,
+    const uint8_t* pauli_sums, const int64_t* pauli_offsets, int64_t num_ops,
+    const float* downstream_grads, int32_t derivative,
+    tfqx_plan** plan, char* err, int64_t err_len);
+
+/* Plan statistics: out[0] circuits, [1] forward ops, [2] forward HBM passes,
+ * [3] max qubits, [4] adjoint ops, [5] adjoint HBM passes, [6] 1 if the
+ * HBM-streaming path is used, [7] workspace bytes.  No reference counterpart. */
+int tfqx_plan_stats(const tfqx_plan* plan, int64_t* out, int64_t n);
+
+/* Largest qubit count in the batch (sizes the samples / state outputs). */
+int64_t tfqx_plan_max_num_qubits(const tfqx_plan* plan);
+/* Device workspace the plan needs (bytes). */
+int64_t tfqx_plan_workspace_bytes(const tfqx_plan* plan);
+/* Bytes of output the plan writes. */
+int64_t tfqx_plan_output_bytes(const tfqx_plan* plan);
+
+/* Copy the device program into the workspace (host -> device, on `stream`). */
+int tfqx_plan_upload(tfqx_plan* plan, void* workspace, void* stream, char* err, int64_t err_len);
+/* Run the plan's kernels on an uploaded workspace; writes `output`. */
+int tfqx_plan_launch(tfqx_plan* plan, void* workspace, void* output, void* stream,
+                     char* err, int64_t err_len);
+/* Same results, one kernel launch per circuit (the un-batched GPU path). */
+int tfqx_plan_launch_plain(tfqx_plan* plan, void* workspace, void* output, void* stream,
+                           char* err, int64_t err_len);
+/* Capture tfqx_plan_launch into a hipGraph and replay it. */
+int tfqx_graph_capture(tfqx_plan* plan, void* workspace, void* output, void* stream,
+                       tfqx_graph** graph, char* err, int64_t err_len);
+int tfqx_graph_launch(tfqx_graph* graph, void* stream, char* err, int64_t err_len);
+void tfqx_graph_destroy(tfqx_graph* graph);
+void tfqx_plan_destroy(tfqx_plan* plan);
+
+/* Fusion width (qubits per fused block) the planner uses; 0 = autotuned. */
+void tfqx_set_fusion_width(int32_t width);
+const char* tfqx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TFQX_H_ */

@@ -1,0 +1,207 @@
+"""ctypes binding of ``libtfqx.so`` (the C-ABI declared in ``include/tfqx.h``).
+
+The reference loads its native ops with ``load_op_library`` through a lazy
+loader (``core/ops/load_module.py:26-85``).  Here the HIP library is loaded
+lazily on first use; a missing library raises immediately -- there is no CPU
+fallback.
+"""
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+from .serialize import pack_strings
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtfqx.so")
+
+TFQX_OK = 0
+TFQX_INVALID_ARGUMENT = 3
+TFQX_GRAD_TFQ_FD = 0
+TFQX_GRAD_ANALYTIC = 1
+
+_lock = threading.Lock()
+_lib = None
+
+
+class InvalidArgumentError(ValueError):
+    """Raised for invalid op inputs (TF's ``InvalidArgumentError`` in TFQ)."""
+
+
+class DeviceError(RuntimeError):
+    pass
+
+
+def _declare(lib):
+    c = ctypes
+    P = c.c_void_p
+    i64 = c.c_int64
+    i32 = c.c_int32
+    u64 = c.c_uint64
+    s = [P, P, i64, P, P, i64, P]  # programs, names, values
+    lib.tfqx_plan_simulate_expectation.argtypes = s + [P, P, i64, P, c.c_char_p, i64]
+    lib.tfqx_plan_simulate_sampled_expectation.argtypes = s + [P, P, i64, P, u64, P, c.c_char_p, i64]
+    lib.tfqx_plan_simulate_samples.argtypes = s + [i32, u64, P, c.c_char_p, i64]
+    lib.tfqx_plan_simulate_state.argtypes = s + [P, c.c_char_p, i64]
+    lib.tfqx_plan_inner_product.argtypes = s + [P, P, i64, P, c.c_char_p, i64]
+    lib.tfqx_plan_adjoint_gradient.argtypes = s + [P, P, i64, P, i32, P, c.c_char_p, i64]
+    lib.tfqx_plan_expectation_and_gradient.argtypes = s + [P, P, i64, P, i32, P, c.c_char_p, i64]
+    lib.tfqx_plan_stats.argtypes = [P, P, i64]
+    lib.tfqx_plan_stats.restype = c.c_int
+    for name in ("tfqx_plan_simulate_expectation", "tfqx_plan_simulate_sampled_expectation",
+                 "tfqx_plan_simulate_samples", "tfqx_plan_simulate_state",
+                 "tfqx_plan_inner_product", "tfqx_plan_adjoint_gradient",
+                 "tfqx_plan_expectation_and_gradient", "tfqx_plan_upload", "tfqx_plan_launch", "tfqx_plan_launch_plain",
+                 "tfqx_graph_capture", "tfqx_graph_launch"):
+        getattr(lib, name).restype = c.c_int
+    for name in ("tfqx_plan_max_num_qubits", "tfqx_plan_workspace_bytes", "tfqx_plan_output_bytes"):
+        getattr(lib, name).argtypes = [P]
+        getattr(lib, name).restype = i64
+    lib.tfqx_plan_upload.argtypes = [P, P, P, c.c_char_p, i64]
+    lib.tfqx_plan_launch.argtypes = [P, P, P, P, c.c_char_p, i64]
+    lib.tfqx_plan_launch_plain.argtypes = [P, P, P, P, c.c_char_p, i64]
+    lib.tfqx_graph_capture.argtypes = [P, P, P, P, P, c.c_char_p, i64]
+    lib.tfqx_graph_launch.argtypes = [P, P, c.c_char_p, i64]
+    lib.tfqx_graph_destroy.argtypes = [P]
+    lib.tfqx_graph_destroy.restype = None
+    lib.tfqx_plan_destroy.argtypes = [P]
+    lib.tfqx_plan_destroy.restype = None
+    lib.tfqx_set_fusion_width.argtypes = [i32]
+    lib.tfqx_set_fusion_width.restype = None
+    lib.tfqx_version.restype = c.c_char_p
+    return lib
+
+
+def lib():
+    """The loaded library (raises if it has not been built)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise ImportError(f"{LIB_PATH} is not built; run `python -m quantum_amd.build_lib` "
+                                  "(there is no CPU fallback).")
+            _lib = _declare(ctypes.CDLL(LIB_PATH))
+        return _lib
+
+
+EXPORTED_SYMBOLS = (
+    "tfqx_plan_simulate_expectation", "tfqx_plan_simulate_sampled_expectation",
+    "tfqx_plan_simulate_samples", "tfqx_plan_simulate_state", "tfqx_plan_inner_product",
+    "tfqx_plan_adjoint_gradient", "tfqx_plan_expectation_and_gradient", "tfqx_plan_stats",
+    "tfqx_plan_max_num_qubits", "tfqx_plan_workspace_bytes",
+    "tfqx_plan_output_bytes", "tfqx_plan_upload", "tfqx_plan_launch", "tfqx_plan_launch_plain",
+    "tfqx_graph_capture", "tfqx_graph_launch", "tfqx_graph_destroy", "tfqx_plan_destroy",
+    "tfqx_set_fusion_width", "tfqx_version")
+
+
+def check(rc, err):
+    if rc == TFQX_OK:
+        return
+    msg = err.value.decode("utf-8", "replace")
+    if rc == TFQX_INVALID_ARGUMENT:
+        raise InvalidArgumentError(msg)
+    raise DeviceError(f"tfqx error {rc}: {msg}")
+
+
+class StringTensor:
+    """A flattened string tensor kept alive for the duration of a call."""
+
+    def __init__(self, arr):
+        self.bytes, self.offsets = pack_strings(arr)
+        self._buf = ctypes.create_string_buffer(self.bytes, max(1, len(self.bytes)))
+        self.n = len(self.offsets) - 1
+
+    @property
+    def ptr(self):
+        return ctypes.addressof(self._buf)
+
+    @property
+    def off(self):
+        return self.offsets.ctypes.data
+
+
+def np_ptr(a):
+    return a.ctypes.data if a.size else None
+
+
+class Plan:
+    """Owns a ``tfqx_plan*``: host parse/fuse result plus its device workspace."""
+
+    def __init__(self, builder, *args):
+        self._lib = lib()
+        self.handle = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(4096)
+        rc = builder(*args, ctypes.byref(self.handle), err, 4096)
+        check(rc, err)
+        self.workspace = None
+
+    @property
+    def max_num_qubits(self):
+        return int(self._lib.tfqx_plan_max_num_qubits(self.handle))
+
+    def stats(self):
+        out = (ctypes.c_int64 * 8)()
+        self._lib.tfqx_plan_stats(self.handle, out, 8)
+        keys = ("circuits", "forward_ops", "forward_passes", "max_qubits", "adjoint_ops",
+                "adjoint_passes", "streaming", "workspace_bytes")
+        return dict(zip(keys, [int(x) for x in out]))
+
+    @property
+    def workspace_bytes(self):
+        return int(self._lib.tfqx_plan_workspace_bytes(self.handle))
+
+    def upload(self, device, stream):
+        import torch
+        nbytes = max(256, self.workspace_bytes)
+        if self.workspace is None or self.workspace.numel() < nbytes:
+            self.workspace = torch.empty(nbytes, dtype=torch.uint8, device=device)
+        err = ctypes.create_string_buffer(4096)
+        check(self._lib.tfqx_plan_upload(self.handle, ctypes.c_void_p(self.workspace.data_ptr()),
+                                         ctypes.c_void_p(stream), err, 4096), err)
+
+    def launch(self, out, stream, plain=False):
+        err = ctypes.create_string_buffer(4096)
+        fn = self._lib.tfqx_plan_launch_plain if plain else self._lib.tfqx_plan_launch
+        check(fn(self.handle, ctypes.c_void_p(self.workspace.data_ptr()),
+                 ctypes.c_void_p(out.data_ptr() if out.numel() else 0),
+                 ctypes.c_void_p(stream), err, 4096), err)
+
+    def capture_graph(self, out, stream):
+        err = ctypes.create_string_buffer(4096)
+        g = ctypes.c_void_p()
+        check(self._lib.tfqx_graph_capture(self.handle, ctypes.c_void_p(self.workspace.data_ptr()),
+                                           ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream),
+                                           ctypes.byref(g), err, 4096), err)
+        return Graph(self._lib, g)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                self._lib.tfqx_plan_destroy(self.handle)
+                self.handle = ctypes.c_void_p()
+        except Exception:
+            pass
+
+
+class Graph:
+    def __init__(self, lib_, handle):
+        self._lib = lib_
+        self.handle = handle
+
+    def launch(self, stream):
+        err = ctypes.create_string_buffer(4096)
+        check(self._lib.tfqx_graph_launch(self.handle, ctypes.c_void_p(stream), err, 4096), err)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                self._lib.tfqx_graph_destroy(self.handle)
+                self.handle = ctypes.c_void_p()
+        except Exception:
+            pass
+
+
+def set_fusion_width(width):
+    lib().tfqx_set_fusion_width(int(width))

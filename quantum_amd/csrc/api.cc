@@ -1,0 +1,561 @@
+// C-ABI of the op family (include/tfqx.h).
+#include "../../include/tfqx.h"
+
+#include <string.h>
+
+#include <algorithm>
+#include <atomic>
+#include <memory>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+
+#include "batch_kernels.h"
+#include "circuit.h"
+#include "large.h"
+#include "plan.h"
+#include "wire.h"
+
+using namespace tfqx;
+
+namespace {
+
+std::atomic<int> g_fusion_width(0);
+
+void SetErr(char* err, int64_t len, const std::string& msg) {
+  if (!err || len <= 0) return;
+  const size_t n = std::min<size_t>(size_t(len - 1), msg.size());
+  memcpy(err, msg.data(), n);
+  err[n] = '\0';
+}
+
+int Fail(char* err, int64_t len, int code, const std::string& msg) {
+  SetErr(err, len, msg);
+  return code;
+}
+
+int HostThreads() {
+  unsigned n = std::thread::hardware_concurrency();
+  return int(std::max(1u, std::min(n, 32u)));
+}
+
+template <typename F>
+void ParallelFor(int n, F f) {
+  const int T = std::min(HostThreads(), n);
+  if (T <= 1) {
+    for (int i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::atomic<int> next(0);
+  std::vector<std::thread> pool;
+  for (int t = 0; t < T; ++t)
+    pool.emplace_back([&]() {
+      while (true) {
+        int i = next.fetch_add(1);
+        if (i >= n) break;
+        f(i);
+      }
+    });
+  for (auto& th : pool) th.join();
+}
+
+struct FirstError {
+  std::atomic<bool> set{false};
+  Status st;
+  void record(const Status& s) {
+    if (s.ok()) return;
+    bool expected = false;
+    if (set.compare_exchange_strong(expected, true)) st = s;
+  }
+};
+
+std::string Bytes(const uint8_t* b, const int64_t* off, int64_t i) {
+  return std::string(reinterpret_cast<const char*>(b) + off[i], size_t(off[i + 1] - off[i]));
+}
+
+size_t Align(size_t x) { return (x + 255) & ~size_t(255); }
+
+}  // namespace
+
+struct tfqx_plan {
+  int mode = kModeExpectation;
+  int64_t B = 0, S = 0, O = 0, K = 0;
+  int max_nq = 0;
+  int nshots = 0;
+  uint64_t seed = 0;
+  BatchProgram prog;
+  std::vector<int32_t> num_samples;
+  // Workspace layout (bytes).
+  size_t off_counters = 0, off_circuits = 0, off_others = 0, off_ops = 0, off_mats = 0,
+         off_terms = 0, off_ident = 0, off_nsamp = 0, off_passes = 0, off_gacc = 0,
+         off_scratch = 0, total = 0;
+  std::vector<uint8_t> staging;
+  int lds_log2 = 0;
+  int grid = 0;
+  bool with_expectation = false;  // adjoint plan also writes expectations
+  // States wider than one LDS tile run on the HBM-streaming path.
+  bool large = false;
+  std::unique_ptr<LargePlan> large_plan;
+};
+
+struct tfqx_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+struct Inputs {
+  std::vector<Program> programs;
+  std::unordered_map<std::string, int> columns;
+  std::vector<Circuit> circuits;
+  std::vector<std::vector<std::vector<Pauli>>> sums;   // [B][O]
+  std::vector<std::vector<Circuit>> others;            // [B][K]
+};
+
+Status ParsePrograms(const uint8_t* b, const int64_t* off, int64_t n, std::vector<Program>* out) {
+  out->assign(size_t(n), Program());
+  FirstError fe;
+  ParallelFor(int(n), [&](int i) {
+    if (!DecodeProgram(b + off[i], size_t(off[i + 1] - off[i]), &(*out)[i]))
+      fe.record(Status::Invalid("Unparseable proto: " + Bytes(b, off, i)));
+  });
+  return fe.set ? fe.st : Status();
+}
+
+Status Prepare(const uint8_t* programs, const int64_t* poff, int64_t B, const uint8_t* names,
+               const int64_t* noff, int64_t S, const float* values, const uint8_t* psums,
+               const int64_t* psoff, int64_t O, const uint8_t* other, const int64_t* ooff,
+               int64_t K, Inputs* in) {
+  if (B < 0 || S < 0 || O < 0 || K < 0) return Status::Invalid("negative dimension");
+  Status s = ParsePrograms(programs, poff, B, &in->programs);
+  if (!s.ok()) return s;
+  std::vector<PauliSum> ps;
+  if (psums) {
+    ps.assign(size_t(B * O), PauliSum());
+    FirstError fe;
+    ParallelFor(int(B * O), [&](int i) {
+      if (!DecodePauliSum(psums + psoff[i], size_t(psoff[i + 1] - psoff[i]), &ps[i]))
+        fe.record(Status::Invalid("Unparseable proto: " + Bytes(psums, psoff, i)));
+    });
+    if (fe.set) return fe.st;
+  }
+  std::vector<Program> others;
+  if (other) {
+    s = ParsePrograms(other, ooff, B * K, &others);
+    if (!s.ok()) return s;
+  }
+  for (int64_t j = 0; j < S; ++j) in->columns[Bytes(names, noff, j)] = int(j);
+  in->circuits.assign(size_t(B), Circuit());
+  in->sums.assign(size_t(B), {});
+  in->others.assign(size_t(B), {});
+  FirstError fe;
+  ParallelFor(int(B), [&](int i) {
+    SymbolMap sm;
+    sm.columns = &in->columns;
+    sm.values = values + size_t(i) * S;
+    Status st = BuildCircuit(in->programs[i], sm, &in->circuits[i]);
+    if (!st.ok()) return fe.record(st);
+    if (psums) {
+      in->sums[i].resize(size_t(O));
+      for (int64_t j = 0; j < O; ++j) {
+        st = ResolvePauliSum(ps[size_t(i * O + j)], in->circuits[i], &in->sums[i][j]);
+        if (!st.ok()) return fe.record(st);
+      }
+    }
+    if (other) {
+      in->others[i].resize(size_t(K));
+      for (int64_t k = 0; k < K; ++k) {
+        st = BuildPairedCircuit(others[size_t(i * K + k)], in->circuits[i], &in->others[i][k]);
+        if (!st.ok()) return fe.record(st);
+      }
+    }
+  });
+  return fe.set ? fe.st : Status();
+}
+
+int AutoFusionWidth(int max_n) {
+  const int w = g_fusion_width.load();
+  if (w > 0) return std::min(w, kMaxFuse);
+  return max_n <= 6 ? 3 : 2;
+}
+
+void Layout(tfqx_plan* p) {
+  const BatchProgram& g = p->prog;
+  size_t off = 0;
+  p->off_counters = off; off = Align(off + 256);
+  p->off_circuits = off; off = Align(off + g.circuits.size() * sizeof(DevCircuit));
+  p->off_others = off;   off = Align(off + g.others.size() * sizeof(DevOther));
+  p->off_ops = off;      off = Align(off + g.ops.size() * sizeof(DevOp));
+  p->off_mats = off;     off = Align(off + g.mats.size() * sizeof(cfloat));
+  p->off_terms = off;    off = Align(off + g.terms.size() * sizeof(DevTerm));
+  p->off_ident = off;    off = Align(off + g.ident.size() * sizeof(float));
+  p->off_nsamp = off;    off = Align(off + p->num_samples.size() * sizeof(int32_t));
+  p->off_passes = off;   off = Align(off + g.passes.size() * sizeof(DevPass));
+  p->off_gacc = off;
+  const size_t image = off;
+  if (p->mode == kModeAdjoint) off = Align(off + size_t(p->B) * size_t(p->S) * sizeof(double));
+  p->off_scratch = off;
+  if (g.any_tiled) off = Align(off + size_t(kTiledGridCap) * 2 * (size_t(1) << p->max_nq) * sizeof(float2));
+  p->total = off;
+  p->staging.assign(image, 0);
+  auto put = [&](size_t o, const void* src, size_t n) {
+    if (n) memcpy(p->staging.data() + o, src, n);
+  };
+  put(p->off_circuits, g.circuits.data(), g.circuits.size() * sizeof(DevCircuit));
+  put(p->off_others, g.others.data(), g.others.size() * sizeof(DevOther));
+  put(p->off_ops, g.ops.data(), g.ops.size() * sizeof(DevOp));
+  put(p->off_mats, g.mats.data(), g.mats.size() * sizeof(cfloat));
+  put(p->off_terms, g.terms.data(), g.terms.size() * sizeof(DevTerm));
+  put(p->off_ident, g.ident.data(), g.ident.size() * sizeof(float));
+  put(p->off_nsamp, p->num_samples.data(), p->num_samples.size() * sizeof(int32_t));
+  put(p->off_passes, g.passes.data(), g.passes.size() * sizeof(DevPass));
+}
+
+int LdsLimit(int mode) {
+  return BatchBuffers(mode) == 2 ? kLdsMaxQubitsTwoBuffers : kLdsMaxQubitsOneBuffer;
+}
+
+int BuildPlan(int mode, Inputs& in, int64_t B, int64_t S, int64_t O, int64_t K,
+              const float* downstream, bool tfq_fd, tfqx_plan** out, char* err, int64_t len,
+              bool with_expectation = false) {
+  auto p = std::make_unique<tfqx_plan>();
+  p->mode = mode;
+  p->with_expectation = with_expectation;
+  p->B = B;
+  p->S = S;
+  p->O = O;
+  p->K = K;
+  for (const auto& c : in.circuits) p->max_nq = std::max(p->max_nq, c.num_qubits);
+  PlanOptions opt;
+  opt.mode = mode;
+  opt.adjoint = mode == kModeAdjoint;
+  opt.tfq_fd = tfq_fd;
+  opt.fuse.max_fuse = AutoFusionWidth(p->max_nq);
+  const int resident = LdsLimit(mode);
+  const bool tiled_ok = (mode == kModeExpectation || mode == kModeAdjoint || mode == kModeState) &&
+                        p->max_nq <= kTiledMaxQubits;
+  opt.tile_log2 = resident;
+  if (p->max_nq > resident && !tiled_ok) {
+    p->large = true;
+    Status s = BuildLargePlan(mode, in.circuits, in.sums, downstream, in.others, B, S, O, K,
+                              tfq_fd, with_expectation, &p->large_plan);
+    if (!s.ok()) return Fail(err, len, s.code, s.msg);
+    p->total = p->large_plan->workspace_bytes;
+    *out = p.release();
+    return TFQX_OK;
+  }
+  std::vector<CircuitJob> jobs(B);
+  for (int64_t i = 0; i < B; ++i) {
+    jobs[i].circuit = &in.circuits[i];
+    jobs[i].sums = in.sums[i].empty() ? nullptr : &in.sums[i];
+    jobs[i].downstream = downstream ? downstream + size_t(i) * O : nullptr;
+    jobs[i].others = K > 0 ? &in.others[i] : nullptr;
+    jobs[i].row = int(i);
+  }
+  const int slots = mode == kModeInnerProduct ? int(K) : int(O);
+  BuildBatchProgram(jobs, slots, int(S), opt, HostThreads(), &p->prog);
+  p->lds_log2 = std::max(1, std::min(p->max_nq, resident));
+  Layout(p.get());
+  *out = p.release();
+  return TFQX_OK;
+}
+
+hipStream_t AsStream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+BatchArgs MakeArgs(tfqx_plan* p, void* ws, void* output) {
+  uint8_t* base = static_cast<uint8_t*>(ws);
+  BatchArgs a{};
+  a.mode = p->mode;
+  a.circuits = reinterpret_cast<const DevCircuit*>(base + p->off_circuits);
+  a.ncirc = int(p->prog.circuits.size());
+  a.nslots = p->mode == kModeInnerProduct ? int(p->K) : int(p->O);
+  a.nsym = int(p->S);
+  a.lds_log2 = p->lds_log2;
+  a.lds_amps = 1 << p->lds_log2;
+  a.ops = reinterpret_cast<const DevOp*>(base + p->off_ops);
+  a.mats = reinterpret_cast<const float2*>(base + p->off_mats);
+  a.terms = reinterpret_cast<const DevTerm*>(base + p->off_terms);
+  a.others = reinterpret_cast<const DevOther*>(base + p->off_others);
+  a.passes = reinterpret_cast<const DevPass*>(base + p->off_passes);
+  a.scratch = reinterpret_cast<float2*>(base + p->off_scratch);
+  a.scratch_stride = 2ll << p->max_nq;
+  a.ident = reinterpret_cast<const float*>(base + p->off_ident);
+  a.num_samples = reinterpret_cast<const int*>(base + p->off_nsamp);
+  a.grad_acc = reinterpret_cast<double*>(base + p->off_gacc);
+  a.queue = reinterpret_cast<int*>(base + p->off_counters);
+  a.done = reinterpret_cast<int*>(base + p->off_counters + 64);
+  a.seed = p->seed;
+  a.nshots = p->nshots;
+  a.max_nq = p->max_nq;
+  a.state_stride = 1ll << p->max_nq;
+  switch (p->mode) {
+    case kModeExpectation:
+    case kModeSampledExpectation: a.exp_out = static_cast<float*>(output); break;
+    case kModeAdjoint:
+      if (p->with_expectation) {
+        a.exp_out = static_cast<float*>(output);
+        a.grad_out = static_cast<float*>(output) + size_t(p->B) * size_t(p->O);
+      } else {
+        a.grad_out = static_cast<float*>(output);
+      }
+      break;
+    case kModeState: a.state_out = static_cast<float2*>(output); break;
+    case kModeSamples: a.samples_out = static_cast<signed char*>(output); break;
+    default: a.ip_out = static_cast<float2*>(output); break;
+  }
+  // Z-basis rotations: Y^-0.5 and X^0.5 (circuit_parser_qsim.cc:897-945).
+  float yp[5] = {-0.5f, 1.0f, 0.0f, 0, 0}, xp[5] = {0.5f, 1.0f, 0.0f, 0, 0};
+  cdouble Uy[16], Ux[16];
+  GateUnitary(kGateY, yp, Uy);
+  GateUnitary(kGateX, xp, Ux);
+  for (int i = 0; i < 4; ++i) {
+    a.rot[i] = make_float2(float(Uy[i].real()), float(Uy[i].imag()));
+    a.rot[4 + i] = make_float2(float(Ux[i].real()), float(Ux[i].imag()));
+  }
+  return a;
+}
+
+int CheckHip(hipError_t e, char* err, int64_t len, const char* what) {
+  if (e == hipSuccess) return TFQX_OK;
+  return Fail(err, len, TFQX_DEVICE_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* tfqx_version(void) { return "tfqx 0.1 (gfx950)"; }
+
+void tfqx_set_fusion_width(int32_t width) { g_fusion_width.store(width); }
+
+int tfqx_plan_simulate_expectation(const uint8_t* programs, const int64_t* poff, int64_t B,
+                                   const uint8_t* names, const int64_t* noff, int64_t S,
+                                   const float* values, const uint8_t* psums,
+                                   const int64_t* psoff, int64_t O, tfqx_plan** plan, char* err,
+                                   int64_t len) {
+  Inputs in;
+  Status s = Prepare(programs, poff, B, names, noff, S, values, psums, psoff, O, nullptr,
+                     nullptr, 0, &in);
+  if (!s.ok()) return Fail(err, len, s.code, s.msg);
+  return BuildPlan(kModeExpectation, in, B, S, O, 0, nullptr, true, plan, err, len);
+}
+
+int tfqx_plan_simulate_sampled_expectation(const uint8_t* programs, const int64_t* poff,
+                                           int64_t B, const uint8_t* names, const int64_t* noff,
+                                           int64_t S, const float* values, const uint8_t* psums,
+                                           const int64_t* psoff, int64_t O,
+                                           const int32_t* num_samples, uint64_t seed,
+                                           tfqx_plan** plan, char* err, int64_t len) {
+  for (int64_t i = 0; i < B * O; ++i)
+    if (num_samples[i] < 1)
+      return Fail(err, len, TFQX_INVALID_ARGUMENT,
+                  "Each element of num_samples must be greater than 0.");
+  Inputs in;
+  Status s = Prepare(programs, poff, B, names, noff, S, values, psums, psoff, O, nullptr,
+                     nullptr, 0, &in);
+  if (!s.ok()) return Fail(err, len, s.code, s.msg);
+  int rc = BuildPlan(kModeSampledExpectation, in, B, S, O, 0, nullptr, true, plan, err, len);
+  if (rc != TFQX_OK) return rc;
+  (*plan)->seed = seed;
+  (*plan)->num_samples.assign(num_samples, num_samples + B * O);
+  if ((*plan)->large) {
+    (*plan)->large_plan->seed = seed;
+    (*plan)->large_plan->num_samples = (*plan)->num_samples;
+  } else {
+    Layout(*plan);
+  }
+  return TFQX_OK;
+}
+
+int tfqx_plan_simulate_samples(const uint8_t* programs, const int64_t* poff, int64_t B,
+                               const uint8_t* names, const int64_t* noff, int64_t S,
+                               const float* values, int32_t num_samples, uint64_t seed,
+                               tfqx_plan** plan, char* err, int64_t len) {
+  if (num_samples < 0)
+    return Fail(err, len, TFQX_INVALID_ARGUMENT, "num_samples must be non-negative.");
+  Inputs in;
+  Status s = Prepare(programs, poff, B, names, noff, S, values, nullptr, nullptr, 0, nullptr,
+                     nullptr, 0, &in);
+  if (!s.ok()) return Fail(err, len, s.code, s.msg);
+  int rc = BuildPlan(kModeSamples, in, B, S, 0, 0, nullptr, true, plan, err, len);
+  if (rc != TFQX_OK) return rc;
+  (*plan)->seed = seed;
+  (*plan)->nshots = num_samples;
+  if ((*plan)->large) {
+    (*plan)->large_plan->seed = seed;
+    (*plan)->large_plan->nshots = num_samples;
+  }
+  return TFQX_OK;
+}
+
+int tfqx_plan_simulate_state(const uint8_t* programs, const int64_t* poff, int64_t B,
+                             const uint8_t* names, const int64_t* noff, int64_t S,
+                             const float* values, tfqx_plan** plan, char* err, int64_t len) {
+  Inputs in;
+  Status s = Prepare(programs, poff, B, names, noff, S, values, nullptr, nullptr, 0, nullptr,
+                     nullptr, 0, &in);
+  if (!s.ok()) return Fail(err, len, s.code, s.msg);
+  return BuildPlan(kModeState, in, B, S, 0, 0, nullptr, true, plan, err, len);
+}
+
+int tfqx_plan_inner_product(const uint8_t* programs, const int64_t* poff, int64_t B,
+                            const uint8_t* names, const int64_t* noff, int64_t S,
+                            const float* values, const uint8_t* other, const int64_t* ooff,
+                            int64_t K, tfqx_plan** plan, char* err, int64_t len) {
+  Inputs in;
+  Status s = Prepare(programs, poff, B, names, noff, S, values, nullptr, nullptr, 0, other, ooff,
+                     K, &in);
+  if (!s.ok()) return Fail(err, len, s.code, s.msg);
+  return BuildPlan(kModeInnerProduct, in, B, S, 0, K, nullptr, true, plan, err, len);
+}
+
+int tfqx_plan_adjoint_gradient(const uint8_t* programs, const int64_t* poff, int64_t B,
+                               const uint8_t* names, const int64_t* noff, int64_t S,
+                               const float* values, const uint8_t* psums, const int64_t* psoff,
+                               int64_t O, const float* downstream, int32_t derivative,
+                               tfqx_plan** plan, char* err, int64_t len) {
+  Inputs in;
+  Status s = Prepare(programs, poff, B, names, noff, S, values, psums, psoff, O, nullptr,
+                     nullptr, 0, &in);
+  if (!s.ok()) return Fail(err, len, s.code, s.msg);
+  return BuildPlan(kModeAdjoint, in, B, S, O, 0, downstream, derivative == TFQX_GRAD_TFQ_FD,
+                   plan, err, len);
+}
+
+int tfqx_plan_expectation_and_gradient(const uint8_t* programs, const int64_t* poff, int64_t B,
+                                       const uint8_t* names, const int64_t* noff, int64_t S,
+                                       const float* values, const uint8_t* psums,
+                                       const int64_t* psoff, int64_t O, const float* downstream,
+                                       int32_t derivative, tfqx_plan** plan, char* err,
+                                       int64_t len) {
+  Inputs in;
+  Status s = Prepare(programs, poff, B, names, noff, S, values, psums, psoff, O, nullptr,
+                     nullptr, 0, &in);
+  if (!s.ok()) return Fail(err, len, s.code, s.msg);
+  return BuildPlan(kModeAdjoint, in, B, S, O, 0, downstream, derivative == TFQX_GRAD_TFQ_FD,
+                   plan, err, len, /*with_expectation=*/true);
+}
+
+int tfqx_plan_stats(const tfqx_plan* p, int64_t* out, int64_t n) {
+  if (!p || !out) return TFQX_INVALID_ARGUMENT;
+  int64_t v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  v[3] = p->max_nq;
+  if (p->large) {
+    const LargePlan& lp = *p->large_plan;
+    v[0] = int64_t(lp.circuits.size());
+    for (const auto& c : lp.circuits) {
+      v[2] += int64_t(c.fwd.passes.size());
+      v[5] += int64_t(c.rev.passes.size());
+      for (const auto& ps : c.fwd.passes) v[1] += ps.op_end - ps.op_begin;
+      for (const auto& ps : c.rev.passes) v[4] += ps.op_end - ps.op_begin;
+    }
+    v[6] = 1;
+  } else {
+    v[0] = int64_t(p->prog.circuits.size());
+    for (const auto& c : p->prog.circuits) {
+      v[1] += c.fwd_end - c.fwd_begin;
+      v[4] += c.rev_end - c.rev_begin;
+    }
+  }
+  v[7] = int64_t(p->total);
+  for (int64_t i = 0; i < n && i < 8; ++i) out[i] = v[i];
+  return TFQX_OK;
+}
+
+int64_t tfqx_plan_max_num_qubits(const tfqx_plan* p) { return p ? p->max_nq : -1; }
+int64_t tfqx_plan_workspace_bytes(const tfqx_plan* p) { return p ? int64_t(p->total) : -1; }
+
+int64_t tfqx_plan_output_bytes(const tfqx_plan* p) {
+  if (!p) return -1;
+  switch (p->mode) {
+    case kModeExpectation:
+    case kModeSampledExpectation: return p->B * p->O * 4;
+    case kModeAdjoint: return (p->B * p->S + (p->with_expectation ? p->B * p->O : 0)) * 4;
+    case kModeState: return p->B * (int64_t(1) << p->max_nq) * 8;
+    case kModeSamples: return p->B * int64_t(p->nshots) * p->max_nq;
+    default: return p->B * p->K * 8;
+  }
+}
+
+int tfqx_plan_upload(tfqx_plan* p, void* ws, void* stream, char* err, int64_t len) {
+  if (!p || (!ws && p->total)) return Fail(err, len, TFQX_INVALID_ARGUMENT, "null plan or workspace");
+  if (p->large) return LargeUpload(p->large_plan.get(), ws, AsStream(stream), err, len);
+  if (p->staging.empty()) return TFQX_OK;
+  return CheckHip(hipMemcpyAsync(ws, p->staging.data(), p->staging.size(), hipMemcpyHostToDevice,
+                                 AsStream(stream)),
+                  err, len, "hipMemcpyAsync");
+}
+
+int tfqx_plan_launch(tfqx_plan* p, void* ws, void* output, void* stream, char* err, int64_t len) {
+  if (!p) return Fail(err, len, TFQX_INVALID_ARGUMENT, "null plan");
+  if (p->large) return LargeLaunch(p->large_plan.get(), ws, output, AsStream(stream), err, len);
+  if (p->prog.circuits.empty()) return TFQX_OK;
+  if (p->grid == 0) {
+    int maxg = BatchMaxGrid(p->lds_log2, p->mode);
+    if (p->prog.any_tiled) maxg = std::min(maxg, kTiledGridCap);
+    p->grid = std::max(1, std::min<int>(maxg, int(p->prog.circuits.size())));
+  }
+  BatchArgs a = MakeArgs(p, ws, output);
+  return CheckHip(LaunchBatch(a, p->grid, AsStream(stream)), err, len, "batch kernel launch");
+}
+
+int tfqx_plan_launch_plain(tfqx_plan* p, void* ws, void* output, void* stream, char* err,
+                           int64_t len) {
+  if (!p) return Fail(err, len, TFQX_INVALID_ARGUMENT, "null plan");
+  if (p->large) return LargeLaunch(p->large_plan.get(), ws, output, AsStream(stream), err, len);
+  BatchArgs a = MakeArgs(p, ws, output);
+  const DevCircuit* all = a.circuits;
+  for (size_t c = 0; c < p->prog.circuits.size(); ++c) {
+    a.circuits = all + c;
+    a.ncirc = 1;
+    hipError_t e = LaunchBatch(a, 1, AsStream(stream));
+    if (e != hipSuccess) return CheckHip(e, err, len, "plain launch");
+  }
+  return TFQX_OK;
+}
+
+int tfqx_graph_capture(tfqx_plan* p, void* ws, void* output, void* stream, tfqx_graph** g,
+                       char* err, int64_t len) {
+  hipStream_t s = AsStream(stream);
+  // Resolve occupancy and kernel attributes before capture.
+  if (!p->large && p->grid == 0 && !p->prog.circuits.empty()) {
+    int maxg = BatchMaxGrid(p->lds_log2, p->mode);
+    if (p->prog.any_tiled) maxg = std::min(maxg, kTiledGridCap);
+    p->grid = std::max(1, std::min<int>(maxg, int(p->prog.circuits.size())));
+  }
+  auto graph = std::make_unique<tfqx_graph>();
+  int rc = CheckHip(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), err, len,
+                    "hipStreamBeginCapture");
+  if (rc) return rc;
+  rc = tfqx_plan_launch(p, ws, output, stream, err, len);
+  hipGraph_t gr = nullptr;
+  hipError_t e = hipStreamEndCapture(s, &gr);
+  if (rc) return rc;
+  if (e != hipSuccess) return CheckHip(e, err, len, "hipStreamEndCapture");
+  graph->graph = gr;
+  rc = CheckHip(hipGraphInstantiate(&graph->exec, gr, nullptr, nullptr, 0), err, len,
+                "hipGraphInstantiate");
+  if (rc) return rc;
+  *g = graph.release();
+  return TFQX_OK;
+}
+
+int tfqx_graph_launch(tfqx_graph* g, void* stream, char* err, int64_t len) {
+  if (!g) return Fail(err, len, TFQX_INVALID_ARGUMENT, "null graph");
+  return CheckHip(hipGraphLaunch(g->exec, AsStream(stream)), err, len, "hipGraphLaunch");
+}
+
+void tfqx_graph_destroy(tfqx_graph* g) {
+  if (!g) return;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+}
+
+void tfqx_plan_destroy(tfqx_plan* p) { delete p; }
+
+}  // extern "C"

@@ -1,0 +1,419 @@
+// Persistent LDS-resident circuit kernel for gfx950.
+//
+// One workgroup (4 wave64) owns one circuit at a time; circuits are pulled from
+// a device work queue in longest-first order, so thousands of circuits of
+// uneven width and depth run in one launch.  The whole state (and, for the
+// adjoint, lambda) stays in LDS for the full circuit.  Per mode:
+//   expectation  fused forward ops (qsim::ApplyFusedGate,
+//                tfq_simulate_expectation_op.cc:196-231), then a single-pass
+//                XOR-mask reduction per Pauli string with FP64 partials
+//                (replaces Copy + Pauli gates + RealInnerProduct, util_qsim.h:142-188)
+//   adjoint      + lambda = sum_j g_j O_j psi in one pass (AccumulateOperators,
+//                util_qsim.h:363-414) and the reverse sweep of
+//                tfq_adj_grad_op.cc:214-276, with un-apply, derivative and inner
+//                product fused into one LDS sweep per parameterised gate
+//   state        copy-out with -2 padding (tfq_simulate_state_op.cc:152-170)
+//   samples      FP64 in-place prefix scan of |psi|^2 + binary search per shot,
+//                bits unpacked to int8 with -2 padding (tfq_simulate_samples_op.cc:157-188)
+//   sampled exp. per term: Z-basis rotation, scan, shots, parity (util_qsim.h:199-270)
+//   inner prod.  <psi_i | phi_ij> with FP64 accumulation (tfq_inner_product.cc:182-214)
+#include <hip/hip_runtime.h>
+
+#include "batch_kernels.h"
+#include "lds_ops.h"
+
+namespace tfqx {
+
+namespace {
+
+__device__ void run_ops(float2* s, int n, const DevOp* ops, int b, int e, const float2* mats) {
+  for (int i = b; i < e; ++i) {
+    const DevOp op = ops[i];
+    apply_op<false>(s, n, op, mats);
+    __syncthreads();
+  }
+}
+
+__device__ void set_zero_state(float2* s, uint32_t N) {
+  for (uint32_t x = threadIdx.x; x < N; x += kThreads) s[x] = make_float2(x == 0 ? 1.f : 0.f, 0.f);
+  __syncthreads();
+}
+
+// In-place: float2 amplitudes -> FP64 inclusive prefix of |a|^2 (same bytes).
+__device__ double build_cdf(float2* s, uint32_t N, double* red) {
+  double* cdf = reinterpret_cast<double*>(s);
+  const uint32_t chunk = (N + kThreads - 1) / kThreads;
+  const uint32_t b = threadIdx.x * chunk;
+  const uint32_t e = min(N, b + chunk);
+  double run = 0.0;
+  for (uint32_t x = b; x < e; ++x) {
+    const float2 v = s[x];
+    run += double(v.x) * double(v.x) + double(v.y) * double(v.y);
+    cdf[x] = run;
+  }
+  double total;
+  const double off = block_exclusive_scan(run, red, &total);
+  for (uint32_t x = b; x < e; ++x) cdf[x] += off;
+  __syncthreads();
+  return total;
+}
+
+// Smallest x with cdf[x] > target (clamped to N-1).
+__device__ __forceinline__ uint32_t cdf_search(const double* cdf, uint32_t N, double target) {
+  uint32_t lo = 0, hi = N - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (cdf[mid] > target) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+// Pauli expectations of state S (LDS or global) for all slots of a circuit.
+__device__ __forceinline__ void expectations(const float2* __restrict__ S, uint32_t N,
+                                             const BatchArgs& a, const DevCircuit& c, int row,
+                                             double* red) {
+  const int tid = threadIdx.x;
+  int t = uniform(c.term_begin);
+  const int te = uniform(c.term_end);
+  for (int j = 0; j < a.nslots; ++j) {
+    double acc = 0.0;
+    bool any = false;
+    while (t < te && a.terms[t].slot == j) {
+      const DevTerm T = a.terms[t];
+      float part = 0.f;
+      for (uint32_t x = tid; x < N; x += kThreads) {
+        const float2 pr = cmul_conj_a(S[x ^ T.fmask], S[x]);
+        const float v = T.re * pr.x - T.im * pr.y;
+        part += (__popc(x & T.zmask) & 1) ? -v : v;
+      }
+      acc += double(part);
+      ++t;
+      any = true;
+    }
+    const double tot = any ? block_sum(acc, red) : 0.0;
+    if (tid == 0)
+      a.exp_out[size_t(row) * a.nslots + j] = float(tot + double(a.ident[size_t(row) * a.nslots + j]));
+  }
+}
+
+// lambda = sum_t w_t P_t psi.
+__device__ __forceinline__ void build_lambda(const float2* __restrict__ S, float2* __restrict__ Lm,
+                                             uint32_t N, const BatchArgs& a, int lb, int le) {
+  for (uint32_t y = threadIdx.x; y < N; y += kThreads) {
+    float2 l = make_float2(0.f, 0.f);
+    for (int t = lb; t < le; ++t) {
+      const DevTerm T = a.terms[t];
+      const uint32_t x = y ^ T.fmask;
+      float2 w = cmul(make_float2(T.re, T.im), S[x]);
+      if (__popc(x & T.zmask) & 1) w = make_float2(-w.x, -w.y);
+      l = cadd(l, w);
+    }
+    Lm[y] = l;
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void param_step(float2* A, float2* B, int n, const DevOp& op,
+                                           const float2* mats, double* gacc, double* red) {
+  double acc[kMaxDeriv] = {0.0, 0.0};
+  switch (op.k) {
+    case 1: apply_param<1>(A, B, n, op, mats, acc); break;
+    case 2: apply_param<2>(A, B, n, op, mats, acc); break;
+    case 3: apply_param<3>(A, B, n, op, mats, acc); break;
+    default: apply_param<4>(A, B, n, op, mats, acc); break;
+  }
+  for (int d = 0; d < op.nderiv; ++d) {
+    const double tot = block_sum(acc[d], red);
+    if (threadIdx.x == 0) gacc[op.col[d]] += 2.0 * tot;
+  }
+}
+
+// Tiled forward: the state lives in global memory G; each pass streams its
+// tiles through LDS buffer A.  The first pass starts from |0>.
+__device__ void tiled_forward(float2* G, int n, const BatchArgs& a, int pb, int pe, float2* A) {
+  const int tid = threadIdx.x;
+  bool first = true;
+  for (int p = pb; p < pe; ++p) {
+    const DevPass ps = a.passes[p];
+    const uint32_t T = 1u << ps.L;
+    const uint32_t ntiles = 1u << (n - ps.L);
+    for (uint32_t t = 0; t < ntiles; ++t) {
+      const uint64_t base = tile_base(t, ps.tilemask, n);
+      for (uint32_t j = tid; j < T; j += kThreads) {
+        const uint64_t g = tile_index(base, j, ps);
+        A[j] = first ? make_float2(g == 0 ? 1.f : 0.f, 0.f) : G[g];
+      }
+      __syncthreads();
+      for (int i = ps.op_begin; i < ps.op_end; ++i) {
+        const DevOp op = a.ops[i];
+        apply_op<false>(A, ps.L, op, a.mats);
+        __syncthreads();
+      }
+      for (uint32_t j = tid; j < T; j += kThreads) G[tile_index(base, j, ps)] = A[j];
+      __syncthreads();
+    }
+    first = false;
+  }
+  if (first) {
+    for (uint32_t x = tid; x < (1u << n); x += kThreads) G[x] = make_float2(x == 0 ? 1.f : 0.f, 0.f);
+    __syncthreads();
+  }
+}
+
+// Tiled adjoint sweep on psi (G) and lambda (Lg) with LDS tiles A and B.
+__device__ void tiled_reverse(float2* G, float2* Lg, int n, const BatchArgs& a, int pb, int pe,
+                              float2* A, float2* B, double* gacc, double* red) {
+  const int tid = threadIdx.x;
+  for (int p = pb; p < pe; ++p) {
+    const DevPass ps = a.passes[p];
+    const uint32_t T = 1u << ps.L;
+    const uint32_t ntiles = 1u << (n - ps.L);
+    for (uint32_t t = 0; t < ntiles; ++t) {
+      const uint64_t base = tile_base(t, ps.tilemask, n);
+      for (uint32_t j = tid; j < T; j += kThreads) {
+        const uint64_t g = tile_index(base, j, ps);
+        A[j] = G[g];
+        B[j] = Lg[g];
+      }
+      __syncthreads();
+      for (int i = ps.op_begin; i < ps.op_end; ++i) {
+        const DevOp op = a.ops[i];
+        if (op.type == kOpParam) {
+          param_step(A, B, ps.L, op, a.mats, gacc, red);
+        } else {
+          apply_op<true>(A, ps.L, op, a.mats);
+          apply_op<true>(B, ps.L, op, a.mats);
+        }
+        __syncthreads();
+      }
+      for (uint32_t j = tid; j < T; j += kThreads) {
+        const uint64_t g = tile_index(base, j, ps);
+        G[g] = A[j];
+        Lg[g] = B[j];
+      }
+      __syncthreads();
+    }
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void batch_kernel(BatchArgs a) {
+  extern __shared__ float2 lds[];
+  __shared__ double red[kWaves];
+  __shared__ int s_ci;
+  float2* psi = lds;
+  float2* buf2 = lds + a.lds_amps;
+  const int tid = threadIdx.x;
+  float2* gpsi = a.scratch + size_t(blockIdx.x) * a.scratch_stride;
+  float2* glam = gpsi + (a.scratch_stride >> 1);
+
+  while (true) {
+    if (tid == 0) s_ci = atomicAdd(a.queue, 1);
+    __syncthreads();
+    const int ci = uniform(s_ci);
+    __syncthreads();
+    if (ci >= a.ncirc) break;
+    const DevCircuit c = a.circuits[ci];
+    const int n = uniform(c.n);
+    const int row = uniform(c.row);
+    const uint32_t N = 1u << n;
+    const bool empty = (c.flags & kFlagEmpty) != 0;
+    const bool tiled = (c.flags & kFlagTiled) != 0;
+
+    if (empty) {
+      if (MODE == kModeExpectation || MODE == kModeSampledExpectation ||
+          (MODE == kModeAdjoint && a.exp_out))
+        for (int j = tid; j < a.nslots; j += kThreads) a.exp_out[size_t(row) * a.nslots + j] = -2.0f;
+      if (MODE == kModeAdjoint)
+        for (int j = tid; j < a.nsym; j += kThreads) a.grad_out[size_t(row) * a.nsym + j] = 0.0f;
+      if (MODE == kModeInnerProduct)
+        for (int j = tid; j < a.nslots; j += kThreads)
+          a.ip_out[size_t(row) * a.nslots + j] = make_float2(1.f, 0.f);
+      if (MODE == kModeState)
+        for (long long x = tid; x < a.state_stride; x += kThreads)
+          a.state_out[size_t(row) * a.state_stride + x] = make_float2(x == 0 ? 1.f : -2.f, 0.f);
+      if (MODE == kModeSamples)
+        for (long long x = tid; x < (long long)a.nshots * a.max_nq; x += kThreads)
+          a.samples_out[size_t(row) * a.nshots * a.max_nq + x] = -2;
+      continue;
+    }
+
+    if (tiled) {
+      tiled_forward(gpsi, n, a, uniform(c.fwd_pass_begin), uniform(c.fwd_pass_end), psi);
+    } else {
+      set_zero_state(psi, N);
+      run_ops(psi, n, a.ops, uniform(c.fwd_begin), uniform(c.fwd_end), a.mats);
+    }
+
+    if (MODE == kModeExpectation || (MODE == kModeAdjoint && a.exp_out)) {
+      if (tiled) expectations(gpsi, N, a, c, row, red);
+      else expectations(psi, N, a, c, row, red);
+    }
+
+    if (MODE == kModeAdjoint) {
+      double* gacc = a.grad_acc + size_t(row) * a.nsym;
+      for (int j = tid; j < a.nsym; j += kThreads) gacc[j] = 0.0;
+      const int lb = uniform(c.lam_begin), le = uniform(c.lam_end);
+      if (tiled) {
+        build_lambda(gpsi, glam, N, a, lb, le);
+        tiled_reverse(gpsi, glam, n, a, uniform(c.rev_pass_begin), uniform(c.rev_pass_end), psi,
+                      buf2, gacc, red);
+      } else {
+        float2* lam = buf2;
+        build_lambda(psi, lam, N, a, lb, le);
+        for (int i = uniform(c.rev_begin); i < c.rev_end; ++i) {
+          const DevOp op = a.ops[i];
+          if (op.type == kOpParam) {
+            param_step(psi, lam, n, op, a.mats, gacc, red);
+          } else {
+            apply_op<true>(psi, n, op, a.mats);
+            apply_op<true>(lam, n, op, a.mats);
+          }
+          __syncthreads();
+        }
+      }
+      __threadfence_block();
+      __syncthreads();
+      for (int j = tid; j < a.nsym; j += kThreads)
+        a.grad_out[size_t(row) * a.nsym + j] = float(gacc[j]);
+    }
+
+    if (MODE == kModeState) {
+      float2* out = a.state_out + size_t(row) * a.state_stride;
+      const float2* S = tiled ? gpsi : psi;
+      for (long long x = tid; x < a.state_stride; x += kThreads)
+        out[x] = x < (long long)N ? S[x] : make_float2(-2.f, 0.f);
+    }
+
+    if (MODE == kModeSamples) {
+      const double total = build_cdf(psi, N, red);
+      const double* cdf = reinterpret_cast<const double*>(psi);
+      signed char* out = a.samples_out + size_t(row) * a.nshots * a.max_nq;
+      const int mq = a.max_nq;
+      for (int k = tid; k < a.nshots; k += kThreads) {
+        const double u = philox_uniform(a.seed, (unsigned long long)k, uint32_t(row), 0xFFFFFFFFu);
+        const uint32_t x = cdf_search(cdf, N, u * total);
+        signed char* o = out + size_t(k) * mq;
+        for (int q = 0; q < mq; ++q)
+          o[mq - 1 - q] = q < n ? (signed char)((x >> q) & 1u) : (signed char)-2;
+      }
+    }
+
+    if (MODE == kModeSampledExpectation) {
+      int t = uniform(c.term_begin);
+      const int te = uniform(c.term_end);
+      float2* work = buf2;
+      for (int j = 0; j < a.nslots; ++j) {
+        double val = 0.0;
+        const int ns = a.num_samples[size_t(row) * a.nslots + j];
+        while (t < te && a.terms[t].slot == j) {
+          const DevTerm T = a.terms[t];
+          for (uint32_t x = tid; x < N; x += kThreads) work[x] = psi[x];
+          __syncthreads();
+          // Z-basis change: X -> Y^-0.5, Y -> X^0.5 (circuit_parser_qsim.cc:897-945).
+          for (int q = 0; q < n; ++q) {
+            if (!((T.fmask >> q) & 1u)) continue;
+            const bool is_y = (T.zmask >> q) & 1u;
+            apply_1q(work, n, q, is_y ? &a.rot[4] : &a.rot[0]);
+            __syncthreads();
+          }
+          const double total = build_cdf(work, N, red);
+          const double* cdf = reinterpret_cast<const double*>(work);
+          const uint32_t pmask = T.fmask | T.zmask;
+          int par = 0;
+          for (int k = tid; k < ns; k += kThreads) {
+            const double u = philox_uniform(a.seed, (unsigned long long)k, uint32_t(row),
+                                            (uint32_t(j) << 16) | uint32_t(T.pad));
+            const uint32_t x = cdf_search(cdf, N, u * total);
+            par += (__popc(x & pmask) & 1) ? -1 : 1;
+          }
+          const double tot = block_sum(double(par), red);
+          val += double(T.re) * tot / double(ns);
+          ++t;
+        }
+        if (tid == 0)
+          a.exp_out[size_t(row) * a.nslots + j] = float(val + double(a.ident[size_t(row) * a.nslots + j]));
+        __syncthreads();
+      }
+    }
+
+    if (MODE == kModeInnerProduct) {
+      float2* phi = buf2;
+      for (int o = uniform(c.other_begin); o < c.other_end; ++o) {
+        const DevOther ot = a.others[o];
+        set_zero_state(phi, N);
+        run_ops(phi, n, a.ops, uniform(ot.op_begin), uniform(ot.op_end), a.mats);
+        double re = 0.0, im = 0.0;
+        for (uint32_t x = tid; x < N; x += kThreads) {
+          const float2 p = cmul_conj_a(psi[x], phi[x]);
+          re += double(p.x);
+          im += double(p.y);
+        }
+        re = block_sum(re, red);
+        im = block_sum(im, red);
+        if (tid == 0) a.ip_out[size_t(row) * a.nslots + ot.col] = make_float2(float(re), float(im));
+        __syncthreads();
+      }
+    }
+    __syncthreads();
+  }
+
+  // The last workgroup out resets the queue so the launch can be replayed
+  // (e.g. from a hipGraph) without a memset node.
+  if (tid == 0) {
+    __threadfence();
+    const int done = atomicAdd(a.done, 1);
+    if (done == int(gridDim.x) - 1) {
+      atomicExch(a.queue, 0);
+      atomicExch(a.done, 0);
+    }
+  }
+}
+
+template <int MODE>
+const void* KernelPtr() {
+  return reinterpret_cast<const void*>(&batch_kernel<MODE>);
+}
+
+const void* KernelFor(int mode) {
+  switch (mode) {
+    case kModeExpectation: return KernelPtr<kModeExpectation>();
+    case kModeAdjoint: return KernelPtr<kModeAdjoint>();
+    case kModeState: return KernelPtr<kModeState>();
+    case kModeSamples: return KernelPtr<kModeSamples>();
+    case kModeSampledExpectation: return KernelPtr<kModeSampledExpectation>();
+    default: return KernelPtr<kModeInnerProduct>();
+  }
+}
+
+}  // namespace
+
+int BatchBuffers(int mode) {
+  return (mode == kModeAdjoint || mode == kModeSampledExpectation || mode == kModeInnerProduct) ? 2 : 1;
+}
+
+size_t BatchLdsBytes(int lds_log2, int mode) {
+  return size_t(BatchBuffers(mode)) * (size_t(1) << lds_log2) * sizeof(float2);
+}
+
+hipError_t LaunchBatch(const BatchArgs& args, int grid, hipStream_t stream) {
+  // gfx950 workgroups may use up to 160 KiB of LDS without an opt-in
+  // attribute (and setting one above 64 KiB is rejected by the runtime).
+  const size_t lds = BatchLdsBytes(args.lds_log2, args.mode);
+  void* params[] = {const_cast<BatchArgs*>(&args)};
+  return hipLaunchKernel(KernelFor(args.mode), dim3(grid), dim3(kThreads), params, lds, stream);
+}
+
+int BatchMaxGrid(int lds_log2, int mode) {
+  int dev = 0, cus = 0, per_cu = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const size_t lds = BatchLdsBytes(lds_log2, mode);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, KernelFor(mode), kThreads, lds);
+  if (per_cu < 1) per_cu = 1;
+  if (cus < 1) cus = 1;
+  return cus * per_cu;
+}
+
+}  // namespace tfqx

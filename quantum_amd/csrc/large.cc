@@ -1,0 +1,284 @@
+// Host planning of the HBM-streaming path (see large.h).
+#include "large.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace tfqx {
+namespace {
+
+constexpr int kMinLowRun = 6;     // >= 64 contiguous amplitudes (512 B) per tile row
+constexpr int kMaxPassDerivs = 64;
+
+inline int Popc(uint64_t x) { return __builtin_popcountll(x); }
+
+uint64_t OpMask(const FusedOp& f) {
+  uint64_t m = f.cmask;
+  for (int j = 0; j < f.k; ++j) m |= uint64_t(1) << f.bits[j];
+  return m;
+}
+
+uint32_t PushM(std::vector<cfloat>* mats, const std::vector<cdouble>& M, bool diag, int dim) {
+  uint32_t off = uint32_t(mats->size());
+  if (diag) {
+    for (int i = 0; i < dim; ++i)
+      mats->push_back(cfloat(float(M[i * dim + i].real()), float(M[i * dim + i].imag())));
+  } else {
+    for (const auto& x : M) mats->push_back(cfloat(float(x.real()), float(x.imag())));
+  }
+  return off;
+}
+
+DevTermWide WideTerm(const Pauli& p, float w, int slot, bool phase_y, int index) {
+  DevTermWide t{};
+  t.fmask = p.xmask;
+  t.zmask = p.zmask;
+  float re = w, im = 0;
+  for (int k = 0; phase_y && k < (p.ny & 3); ++k) {
+    float nr = -im, ni = re;
+    re = nr;
+    im = ni;
+  }
+  t.re = re;
+  t.im = im;
+  t.slot = slot;
+  t.pad = index;
+  return t;
+}
+
+}  // namespace
+
+void SchedulePasses(const std::vector<FusedOp>& ops, int n, int L, std::vector<DevPass>* passes,
+                    std::vector<DevOp>* dev_ops, std::vector<cfloat>* mats) {
+  L = std::min(L, n);
+  const int low_run = std::min(kMinLowRun, L);
+  const uint64_t lowmask = (uint64_t(1) << low_run) - 1;
+  std::vector<int> remaining(ops.size());
+  for (size_t i = 0; i < ops.size(); ++i) remaining[i] = int(i);
+  while (!remaining.empty()) {
+    uint64_t needed = 0, blocked = 0;
+    int derivs = 0;
+    std::vector<int> chosen, deferred;
+    for (int i : remaining) {
+      const FusedOp& f = ops[i];
+      const uint64_t Q = OpMask(f);
+      const int nd = f.param ? f.nderiv : 0;
+      if (Q & blocked) {
+        deferred.push_back(i);
+        blocked |= Q;
+        continue;
+      }
+      const uint64_t nn = needed | Q;
+      if (Popc(nn | lowmask) <= L && derivs + nd <= kMaxPassDerivs) {
+        chosen.push_back(i);
+        needed = nn;
+        derivs += nd;
+      } else {
+        deferred.push_back(i);
+        blocked |= Q;
+      }
+    }
+    if (chosen.empty()) {
+      // An op wider than the low run allows: give it a tile of its own qubits
+      // (it always fits: fused ops span at most kMaxFuse targets plus controls,
+      // and the planner rejects ops wider than a tile).
+      const int i = deferred.front();
+      chosen.push_back(i);
+      deferred.erase(deferred.begin());
+      needed = OpMask(ops[i]);
+    }
+    uint64_t T = (Popc(needed | lowmask) <= L) ? (needed | lowmask) : needed;
+    for (int b = 0; b < n && Popc(T) < L; ++b) T |= uint64_t(1) << b;
+    DevPass p{};
+    p.L = L;
+    p.tilemask = T;
+    int j = 0;
+    int local[64];
+    for (int b = 0; b < 64; ++b) {
+      local[b] = -1;
+      if (T >> b & 1) {
+        p.tbit[j] = uint8_t(b);
+        local[b] = j++;
+      }
+    }
+    p.nlow = 0;
+    while (p.nlow < L && p.tbit[p.nlow] == p.nlow) p.nlow++;
+    p.op_begin = int(dev_ops->size());
+    for (int i : chosen) {
+      const FusedOp& f = ops[i];
+      DevOp d{};
+      d.k = uint8_t(f.k);
+      for (int q = 0; q < f.k; ++q) d.q[q] = uint8_t(local[f.bits[q]]);
+      for (int b = 0; b < 64; ++b) {
+        if (f.cmask >> b & 1) {
+          d.cmask |= 1u << local[b];
+          if (f.cval >> b & 1) d.cval |= 1u << local[b];
+        }
+      }
+      const int dim = 1 << f.k;
+      if (f.param) {
+        d.type = kOpParam;
+        d.mat = PushM(mats, f.M, false, dim);
+        d.nderiv = uint8_t(f.nderiv);
+        for (int k = 0; k < f.nderiv; ++k) {
+          d.dmat[k] = PushM(mats, f.dM[k], false, dim);
+          d.col[k] = f.col[k];  // partial slot (remapped by the caller)
+        }
+      } else {
+        d.type = f.diag ? kOpDiag : kOpUnitary;
+        d.mat = PushM(mats, f.M, f.diag, dim);
+      }
+      dev_ops->push_back(d);
+    }
+    p.op_end = int(dev_ops->size());
+    passes->push_back(p);
+    remaining.swap(deferred);
+  }
+}
+
+namespace {
+
+int TileQubits(int mode, bool two_buffers) {
+  (void)mode;
+  return two_buffers ? 11 : 12;
+}
+
+void AddPasses(LargePlan* lp, const std::vector<FusedOp>& ops, int n, int L, PassPlan* pp) {
+  for (const auto& f : ops)
+    if (Popc(OpMask(f)) > std::min(L, n)) lp->too_wide = true;
+  if (lp->too_wide) return;
+  std::vector<DevPass> passes;
+  SchedulePasses(ops, n, L, &passes, &lp->ops, &lp->mats);
+  pp->passes = passes;
+}
+
+}  // namespace
+
+Status BuildLargePlan(int mode, const std::vector<Circuit>& circuits,
+                      const std::vector<std::vector<std::vector<Pauli>>>& sums,
+                      const float* downstream, const std::vector<std::vector<Circuit>>& others,
+                      int64_t B, int64_t S, int64_t O, int64_t K, bool tfq_fd,
+                      bool with_expectation, std::unique_ptr<LargePlan>* out) {
+  auto lp = std::make_unique<LargePlan>();
+  lp->mode = mode;
+  lp->with_expectation = with_expectation;
+  lp->B = B;
+  lp->S = S;
+  lp->O = O;
+  lp->K = K;
+  FuseOptions fo;
+  fo.max_fuse = 2;
+  for (const auto& c : circuits) lp->max_nq = std::max(lp->max_nq, c.num_qubits);
+  lp->circuits.resize(size_t(B));
+  for (int64_t i = 0; i < B; ++i) {
+    const Circuit& c = circuits[i];
+    LargeCircuit& lc = lp->circuits[i];
+    lc.n = c.num_qubits;
+    lc.row = int(i);
+    lc.empty = c.gates.empty();
+    lc.ident.assign(size_t(mode == kModeInnerProduct ? K : O), 0.0f);
+    if (lc.empty) continue;
+    const bool two = mode == kModeAdjoint;
+    lc.L = std::min(lc.n, TileQubits(mode, two));
+    std::vector<FusedOp> fused;
+    FuseGates(c, 0, int(c.gates.size()), fo, &fused);
+    AddPasses(lp.get(), fused, lc.n, std::min(lc.n, TileQubits(mode, false)), &lc.fwd);
+    // Terms.
+    lc.term_begin = int(lp->terms.size());
+    const bool sampled = mode == kModeSampledExpectation;
+    if (mode == kModeExpectation || sampled || mode == kModeAdjoint) {
+      for (int64_t j = 0; j < O; ++j) {
+        int idx = 0;
+        for (const auto& p : sums[i][j]) {
+          if (p.ops.empty()) {
+            lc.ident[j] += p.coef;
+          } else if (mode != kModeAdjoint || with_expectation) {
+            lp->terms.push_back(WideTerm(p, p.coef, int(j), !sampled, idx));
+          }
+          ++idx;
+        }
+      }
+    }
+    if (sampled) {
+      // Group terms of each slot by rotation pattern (stable), so terms that
+      // measure in the same basis share one CDF.
+      std::stable_sort(lp->terms.begin() + lc.term_begin, lp->terms.end(),
+                       [](const DevTermWide& a, const DevTermWide& b) {
+                         if (a.slot != b.slot) return a.slot < b.slot;
+                         if (a.fmask != b.fmask) return a.fmask < b.fmask;
+                         return (a.fmask & a.zmask) < (b.fmask & b.zmask);
+                       });
+    }
+    lc.term_end = int(lp->terms.size());
+    if (mode == kModeAdjoint) {
+      lc.lam_begin = int(lp->terms.size());
+      for (int64_t j = 0; j < O; ++j) {
+        const float gj = downstream[i * O + j];
+        for (const auto& p : sums[i][j]) {
+          const float w = gj * p.coef;
+          if (std::fabs(w) < 1e-5f) continue;
+          lp->terms.push_back(WideTerm(p, w, 0, true, 0));
+        }
+      }
+      lc.lam_end = int(lp->terms.size());
+      // Reverse program (as the LDS planner), param ops numbered by partial slot.
+      std::vector<int> pg;
+      for (int g = 0; g < int(c.gates.size()); ++g)
+        if (c.gates[g].nsym > 0) pg.push_back(g);
+      std::vector<FusedOp> rev;
+      int seg_end = int(c.gates.size());
+      for (int p = int(pg.size()); p >= 0; --p) {
+        const int seg_begin = p == 0 ? 0 : pg[p - 1] + 1;
+        std::vector<FusedOp> seg;
+        FuseGates(c, seg_begin, seg_end, fo, &seg);
+        for (int s = int(seg.size()) - 1; s >= 0; --s) rev.push_back(seg[s]);
+        if (p == 0) break;
+        const Gate& g = c.gates[pg[p - 1]];
+        FusedOp f = GateToOp(g, true);
+        f.param = true;
+        f.nderiv = g.nsym;
+        for (int d = 0; d < g.nsym; ++d) {
+          cdouble dU[16];
+          GateDerivative(g.kind, g.params, g.sym_pidx[d], tfq_fd, dU);
+          // Embed the derivative with the same layout as the unitary.
+          Gate dg = g;
+          for (int e = 0; e < 16; ++e) dg.U[e] = dU[e];
+          f.dM[d] = GateToOp(dg, true).M;
+          f.col[d] = g.sym_col[d];  // symbol column; replaced by a partial slot below
+        }
+        rev.push_back(f);
+        seg_end = pg[p - 1];
+      }
+      const size_t op0 = lp->ops.size();
+      AddPasses(lp.get(), rev, lc.n, lc.L, &lc.rev);
+      // Number gradient partial slots in pass order so each pass owns a
+      // contiguous slot range; param_cols maps slot -> symbol column.
+      for (size_t i = op0; i < lp->ops.size(); ++i) {
+        DevOp& d = lp->ops[i];
+        if (d.type != kOpParam) continue;
+        for (int k = 0; k < d.nderiv; ++k) {
+          const int col = d.col[k];
+          d.col[k] = int(lp->param_cols.size());
+          lp->param_cols.push_back(col);
+        }
+      }
+    }
+    if (mode == kModeInnerProduct) {
+      for (int64_t k = 0; k < K; ++k) {
+        const Circuit& oc = others[i][k];
+        std::vector<FusedOp> of;
+        FuseGates(oc, 0, int(oc.gates.size()), fo, &of);
+        PassPlan pp;
+        AddPasses(lp.get(), of, lc.n, std::min(lc.n, TileQubits(mode, false)), &pp);
+        lc.others.push_back(pp);
+      }
+    }
+  }
+  if (lp->too_wide)
+    return Status::Invalid("A gate with its controls acts on more qubits than a state tile holds.");
+  lp->workspace_bytes = LargeWorkspaceBytes(lp.get());
+  *out = std::move(lp);
+  return Status();
+}
+
+}  // namespace tfqx

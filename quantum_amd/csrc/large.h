@@ -1,0 +1,81 @@
+// HBM-streaming path for states wider than one LDS tile.
+//
+// Used when a batch holds a circuit wider than the LDS-resident kernel takes
+// (e.g. 26- and 30-qubit states).  Circuits run one after another, each with
+// the whole GPU: the fused ops are grouped into passes; a pass streams every
+// 2^L-amplitude tile of the state through LDS once (coalesced runs of at
+// least 2^6 contiguous amplitudes) and applies all of its ops there.
+// Reductions (Pauli expectations, lambda, inner products, FP64 CDF for
+// sampling) are separate streaming kernels with FP64 partials.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <vector>
+
+#include "circuit.h"
+#include "plan.h"
+#include "tfqx_internal.h"
+
+namespace tfqx {
+
+struct PassPlan {
+  std::vector<DevPass> passes;
+  int first_param = 0;   // index of this program's first kOpParam in the param table
+};
+
+struct LargeCircuit {
+  int n = 0;
+  int row = 0;
+  bool empty = false;
+  int L = 0;                          // tile qubits
+  PassPlan fwd;                       // forward passes
+  PassPlan rev;                       // adjoint reverse passes (two buffers)
+  std::vector<PassPlan> others;       // inner product paired programs
+  int term_begin = 0, term_end = 0;   // DevTermWide ranges in the plan's term table
+  int lam_begin = 0, lam_end = 0;
+  std::vector<int> rot_groups;        // sampled expectation: term ranges per rotation pattern
+  std::vector<float> ident;           // O identity constants
+};
+
+struct LargePlan {
+  int mode = 0;
+  int64_t B = 0, S = 0, O = 0, K = 0;
+  int max_nq = 0;
+  uint64_t seed = 0;
+  int nshots = 0;
+  std::vector<int32_t> num_samples;
+  std::vector<LargeCircuit> circuits;
+  std::vector<DevOp> ops;
+  std::vector<cfloat> mats;
+  std::vector<DevPass> passes;
+  std::vector<DevTermWide> terms;
+  std::vector<int32_t> param_cols;    // symbol column of every (param op, derivative)
+  // Workspace layout.
+  size_t off_ops = 0, off_mats = 0, off_passes = 0, off_terms = 0, off_partials = 0,
+         off_gacc = 0, off_psi = 0, off_buf2 = 0, off_cdf = 0, off_blocks = 0,
+         workspace_bytes = 0;
+  size_t image_bytes = 0;
+  std::vector<uint8_t> staging;
+  int grid = 0;
+  size_t partial_slots = 0;
+  bool too_wide = false;
+  bool with_expectation = false;  // adjoint mode also writes expectations
+};
+
+Status BuildLargePlan(int mode, const std::vector<Circuit>& circuits,
+                      const std::vector<std::vector<std::vector<Pauli>>>& sums,
+                      const float* downstream, const std::vector<std::vector<Circuit>>& others,
+                      int64_t B, int64_t S, int64_t O, int64_t K, bool tfq_fd,
+                      bool with_expectation, std::unique_ptr<LargePlan>* out);
+
+// Group fused ops into tile passes (exposed for tests of the scheduler).
+void SchedulePasses(const std::vector<FusedOp>& ops, int n, int L, std::vector<DevPass>* passes,
+                    std::vector<DevOp>* dev_ops, std::vector<cfloat>* mats);
+
+size_t LargeWorkspaceBytes(LargePlan* p);
+int LargeUpload(LargePlan* p, void* ws, hipStream_t stream, char* err, int64_t len);
+int LargeLaunch(LargePlan* p, void* ws, void* output, hipStream_t stream, char* err, int64_t len);
+
+}  // namespace tfqx

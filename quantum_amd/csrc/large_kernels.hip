@@ -1,0 +1,682 @@
+// HBM-streaming kernels for states wider than one LDS tile (gfx950).
+//
+//   pass_kernel      one pass of fused ops: every 2^L tile of the state is
+//                    loaded into LDS (runs of >= 64 contiguous amplitudes),
+//                    the pass's ops are applied there, and the tile is stored
+//                    back: 16 bytes of HBM traffic per amplitude per pass.
+//                    The two-buffer variant runs the adjoint reverse sweep on
+//                    psi and lambda together and accumulates FP64 gradient
+//                    partials per workgroup.
+//   expect_kernel    XOR-mask Pauli reduction over the state, FP64 partials.
+//   lambda_kernel    lambda = sum_t w_t P_t psi (one streaming pass).
+//   ip_kernel        <psi|phi> with FP64 partials.
+//   cdf_* / shots_*  FP64 prefix of |psi|^2 in 4096-amplitude blocks plus a
+//                    scan of block sums; each shot draws a Philox uniform and
+//                    binary-searches blocks, then its block.
+#include <hip/hip_runtime.h>
+
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+
+#include "large.h"
+#include "lds_ops.h"
+
+namespace tfqx {
+namespace {
+
+constexpr int kMaxPassDerivs = 64;
+constexpr int kCdfBlockLog = 12;
+
+struct PassArgs {
+  const float2* src;
+  float2* dst;
+  float2* psi;
+  float2* lam;
+  const DevOp* ops;
+  const float2* mats;
+  double* partials;
+  DevPass pass;
+  int n;
+  int init;
+};
+
+template <bool TWO>
+__global__ __launch_bounds__(kThreads) void pass_kernel(PassArgs a) {
+  extern __shared__ float2 lds[];
+  __shared__ double red[kWaves];
+  __shared__ double pacc[kMaxPassDerivs];
+  const int L = a.pass.L;
+  const uint32_t T = 1u << L;
+  float2* A = lds;
+  float2* Bf = lds + T;
+  const int tid = threadIdx.x;
+  const uint64_t ntiles = uint64_t(1) << (a.n - L);
+  if (TWO) {
+    for (int k = tid; k < kMaxPassDerivs; k += kThreads) pacc[k] = 0.0;
+    __syncthreads();
+  }
+  for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const uint64_t base = tile_base(t, a.pass.tilemask, a.n);
+    for (uint32_t j = tid; j < T; j += kThreads) {
+      const uint64_t g = tile_index(base, j, a.pass);
+      if (TWO) {
+        A[j] = a.psi[g];
+        Bf[j] = a.lam[g];
+      } else {
+        A[j] = a.init ? make_float2(g == 0 ? 1.f : 0.f, 0.f) : a.src[g];
+      }
+    }
+    __syncthreads();
+    int pl = 0;
+    for (int i = a.pass.op_begin; i < a.pass.op_end; ++i) {
+      const DevOp op = a.ops[i];
+      if (!TWO) {
+        apply_op<false>(A, L, op, a.mats);
+      } else if (op.type == kOpParam) {
+        double acc[kMaxDeriv] = {0.0, 0.0};
+        switch (op.k) {
+          case 1: apply_param<1>(A, Bf, L, op, a.mats, acc); break;
+          case 2: apply_param<2>(A, Bf, L, op, a.mats, acc); break;
+          case 3: apply_param<3>(A, Bf, L, op, a.mats, acc); break;
+          default: apply_param<4>(A, Bf, L, op, a.mats, acc); break;
+        }
+        for (int d = 0; d < op.nderiv; ++d) {
+          const double tot = block_sum(acc[d], red);
+          if (tid == 0) pacc[pl + d] += tot;
+        }
+        pl += op.nderiv;
+      } else {
+        apply_op<true>(A, L, op, a.mats);
+        apply_op<true>(Bf, L, op, a.mats);
+      }
+      __syncthreads();
+    }
+    for (uint32_t j = tid; j < T; j += kThreads) {
+      const uint64_t g = tile_index(base, j, a.pass);
+      if (TWO) {
+        a.psi[g] = A[j];
+        a.lam[g] = Bf[j];
+      } else {
+        a.dst[g] = A[j];
+      }
+    }
+    __syncthreads();
+  }
+  if (TWO && tid == 0) {
+    int pl = 0;
+    for (int i = a.pass.op_begin; i < a.pass.op_end; ++i) {
+      const DevOp op = a.ops[i];
+      if (op.type != kOpParam) continue;
+      for (int d = 0; d < op.nderiv; ++d)
+        a.partials[size_t(op.col[d]) * gridDim.x + blockIdx.x] = pacc[pl + d];
+      pl += op.nderiv;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void expect_kernel(const float2* __restrict__ psi, int n,
+                                                          const DevTermWide* __restrict__ terms,
+                                                          int tb, int te, double* partials) {
+  __shared__ double red[kWaves];
+  const uint64_t N = uint64_t(1) << n;
+  double acc = 0.0;
+  for (uint64_t x = uint64_t(blockIdx.x) * kThreads + threadIdx.x; x < N;
+       x += uint64_t(gridDim.x) * kThreads) {
+    const float2 va = psi[x];
+    float part = 0.f;
+    for (int t = tb; t < te; ++t) {
+      const DevTermWide T = terms[t];
+      const float2 pr = cmul_conj_a(psi[x ^ T.fmask], va);
+      const float v = T.re * pr.x - T.im * pr.y;
+      part += (__popcll(x & T.zmask) & 1) ? -v : v;
+    }
+    acc += double(part);
+  }
+  const double tot = block_sum(acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kThreads) void lambda_kernel(const float2* __restrict__ psi,
+                                                          float2* __restrict__ lam, int n,
+                                                          const DevTermWide* __restrict__ terms,
+                                                          int tb, int te) {
+  const uint64_t N = uint64_t(1) << n;
+  for (uint64_t y = uint64_t(blockIdx.x) * kThreads + threadIdx.x; y < N;
+       y += uint64_t(gridDim.x) * kThreads) {
+    float2 l = make_float2(0.f, 0.f);
+    for (int t = tb; t < te; ++t) {
+      const DevTermWide T = terms[t];
+      const uint64_t x = y ^ T.fmask;
+      float2 w = cmul(make_float2(T.re, T.im), psi[x]);
+      if (__popcll(x & T.zmask) & 1) w = make_float2(-w.x, -w.y);
+      l = cadd(l, w);
+    }
+    lam[y] = l;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void ip_kernel(const float2* __restrict__ psi,
+                                                      const float2* __restrict__ phi, int n,
+                                                      double* partials) {
+  __shared__ double red[kWaves];
+  const uint64_t N = uint64_t(1) << n;
+  double re = 0.0, im = 0.0;
+  for (uint64_t x = uint64_t(blockIdx.x) * kThreads + threadIdx.x; x < N;
+       x += uint64_t(gridDim.x) * kThreads) {
+    const float2 p = cmul_conj_a(psi[x], phi[x]);
+    re += double(p.x);
+    im += double(p.y);
+  }
+  re = block_sum(re, red);
+  im = block_sum(im, red);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = re;
+    partials[2 * blockIdx.x + 1] = im;
+  }
+}
+
+// out[r] = sum_{i<count} partials[r * row_stride + i * stride], r = blockIdx.x.
+__global__ __launch_bounds__(kThreads) void reduce_kernel(const double* __restrict__ partials,
+                                                          int count, int stride, int row_stride,
+                                                          double* __restrict__ out) {
+  __shared__ double red[kWaves];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < count; i += kThreads)
+    s += partials[size_t(blockIdx.x) * row_stride + size_t(i) * stride];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+// Serial finishing steps (one thread): avoids host round trips between kernels.
+__global__ void finish_expect(const double* sums, float ident, float* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *out = float(sums[0] + double(ident));
+}
+
+__global__ void finish_sampled(const double* sums, int count, const float* coefs,
+                               const int* shots, float ident, float* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double v = double(ident);
+  for (int i = 0; i < count; ++i) v += double(coefs[i]) * sums[i] / double(shots[i]);
+  *out = float(v);
+}
+
+__global__ void finish_ip(const double* sums, float2* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *out = make_float2(float(sums[0]), float(sums[1]));
+}
+
+__global__ void accumulate_grads(const double* sums, const int* cols, int count, double* gacc) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int i = 0; i < count; ++i) gacc[cols[i]] += 2.0 * sums[i];
+}
+
+__global__ void fill_f64(double* p, int64_t n, double v) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void grads_to_float(const double* gacc, int S, float* out) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < S; i += gridDim.x * blockDim.x)
+    out[i] = float(gacc[i]);
+}
+
+__global__ void fill_row_f32(float* out, int count, float v) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
+    out[i] = v;
+}
+
+__global__ void fill_row_c64(float2* out, int64_t count, float2 first, float2 rest) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+       i += int64_t(gridDim.x) * blockDim.x)
+    out[i] = i == 0 ? first : rest;
+}
+
+__global__ void fill_i8(signed char* out, int64_t count, signed char v) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+       i += int64_t(gridDim.x) * blockDim.x)
+    out[i] = v;
+}
+
+__global__ void copy_state(const float2* __restrict__ psi, int n, float2* __restrict__ out,
+                           int64_t stride) {
+  const int64_t N = int64_t(1) << n;
+  for (int64_t x = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; x < stride;
+       x += int64_t(gridDim.x) * blockDim.x)
+    out[x] = x < N ? psi[x] : make_float2(-2.f, 0.f);
+}
+
+__global__ void rot1q_kernel(const float2* __restrict__ src, float2* __restrict__ dst, int n, int q,
+                             float2 m00, float2 m01, float2 m10, float2 m11) {
+  const uint64_t half = uint64_t(1) << (n - 1);
+  for (uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; g < half;
+       g += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t b = ((g >> q) << (q + 1)) | (g & ((uint64_t(1) << q) - 1));
+    const uint64_t b1 = b | (uint64_t(1) << q);
+    const float2 a0 = src[b], a1 = src[b1];
+    dst[b] = cadd(cmul(m00, a0), cmul(m01, a1));
+    dst[b1] = cadd(cmul(m10, a0), cmul(m11, a1));
+  }
+}
+
+// Block-local inclusive FP64 prefix of |s|^2 and block totals.
+__global__ __launch_bounds__(kThreads) void cdf_block_kernel(const float2* __restrict__ s, int n,
+                                                             double* __restrict__ cdf,
+                                                             double* __restrict__ bsum) {
+  __shared__ double red[kWaves];
+  const int blog = min(n, kCdfBlockLog);
+  const uint32_t BLK = 1u << blog;
+  const uint64_t nb = uint64_t(1) << (n - blog);
+  const uint32_t chunk = (BLK + kThreads - 1) / kThreads;
+  for (uint64_t b = blockIdx.x; b < nb; b += gridDim.x) {
+    const uint64_t off = b * BLK;
+    const uint32_t lo = threadIdx.x * chunk, hi = min(BLK, lo + chunk);
+    double run = 0.0;
+    for (uint32_t i = lo; i < hi; ++i) {
+      const float2 v = s[off + i];
+      run += double(v.x) * double(v.x) + double(v.y) * double(v.y);
+    }
+    double total;
+    double pre = block_exclusive_scan(run, red, &total);
+    for (uint32_t i = lo; i < hi; ++i) {
+      const float2 v = s[off + i];
+      pre += double(v.x) * double(v.x) + double(v.y) * double(v.y);
+      cdf[off + i] = pre;
+    }
+    if (threadIdx.x == 0) bsum[b] = total;
+    __syncthreads();
+  }
+}
+
+// Inclusive scan of block totals (single workgroup).
+__global__ __launch_bounds__(kThreads) void cdf_scan_kernel(const double* __restrict__ bsum,
+                                                            int64_t nb, double* __restrict__ bincl) {
+  __shared__ double red[kWaves];
+  const int64_t chunk = (nb + kThreads - 1) / kThreads;
+  const int64_t lo = threadIdx.x * chunk, hi = min(nb, lo + chunk);
+  double run = 0.0;
+  for (int64_t i = lo; i < hi; ++i) run += bsum[i];
+  double total;
+  double pre = block_exclusive_scan(run, red, &total);
+  for (int64_t i = lo; i < hi; ++i) {
+    pre += bsum[i];
+    bincl[i] = pre;
+  }
+}
+
+__device__ __forceinline__ uint64_t sample_index(const double* __restrict__ cdf,
+                                                 const double* __restrict__ bincl, int n,
+                                                 double u) {
+  const int blog = min(n, kCdfBlockLog);
+  const uint64_t nb = uint64_t(1) << (n - blog);
+  const double total = bincl[nb - 1];
+  const double t = u * total;
+  uint64_t lo = 0, hi = nb - 1;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (bincl[mid] > t) hi = mid;
+    else lo = mid + 1;
+  }
+  const uint64_t b = lo;
+  const double t2 = t - (b > 0 ? bincl[b - 1] : 0.0);
+  const double* c = cdf + (b << blog);
+  uint32_t l2 = 0, h2 = (1u << blog) - 1;
+  while (l2 < h2) {
+    const uint32_t mid = (l2 + h2) >> 1;
+    if (c[mid] > t2) h2 = mid;
+    else l2 = mid + 1;
+  }
+  return (b << blog) + l2;
+}
+
+__global__ __launch_bounds__(kThreads) void shots_samples_kernel(
+    const double* __restrict__ cdf, const double* __restrict__ bincl, int n,
+    unsigned long long seed, uint32_t row, int nshots, int max_nq, signed char* __restrict__ out) {
+  for (int k = blockIdx.x * kThreads + threadIdx.x; k < nshots; k += gridDim.x * kThreads) {
+    const double u = philox_uniform(seed, (unsigned long long)k, row, 0xFFFFFFFFu);
+    const uint64_t x = sample_index(cdf, bincl, n, u);
+    signed char* o = out + size_t(k) * max_nq;
+    for (int q = 0; q < max_nq; ++q)
+      o[max_nq - 1 - q] = q < n ? (signed char)((x >> q) & 1u) : (signed char)-2;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void shots_parity_kernel(
+    const double* __restrict__ cdf, const double* __restrict__ bincl, int n,
+    unsigned long long seed, uint32_t s_lo, uint32_t s_hi, int nshots, uint64_t pmask,
+    double* partials) {
+  __shared__ double red[kWaves];
+  int par = 0;
+  for (int k = blockIdx.x * kThreads + threadIdx.x; k < nshots; k += gridDim.x * kThreads) {
+    const double u = philox_uniform(seed, (unsigned long long)k, s_lo, s_hi);
+    const uint64_t x = sample_index(cdf, bincl, n, u);
+    par += (__popcll(x & pmask) & 1) ? -1 : 1;
+  }
+  const double tot = block_sum(double(par), red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+size_t AlignUp(size_t x) { return (x + 255) & ~size_t(255); }
+
+int Err(char* err, int64_t len, const std::string& msg) {
+  if (err && len > 0) {
+    const size_t n = std::min<size_t>(size_t(len - 1), msg.size());
+    memcpy(err, msg.data(), n);
+    err[n] = '\0';
+  }
+  return 14;
+}
+
+// Workgroups of the streaming kernels: 8 per CU of an MI355X (256 CUs).  A
+// constant (not a device query) so plans can be built without a GPU.
+int StreamGrid() { return 2048; }
+
+
+}  // namespace
+
+// Workspace layout: the image (ops, mats, terms, param columns, identity
+// constants, per-term coefficients/shots) followed by scratch and state buffers.
+static void LargeLayout(LargePlan* p, size_t* off_pcols, size_t* off_ident, size_t* off_coefs,
+                        size_t* off_shots, size_t* off_red) {
+  const int grid = StreamGrid();
+  p->grid = grid;
+  size_t off = 0;
+  p->off_ops = off; off = AlignUp(off + p->ops.size() * sizeof(DevOp));
+  p->off_mats = off; off = AlignUp(off + p->mats.size() * sizeof(cfloat));
+  p->off_terms = off; off = AlignUp(off + p->terms.size() * sizeof(DevTermWide));
+  *off_pcols = off; off = AlignUp(off + p->param_cols.size() * sizeof(int32_t));
+  const size_t nslots = size_t(p->mode == kModeInnerProduct ? p->K : p->O);
+  *off_ident = off; off = AlignUp(off + size_t(p->B) * nslots * sizeof(float));
+  *off_coefs = off; off = AlignUp(off + p->terms.size() * sizeof(float));
+  *off_shots = off; off = AlignUp(off + p->terms.size() * sizeof(int32_t));
+  p->image_bytes = off;
+  const size_t slots = std::max<size_t>({p->param_cols.size(), size_t(2), size_t(64)});
+  p->partial_slots = slots;
+  p->off_partials = off; off = AlignUp(off + slots * size_t(grid) * sizeof(double));
+  *off_red = off; off = AlignUp(off + std::max<size_t>(slots, p->terms.size() + 8) * sizeof(double));
+  p->off_gacc = off; off = AlignUp(off + size_t(std::max<int64_t>(p->S, 1)) * sizeof(double));
+  const size_t amps = size_t(1) << p->max_nq;
+  p->off_psi = off; off = AlignUp(off + amps * sizeof(float2));
+  const bool two = p->mode == kModeAdjoint || p->mode == kModeInnerProduct ||
+                   p->mode == kModeSampledExpectation;
+  p->off_buf2 = off;
+  if (two) off = AlignUp(off + amps * sizeof(float2));
+  p->off_cdf = off;
+  const bool sampling = p->mode == kModeSamples || p->mode == kModeSampledExpectation;
+  if (sampling) off = AlignUp(off + amps * sizeof(double));
+  p->off_blocks = off;
+  if (sampling) {
+    const size_t nb = std::max<size_t>(1, amps >> kCdfBlockLog);
+    off = AlignUp(off + 2 * nb * sizeof(double));
+  }
+  p->workspace_bytes = off;
+}
+
+struct LargeOffsets {
+  size_t pcols, ident, coefs, shots, red;
+};
+
+static LargeOffsets Offsets(LargePlan* p) {
+  LargeOffsets o{};
+  LargeLayout(p, &o.pcols, &o.ident, &o.coefs, &o.shots, &o.red);
+  return o;
+}
+
+size_t LargeWorkspaceBytes(LargePlan* p) {
+  Offsets(p);
+  return p->workspace_bytes;
+}
+
+int LargeUpload(LargePlan* p, void* ws, hipStream_t stream, char* err, int64_t len) {
+  LargeOffsets o = Offsets(p);
+  std::vector<uint8_t> img(p->image_bytes, 0);
+  auto put = [&](size_t off, const void* src, size_t n) {
+    if (n) memcpy(img.data() + off, src, n);
+  };
+  put(p->off_ops, p->ops.data(), p->ops.size() * sizeof(DevOp));
+  put(p->off_mats, p->mats.data(), p->mats.size() * sizeof(cfloat));
+  put(p->off_terms, p->terms.data(), p->terms.size() * sizeof(DevTermWide));
+  put(o.pcols, p->param_cols.data(), p->param_cols.size() * sizeof(int32_t));
+  const size_t nslots = size_t(p->mode == kModeInnerProduct ? p->K : p->O);
+  std::vector<float> ident(size_t(p->B) * nslots, 0.0f), coefs(p->terms.size(), 0.0f);
+  std::vector<int32_t> shots(p->terms.size(), 1);
+  for (const auto& lc : p->circuits) {
+    for (size_t j = 0; j < lc.ident.size(); ++j) ident[size_t(lc.row) * nslots + j] = lc.ident[j];
+    for (int t = lc.term_begin; t < lc.term_end; ++t) {
+      coefs[t] = p->terms[t].re;
+      if (!p->num_samples.empty())
+        shots[t] = p->num_samples[size_t(lc.row) * p->O + p->terms[t].slot];
+    }
+  }
+  put(o.ident, ident.data(), ident.size() * sizeof(float));
+  put(o.coefs, coefs.data(), coefs.size() * sizeof(float));
+  put(o.shots, shots.data(), shots.size() * sizeof(int32_t));
+  p->staging.swap(img);
+  hipError_t e = hipMemcpyAsync(ws, p->staging.data(), p->staging.size(), hipMemcpyHostToDevice, stream);
+  if (e != hipSuccess) return Err(err, len, std::string("hipMemcpyAsync: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int LargeLaunch(LargePlan* p, void* ws, void* output, hipStream_t stream, char* err, int64_t len) {
+  LargeOffsets o = Offsets(p);
+  uint8_t* base = static_cast<uint8_t*>(ws);
+  const DevOp* ops = reinterpret_cast<const DevOp*>(base + p->off_ops);
+  const float2* mats = reinterpret_cast<const float2*>(base + p->off_mats);
+  const DevTermWide* terms = reinterpret_cast<const DevTermWide*>(base + p->off_terms);
+  const int* pcols = reinterpret_cast<const int*>(base + o.pcols);
+  const float* coefs = reinterpret_cast<const float*>(base + o.coefs);
+  const int* shots = reinterpret_cast<const int*>(base + o.shots);
+  double* partials = reinterpret_cast<double*>(base + p->off_partials);
+  double* redbuf = reinterpret_cast<double*>(base + o.red);
+  double* gacc = reinterpret_cast<double*>(base + p->off_gacc);
+  float2* psi = reinterpret_cast<float2*>(base + p->off_psi);
+  float2* buf2 = reinterpret_cast<float2*>(base + p->off_buf2);
+  double* cdf = reinterpret_cast<double*>(base + p->off_cdf);
+  const size_t amps_max = size_t(1) << p->max_nq;
+  double* bsum = reinterpret_cast<double*>(base + p->off_blocks);
+  double* bincl = bsum + std::max<size_t>(1, amps_max >> kCdfBlockLog);
+  const int grid = p->grid;
+  const int64_t nslots = p->mode == kModeInnerProduct ? p->K : p->O;
+
+  auto run_passes = [&](const PassPlan& pp, int n, float2* dst, bool init_first) {
+    bool first = init_first;
+    for (const DevPass& ps : pp.passes) {
+      PassArgs a{};
+      a.src = dst;
+      a.dst = dst;
+      a.ops = ops;
+      a.mats = mats;
+      a.pass = ps;
+      a.n = n;
+      a.init = first ? 1 : 0;
+      const uint64_t ntiles = uint64_t(1) << (n - ps.L);
+      const int g = int(std::min<uint64_t>(ntiles, uint64_t(grid)));
+      const size_t lds = (size_t(1) << ps.L) * sizeof(float2);
+      hipLaunchKernelGGL(pass_kernel<false>, dim3(g), dim3(kThreads), lds, stream, a);
+      first = false;
+    }
+    if (first) {  // no ops at all: still need |0>
+      hipLaunchKernelGGL(fill_row_c64, dim3(std::min<int64_t>(grid, ((int64_t(1) << n) + 255) / 256)),
+                         dim3(256), 0, stream, dst, int64_t(1) << n, make_float2(1.f, 0.f),
+                         make_float2(0.f, 0.f));
+    }
+  };
+
+  for (const LargeCircuit& lc : p->circuits) {
+    const int row = lc.row;
+    const int n = lc.n;
+    if (lc.empty) {
+      switch (p->mode) {
+        case kModeExpectation:
+        case kModeSampledExpectation:
+          hipLaunchKernelGGL(fill_row_f32, dim3(1), dim3(256), 0, stream,
+                             static_cast<float*>(output) + size_t(row) * nslots, int(nslots), -2.0f);
+          break;
+        case kModeAdjoint: {
+          float* gb = static_cast<float*>(output);
+          if (p->with_expectation) {
+            hipLaunchKernelGGL(fill_row_f32, dim3(1), dim3(256), 0, stream,
+                               gb + size_t(row) * nslots, int(nslots), -2.0f);
+            gb += size_t(p->B) * size_t(p->O);
+          }
+          hipLaunchKernelGGL(fill_row_f32, dim3(1), dim3(256), 0, stream, gb + size_t(row) * p->S,
+                             int(p->S), 0.0f);
+          break;
+        }
+        case kModeInnerProduct:
+          hipLaunchKernelGGL(fill_row_c64, dim3(1), dim3(256), 0, stream,
+                             static_cast<float2*>(output) + size_t(row) * p->K, int64_t(p->K),
+                             make_float2(1.f, 0.f), make_float2(1.f, 0.f));
+          break;
+        case kModeState:
+          hipLaunchKernelGGL(fill_row_c64, dim3(grid), dim3(256), 0, stream,
+                             static_cast<float2*>(output) + size_t(row) * amps_max, int64_t(amps_max),
+                             make_float2(1.f, 0.f), make_float2(-2.f, 0.f));
+          break;
+        case kModeSamples:
+          hipLaunchKernelGGL(fill_i8, dim3(grid), dim3(256), 0, stream,
+                             static_cast<signed char*>(output) + size_t(row) * p->nshots * p->max_nq,
+                             int64_t(p->nshots) * p->max_nq, (signed char)-2);
+          break;
+      }
+      continue;
+    }
+    const int sgrid = int(std::min<uint64_t>(uint64_t(grid), ((uint64_t(1) << n) + 255) / 256));
+    run_passes(lc.fwd, n, psi, true);
+    const float* ident = reinterpret_cast<const float*>(base + o.ident) + size_t(row) * nslots;
+    if (p->mode == kModeExpectation || (p->mode == kModeAdjoint && p->with_expectation)) {
+      float* exp_base = static_cast<float*>(output);
+      int t = lc.term_begin;
+      for (int64_t j = 0; j < nslots; ++j) {
+        const int tb = t;
+        while (t < lc.term_end && p->terms[t].slot == j) ++t;
+        float* out = exp_base + size_t(row) * nslots + j;
+        if (t == tb) {
+          hipLaunchKernelGGL(fill_row_f32, dim3(1), dim3(64), 0, stream, out, 1, lc.ident[j]);
+          continue;
+        }
+        hipLaunchKernelGGL(expect_kernel, dim3(sgrid), dim3(kThreads), 0, stream, psi, n, terms, tb,
+                           t, partials);
+        hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kThreads), 0, stream, partials, sgrid, 1, 0, redbuf);
+        hipLaunchKernelGGL(finish_expect, dim3(1), dim3(64), 0, stream, redbuf, lc.ident[j], out);
+      }
+    }
+    if (p->mode == kModeAdjoint) {
+      float* grad_base = static_cast<float*>(output) +
+                         (p->with_expectation ? size_t(p->B) * size_t(p->O) : 0);
+      hipLaunchKernelGGL(lambda_kernel, dim3(sgrid), dim3(kThreads), 0, stream, psi, buf2, n, terms,
+                         lc.lam_begin, lc.lam_end);
+      hipLaunchKernelGGL(fill_f64, dim3(1), dim3(256), 0, stream, gacc, int64_t(p->S), 0.0);
+      for (const DevPass& ps : lc.rev.passes) {
+        PassArgs a{};
+        a.psi = psi;
+        a.lam = buf2;
+        a.ops = ops;
+        a.mats = mats;
+        a.pass = ps;
+        a.n = n;
+        a.partials = partials;
+        const uint64_t ntiles = uint64_t(1) << (n - ps.L);
+        const int g = int(std::min<uint64_t>(ntiles, uint64_t(grid)));
+        const size_t lds = 2 * (size_t(1) << ps.L) * sizeof(float2);
+        hipLaunchKernelGGL(pass_kernel<true>, dim3(g), dim3(kThreads), lds, stream, a);
+        // Partial slots of a pass are contiguous [s0, s0 + nd) (numbered at planning).
+        int s0 = -1, nd = 0;
+        for (int i = ps.op_begin; i < ps.op_end; ++i) {
+          const DevOp& op = p->ops[i];
+          if (op.type != kOpParam) continue;
+          if (s0 < 0) s0 = op.col[0];
+          nd += op.nderiv;
+        }
+        if (nd == 0) continue;
+        // partials[slot * g + wg]: row r = slot - s0 -> base + r * g.
+        hipLaunchKernelGGL(reduce_kernel, dim3(nd), dim3(kThreads), 0, stream,
+                           partials + size_t(s0) * g, g, 1, g, redbuf);
+        hipLaunchKernelGGL(accumulate_grads, dim3(1), dim3(64), 0, stream, redbuf, pcols + s0, nd,
+                           gacc);
+      }
+      hipLaunchKernelGGL(grads_to_float, dim3(std::max<int>(1, int((p->S + 255) / 256))), dim3(256), 0,
+                         stream, gacc, int(p->S), grad_base + size_t(row) * p->S);
+    } else if (p->mode == kModeState) {
+      hipLaunchKernelGGL(copy_state, dim3(grid), dim3(256), 0, stream, psi, n,
+                         static_cast<float2*>(output) + size_t(row) * amps_max, int64_t(amps_max));
+    } else if (p->mode == kModeSamples) {
+      const uint64_t nb = uint64_t(1) << (n - std::min(n, kCdfBlockLog));
+      hipLaunchKernelGGL(cdf_block_kernel, dim3(int(std::min<uint64_t>(nb, uint64_t(grid)))),
+                         dim3(kThreads), 0, stream, psi, n, cdf, bsum);
+      hipLaunchKernelGGL(cdf_scan_kernel, dim3(1), dim3(kThreads), 0, stream, bsum, int64_t(nb), bincl);
+      if (p->nshots > 0)
+        hipLaunchKernelGGL(shots_samples_kernel,
+                           dim3(std::max(1, std::min(grid, (p->nshots + kThreads - 1) / kThreads))),
+                           dim3(kThreads), 0, stream, cdf, bincl, n, (unsigned long long)p->seed,
+                           uint32_t(row), p->nshots, p->max_nq,
+                           static_cast<signed char*>(output) + size_t(row) * p->nshots * p->max_nq);
+    } else if (p->mode == kModeSampledExpectation) {
+      // Terms are ordered by slot, then by measurement basis: a CDF is rebuilt
+      // only when the basis changes.  Each term draws its own shots.
+      uint64_t cur_f = ~uint64_t(0), cur_y = ~uint64_t(0);
+      int cur_slot = -1;
+      for (int t = lc.term_begin; t < lc.term_end; ++t) {
+        const DevTermWide& T = p->terms[t];
+        const uint64_t yb = T.fmask & T.zmask;
+        if (T.fmask != cur_f || yb != cur_y || T.slot != cur_slot) {
+          cur_f = T.fmask;
+          cur_y = yb;
+          cur_slot = T.slot;
+          const float2* src = psi;
+          if (T.fmask) {
+            bool first = true;
+            for (int q = 0; q < n; ++q) {
+              if (!((T.fmask >> q) & 1)) continue;
+              float yp[5] = {-0.5f, 1.0f, 0.0f, 0, 0}, xp[5] = {0.5f, 1.0f, 0.0f, 0, 0};
+              cdouble U[16];
+              if ((yb >> q) & 1) GateUnitary(kGateX, xp, U);
+              else GateUnitary(kGateY, yp, U);
+              auto f2 = [&](int i) { return make_float2(float(U[i].real()), float(U[i].imag())); };
+              hipLaunchKernelGGL(rot1q_kernel, dim3(grid), dim3(256), 0, stream, first ? psi : buf2,
+                                 buf2, n, q, f2(0), f2(1), f2(2), f2(3));
+              first = false;
+            }
+            src = buf2;
+          }
+          const uint64_t nb = uint64_t(1) << (n - std::min(n, kCdfBlockLog));
+          hipLaunchKernelGGL(cdf_block_kernel, dim3(int(std::min<uint64_t>(nb, uint64_t(grid)))),
+                             dim3(kThreads), 0, stream, src, n, cdf, bsum);
+          hipLaunchKernelGGL(cdf_scan_kernel, dim3(1), dim3(kThreads), 0, stream, bsum, int64_t(nb),
+                             bincl);
+        }
+        const int ns = p->num_samples[size_t(row) * p->O + T.slot];
+        const int g = std::max(1, std::min(grid, (ns + kThreads - 1) / kThreads));
+        hipLaunchKernelGGL(shots_parity_kernel, dim3(g), dim3(kThreads), 0, stream, cdf, bincl, n,
+                           (unsigned long long)p->seed, uint32_t(row),
+                           (uint32_t(T.slot) << 16) | uint32_t(T.pad), ns, T.fmask | T.zmask,
+                           partials);
+        hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kThreads), 0, stream, partials, g, 1, 0,
+                           redbuf + (t - lc.term_begin));
+      }
+      int t = lc.term_begin;
+      for (int64_t j = 0; j < nslots; ++j) {
+        const int tb = t;
+        while (t < lc.term_end && p->terms[t].slot == j) ++t;
+        float* out = static_cast<float*>(output) + size_t(row) * nslots + j;
+        hipLaunchKernelGGL(finish_sampled, dim3(1), dim3(64), 0, stream,
+                           redbuf + (tb - lc.term_begin), t - tb, coefs + tb, shots + tb,
+                           lc.ident[j], out);
+      }
+    } else if (p->mode == kModeInnerProduct) {
+      for (size_t k = 0; k < lc.others.size(); ++k) {
+        run_passes(lc.others[k], n, buf2, true);
+        hipLaunchKernelGGL(ip_kernel, dim3(sgrid), dim3(kThreads), 0, stream, psi, buf2, n, partials);
+        hipLaunchKernelGGL(reduce_kernel, dim3(2), dim3(kThreads), 0, stream, partials, sgrid, 2, 1, redbuf);
+        hipLaunchKernelGGL(finish_ip, dim3(1), dim3(64), 0, stream, redbuf,
+                           static_cast<float2*>(output) + size_t(row) * p->K + k);
+      }
+    }
+    (void)ident;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return Err(err, len, std::string("large-state kernels: ") + hipGetErrorString(e));
+  return 0;
+}
+
+}  // namespace tfqx

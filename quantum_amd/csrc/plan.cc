@@ -1,0 +1,466 @@
+// Host planner (see plan.h).
+#include "plan.h"
+
+#include "large.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <numeric>
+#include <thread>
+
+namespace tfqx {
+namespace {
+
+inline int Popc(uint64_t x) { return __builtin_popcountll(x); }
+
+// Embed a (controlled) gate given in MSB-first order on `ubits` into the
+// little-endian matrix over ascending `obits`.  All control bits must be in obits.
+void Embed(const cdouble* U, const int* ubits, int uk, uint64_t cmask, uint64_t cval,
+           const int* obits, int ok, cdouble* out) {
+  const int dim = 1 << ok;
+  for (int i = 0; i < dim * dim; ++i) out[i] = 0;
+  int upos[kMaxFuse + 4];  // position of ubits[j] inside obits
+  for (int j = 0; j < uk; ++j)
+    for (int m = 0; m < ok; ++m)
+      if (obits[m] == ubits[j]) upos[j] = m;
+  uint64_t cm_local = 0, cv_local = 0;
+  for (int m = 0; m < ok; ++m) {
+    if (cmask >> obits[m] & 1) {
+      cm_local |= 1u << m;
+      if (cval >> obits[m] & 1) cv_local |= 1u << m;
+    }
+  }
+  const int udim = 1 << uk;
+  for (int c = 0; c < dim; ++c) {
+    if ((c & cm_local) != cv_local) {
+      out[c * dim + c] += 1.0;
+      continue;
+    }
+    int sub = 0;
+    for (int j = 0; j < uk; ++j) sub |= ((c >> upos[j]) & 1) << (uk - 1 - j);
+    for (int r = 0; r < udim; ++r) {
+      int row = c;
+      for (int j = 0; j < uk; ++j) {
+        const int bit = (r >> (uk - 1 - j)) & 1;
+        row = (row & ~(1 << upos[j])) | (bit << upos[j]);
+      }
+      out[row * dim + c] += U[r * udim + sub];
+    }
+  }
+}
+
+void MatMul(const cdouble* A, const cdouble* B, int dim, cdouble* C) {
+  for (int r = 0; r < dim; ++r)
+    for (int c = 0; c < dim; ++c) {
+      cdouble s = 0;
+      for (int k = 0; k < dim; ++k) s += A[r * dim + k] * B[k * dim + c];
+      C[r * dim + c] = s;
+    }
+}
+
+bool IsDiagonal(const std::vector<cdouble>& M, int dim) {
+  for (int r = 0; r < dim; ++r)
+    for (int c = 0; c < dim; ++c)
+      if (r != c && M[r * dim + c] != cdouble(0, 0)) return false;
+  return true;
+}
+
+uint64_t GateMask(const Gate& g) {
+  uint64_t m = g.cmask;
+  for (int j = 0; j < g.nt; ++j) m |= uint64_t(1) << g.bits[j];
+  return m;
+}
+
+void MaskToBits(uint64_t m, int* bits, int* k) {
+  *k = 0;
+  for (int b = 0; b < 64; ++b)
+    if (m >> b & 1) bits[(*k)++] = b;
+}
+
+FusedOp BlockToOp(const Circuit& c, const std::vector<int>& gates, uint64_t qmask) {
+  FusedOp op;
+  MaskToBits(qmask, op.bits, &op.k);
+  const int dim = 1 << op.k;
+  std::vector<cdouble> acc(dim * dim, 0.0), e(dim * dim), tmp(dim * dim);
+  for (int i = 0; i < dim; ++i) acc[i * dim + i] = 1.0;
+  for (int gi : gates) {
+    const Gate& g = c.gates[gi];
+    Embed(g.U, g.bits, g.nt, g.cmask, g.cval, op.bits, op.k, e.data());
+    MatMul(e.data(), acc.data(), dim, tmp.data());
+    acc.swap(tmp);
+  }
+  op.M = std::move(acc);
+  op.diag = IsDiagonal(op.M, dim);
+  return op;
+}
+
+}  // namespace
+
+FusedOp GateToOp(const Gate& g, bool keep_controls_outside) {
+  FusedOp op;
+  uint64_t m = 0;
+  for (int j = 0; j < g.nt; ++j) m |= uint64_t(1) << g.bits[j];
+  if (!keep_controls_outside) m |= g.cmask;
+  MaskToBits(m, op.bits, &op.k);
+  const int dim = 1 << op.k;
+  op.M.assign(dim * dim, 0.0);
+  if (keep_controls_outside) {
+    Embed(g.U, g.bits, g.nt, 0, 0, op.bits, op.k, op.M.data());
+    op.cmask = g.cmask;
+    op.cval = g.cval;
+  } else {
+    Embed(g.U, g.bits, g.nt, g.cmask, g.cval, op.bits, op.k, op.M.data());
+  }
+  op.diag = IsDiagonal(op.M, dim);
+  return op;
+}
+
+void FuseGates(const Circuit& c, int begin, int end, const FuseOptions& opt,
+               std::vector<FusedOp>* out) {
+  struct Block {
+    std::vector<int> gates;
+    uint64_t qmask;
+    int created;
+    bool open;
+  };
+  std::vector<Block> blocks;
+  std::vector<int> owner(64, -1);
+  auto close = [&](int bi) {
+    Block& b = blocks[bi];
+    if (!b.open) return;
+    b.open = false;
+    for (int q = 0; q < 64; ++q)
+      if (b.qmask >> q & 1) owner[q] = -1;
+    out->push_back(BlockToOp(c, b.gates, b.qmask));
+  };
+  for (int gi = begin; gi < end; ++gi) {
+    const Gate& g = c.gates[gi];
+    const uint64_t Q = GateMask(g);
+    std::vector<int> touching;
+    for (int q = 0; q < 64; ++q)
+      if ((Q >> q & 1) && owner[q] >= 0 &&
+          std::find(touching.begin(), touching.end(), owner[q]) == touching.end())
+        touching.push_back(owner[q]);
+    std::sort(touching.begin(), touching.end());
+    if (Popc(Q) > opt.max_fuse) {
+      // Unfusable: flush what it touches, emit alone with controls outside.
+      for (int bi : touching) close(bi);
+      out->push_back(GateToOp(g, true));
+      continue;
+    }
+    uint64_t U = Q;
+    for (int bi : touching) U |= blocks[bi].qmask;
+    if (Popc(U) <= opt.max_fuse) {
+      int keep;
+      if (touching.empty()) {
+        blocks.push_back(Block{{}, 0, int(blocks.size()), true});
+        keep = int(blocks.size()) - 1;
+      } else {
+        keep = touching[0];
+        for (size_t t = 1; t < touching.size(); ++t) {
+          Block& o = blocks[touching[t]];
+          blocks[keep].gates.insert(blocks[keep].gates.end(), o.gates.begin(), o.gates.end());
+          blocks[keep].qmask |= o.qmask;
+          o.open = false;
+          o.gates.clear();
+        }
+      }
+      blocks[keep].gates.push_back(gi);
+      blocks[keep].qmask |= Q;
+      for (int q = 0; q < 64; ++q)
+        if (blocks[keep].qmask >> q & 1) owner[q] = keep;
+    } else {
+      for (int bi : touching) close(bi);
+      blocks.push_back(Block{{gi}, Q, int(blocks.size()), true});
+      for (int q = 0; q < 64; ++q)
+        if (Q >> q & 1) owner[q] = int(blocks.size()) - 1;
+    }
+  }
+  for (size_t bi = 0; bi < blocks.size(); ++bi) close(int(bi));
+}
+
+namespace {
+
+// One circuit's program, with offsets relative to its own arrays.
+struct Piece {
+  DevCircuit dc;
+  std::vector<DevOther> others;
+  std::vector<DevPass> passes;
+  std::vector<DevOp> ops;
+  std::vector<cfloat> mats;
+  std::vector<DevTerm> terms;
+  std::vector<float> ident;  // O entries
+  double cost = 0;
+};
+
+uint32_t PushMatrix(std::vector<cfloat>* mats, const std::vector<cdouble>& M, bool diag_only,
+                    int dim) {
+  uint32_t off = uint32_t(mats->size());
+  if (diag_only) {
+    for (int i = 0; i < dim; ++i) mats->push_back(cfloat(float(M[i * dim + i].real()),
+                                                         float(M[i * dim + i].imag())));
+  } else {
+    for (const auto& x : M) mats->push_back(cfloat(float(x.real()), float(x.imag())));
+  }
+  return off;
+}
+
+DevOp MakeDevOp(const FusedOp& f, std::vector<cfloat>* mats) {
+  DevOp d{};
+  d.k = uint8_t(f.k);
+  for (int j = 0; j < f.k; ++j) d.q[j] = uint8_t(f.bits[j]);
+  d.cmask = uint32_t(f.cmask);
+  d.cval = uint32_t(f.cval);
+  const int dim = 1 << f.k;
+  if (f.param) {
+    d.type = kOpParam;
+    d.mat = PushMatrix(mats, f.M, false, dim);
+    d.nderiv = uint8_t(f.nderiv);
+    for (int i = 0; i < f.nderiv; ++i) {
+      d.dmat[i] = PushMatrix(mats, f.dM[i], false, dim);
+      d.col[i] = f.col[i];
+    }
+  } else {
+    d.type = f.diag ? kOpDiag : kOpUnitary;
+    d.mat = PushMatrix(mats, f.M, f.diag, dim);
+  }
+  return d;
+}
+
+// XOR-mask term with weight w * i^{#Y} (phase_y) or the plain coefficient
+// (sampled expectation, which measures in the rotated Z basis).  `index` is the
+// term's position in its PauliSum (sampled-expectation RNG stream).
+DevTerm MakeTerm(const Pauli& p, float w, int slot, bool phase_y = true, int index = 0) {
+  DevTerm t{};
+  t.fmask = uint32_t(p.xmask);
+  t.zmask = uint32_t(p.zmask);
+  float re = w, im = 0;
+  for (int k = 0; phase_y && k < (p.ny & 3); ++k) {
+    float nr = -im, ni = re;
+    re = nr;
+    im = ni;
+  }
+  t.re = re;
+  t.im = im;
+  t.slot = slot;
+  t.pad = index;
+  return t;
+}
+
+void BuildPiece(const CircuitJob& job, int num_slots, const PlanOptions& opt, Piece* pc) {
+  const Circuit& c = *job.circuit;
+  DevCircuit& dc = pc->dc;
+  dc = DevCircuit{};
+  dc.n = c.num_qubits;
+  dc.row = job.row;
+  pc->ident.assign(num_slots, 0.0f);
+  const bool empty = c.gates.empty();
+  if (empty) dc.flags |= kFlagEmpty;
+  // Forward program.
+  std::vector<FusedOp> fused;
+  if (!empty) FuseGates(c, 0, int(c.gates.size()), opt.fuse, &fused);
+  const bool tiled = !empty && dc.n > opt.tile_log2;
+  if (tiled) dc.flags |= kFlagTiled;
+  dc.fwd_begin = 0;
+  dc.fwd_pass_begin = 0;
+  if (tiled) {
+    SchedulePasses(fused, dc.n, opt.tile_log2, &pc->passes, &pc->ops, &pc->mats);
+  } else {
+    for (const auto& f : fused) pc->ops.push_back(MakeDevOp(f, &pc->mats));
+  }
+  dc.fwd_end = int(pc->ops.size());
+  dc.fwd_pass_end = int(pc->passes.size());
+  // Expectation terms (ComputeExpectationQsim, util_qsim.h:152-186; sampled:
+  // ComputeSampledExpectationQsim, util_qsim.h:199-270).
+  dc.term_begin = 0;
+  const bool sampled = opt.mode == kModeSampledExpectation;
+  if (!empty && job.sums) {
+    for (int j = 0; j < num_slots; ++j) {
+      int idx = 0;
+      for (const auto& p : (*job.sums)[j]) {
+        if (p.ops.empty()) {
+          pc->ident[j] += p.coef;
+        } else {
+          pc->terms.push_back(MakeTerm(p, p.coef, j, !sampled, idx));
+        }
+        ++idx;
+      }
+    }
+  }
+  dc.term_end = int(pc->terms.size());
+  dc.lam_begin = dc.lam_end = dc.term_end;
+  dc.rev_begin = dc.rev_end = dc.fwd_end;
+  double cost = 0;
+  const double amps = std::ldexp(1.0, std::max(dc.n, 0));
+  for (const auto& f : fused) cost += amps * (f.diag ? 1 : (1 << f.k)) + 2048.0;
+  cost += amps * (dc.term_end - dc.term_begin);
+  if (opt.adjoint && !empty) {
+    // Lambda terms (AccumulateOperators, util_qsim.h:363-414): w = g_j * c_t in
+    // float32, terms with |w| < 1e-5 dropped.
+    for (int j = 0; j < num_slots; ++j) {
+      const float gj = job.downstream[j];
+      for (const auto& p : (*job.sums)[j]) {
+        const float w = gj * p.coef;
+        if (std::fabs(w) < 1e-5f) continue;
+        pc->terms.push_back(MakeTerm(p, w, 0));
+      }
+    }
+    dc.lam_end = int(pc->terms.size());
+    cost += amps * (dc.lam_end - dc.lam_begin);
+    // Reverse program: segments between parameterised gates, last first.
+    std::vector<int> pg;
+    for (int i = 0; i < int(c.gates.size()); ++i)
+      if (c.gates[i].nsym > 0) pg.push_back(i);
+    dc.rev_begin = int(pc->ops.size());
+    dc.rev_pass_begin = int(pc->passes.size());
+    std::vector<FusedOp> rev_ops;
+    int seg_end = int(c.gates.size());
+    for (int p = int(pg.size()); p >= 0; --p) {
+      const int seg_begin = p == 0 ? 0 : pg[p - 1] + 1;
+      std::vector<FusedOp> seg;
+      FuseGates(c, seg_begin, seg_end, opt.fuse, &seg);
+      for (int s = int(seg.size()) - 1; s >= 0; --s) {
+        rev_ops.push_back(seg[s]);
+        cost += 2 * amps * (seg[s].diag ? 1 : (1 << seg[s].k)) + 2048.0;
+      }
+      if (p == 0) break;
+      const Gate& g = c.gates[pg[p - 1]];
+      FusedOp f = GateToOp(g, true);
+      f.param = true;
+      f.nderiv = g.nsym;
+      for (int d = 0; d < g.nsym; ++d) {
+        cdouble dU[16];
+        GateDerivative(g.kind, g.params, g.sym_pidx[d], opt.tfq_fd, dU);
+        f.dM[d].assign((1 << f.k) * (1 << f.k), 0.0);
+        Embed(dU, g.bits, g.nt, 0, 0, f.bits, f.k, f.dM[d].data());
+        f.col[d] = g.sym_col[d];
+      }
+      rev_ops.push_back(f);
+      cost += amps * (1 << f.k) * (3 + f.nderiv) + 4096.0;
+      seg_end = pg[p - 1];
+    }
+    if (tiled) {
+      SchedulePasses(rev_ops, dc.n, opt.tile_log2, &pc->passes, &pc->ops, &pc->mats);
+    } else {
+      for (const auto& f : rev_ops) pc->ops.push_back(MakeDevOp(f, &pc->mats));
+    }
+    dc.rev_end = int(pc->ops.size());
+    dc.rev_pass_end = int(pc->passes.size());
+  }
+  if (tiled) cost *= 3.0;  // streams through L2 instead of staying in LDS
+  if (opt.mode == kModeInnerProduct && !empty && job.others) {
+    for (size_t k = 0; k < job.others->size(); ++k) {
+      const Circuit& oc = (*job.others)[k];
+      std::vector<FusedOp> of;
+      FuseGates(oc, 0, int(oc.gates.size()), opt.fuse, &of);
+      DevOther o{};
+      o.op_begin = int(pc->ops.size());
+      for (const auto& f : of) {
+        pc->ops.push_back(MakeDevOp(f, &pc->mats));
+        cost += amps * (f.diag ? 1 : (1 << f.k)) + 2048.0;
+      }
+      o.op_end = int(pc->ops.size());
+      o.col = int(k);
+      pc->others.push_back(o);
+    }
+  }
+  dc.other_begin = 0;
+  dc.other_end = int(pc->others.size());
+  pc->cost = cost;
+}
+
+}  // namespace
+
+double CircuitCost(const DevCircuit& c, const BatchProgram& p) {
+  (void)p;
+  return std::ldexp(1.0, c.n) * double((c.fwd_end - c.fwd_begin) + (c.rev_end - c.rev_begin));
+}
+
+void BuildBatchProgram(const std::vector<CircuitJob>& jobs, int num_slots, int num_symbols,
+                       const PlanOptions& opt, int threads, BatchProgram* out) {
+  const int B = int(jobs.size());
+  std::vector<Piece> pieces(B);
+  std::atomic<int> next(0);
+  auto work = [&]() {
+    while (true) {
+      int i = next.fetch_add(1);
+      if (i >= B) break;
+      BuildPiece(jobs[i], num_slots, opt, &pieces[i]);
+    }
+  };
+  threads = std::max(1, std::min(threads, B));
+  if (threads == 1) {
+    work();
+  } else {
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; ++t) pool.emplace_back(work);
+    for (auto& th : pool) th.join();
+  }
+  // Longest-first launch order (LPT) so the persistent kernel's tail is short.
+  std::vector<int> order(B);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return pieces[a].cost > pieces[b].cost; });
+  out->circuits.clear();
+  out->others.clear();
+  out->passes.clear();
+  out->any_tiled = false;
+  out->ops.clear();
+  out->mats.clear();
+  out->terms.clear();
+  out->ident.assign(size_t(B) * num_slots, 0.0f);
+  out->num_rows = B;
+  out->num_slots = num_slots;
+  out->num_symbols = num_symbols;
+  out->adjoint = opt.adjoint;
+  out->max_n = 0;
+  size_t nops = 0, nmats = 0, nterms = 0;
+  for (const auto& p : pieces) {
+    nops += p.ops.size();
+    nmats += p.mats.size();
+    nterms += p.terms.size();
+  }
+  out->ops.reserve(nops);
+  out->mats.reserve(nmats);
+  out->terms.reserve(nterms);
+  for (int oi : order) {
+    Piece& p = pieces[oi];
+    const int op_base = int(out->ops.size());
+    const uint32_t mat_base = uint32_t(out->mats.size());
+    const int term_base = int(out->terms.size());
+    DevCircuit dc = p.dc;
+    const int pass_base = int(out->passes.size());
+    for (DevPass ps : p.passes) {
+      ps.op_begin += op_base;
+      ps.op_end += op_base;
+      out->passes.push_back(ps);
+    }
+    dc.fwd_pass_begin += pass_base; dc.fwd_pass_end += pass_base;
+    dc.rev_pass_begin += pass_base; dc.rev_pass_end += pass_base;
+    if (dc.flags & kFlagTiled) out->any_tiled = true;
+    const int other_base = int(out->others.size());
+    dc.other_begin += other_base; dc.other_end += other_base;
+    for (DevOther o : p.others) {
+      o.op_begin += op_base;
+      o.op_end += op_base;
+      out->others.push_back(o);
+    }
+    dc.fwd_begin += op_base; dc.fwd_end += op_base;
+    dc.rev_begin += op_base; dc.rev_end += op_base;
+    dc.term_begin += term_base; dc.term_end += term_base;
+    dc.lam_begin += term_base; dc.lam_end += term_base;
+    for (DevOp o : p.ops) {
+      o.mat += mat_base;
+      for (int d = 0; d < o.nderiv; ++d) o.dmat[d] += mat_base;
+      out->ops.push_back(o);
+    }
+    out->mats.insert(out->mats.end(), p.mats.begin(), p.mats.end());
+    out->terms.insert(out->terms.end(), p.terms.begin(), p.terms.end());
+    for (int j = 0; j < num_slots; ++j) out->ident[size_t(dc.row) * num_slots + j] = p.ident[j];
+    out->circuits.push_back(dc);
+    out->max_n = std::max(out->max_n, dc.n);
+  }
+}
+
+}  // namespace tfqx

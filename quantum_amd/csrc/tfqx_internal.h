@@ -1,0 +1,111 @@
+// Internal data structures shared by the host planner and the gfx950 kernels.
+//
+// A circuit reaches the device as a small "program": a list of ops applied to
+// a state tile in LDS.  A tile is 2^L amplitudes; for circuits of L or fewer
+// qubits the whole state is one tile that stays in LDS for the full circuit.
+// Larger states are processed in passes; each pass covers a tile-qubit set and
+// streams every tile of the state through LDS once.
+#pragma once
+
+#include <stdint.h>
+
+namespace tfqx {
+
+// Maximum target qubits of one fused op applied in LDS.
+constexpr int kMaxFuse = 4;
+// Maximum derivative matrices attached to one parameterised gate (FSIM theta+phi,
+// PhasedX / PhasedISwap phase+exponent).
+constexpr int kMaxDeriv = 2;
+
+enum OpType : uint8_t {
+  kOpUnitary = 0,   // apply M (forward) or M^dagger (adjoint sweep) on targets
+  kOpParam = 1,     // adjoint: psi <- G^dag psi; grad += 2Re<lam|dG psi>; lam <- G^dag lam
+  kOpDiag = 2,      // diagonal M: only the 2^k diagonal entries are stored
+};
+
+// One op on a tile.  Qubit positions are tile-local bit indices (ascending).
+// Matrix layout: row-major 2^k x 2^k complex64, index bit j <-> q[j].
+struct alignas(8) DevOp {
+  uint8_t type;
+  uint8_t k;
+  uint8_t nderiv;
+  uint8_t pad0;
+  uint8_t q[kMaxFuse];
+  uint32_t cmask;    // tile-local control bits
+  uint32_t cval;     // required values of the control bits
+  uint32_t mat;      // float2 offset of the unitary (or diagonal)
+  uint32_t dmat[kMaxDeriv];  // float2 offsets of derivative matrices
+  int32_t col[kMaxDeriv];    // symbol column each derivative accumulates into
+};
+static_assert(sizeof(DevOp) == 40, "DevOp layout");
+
+// A Pauli string folded to XOR-mask form: P|x> = ph * (-1)^popc(x & zmask) |x ^ fmask>
+// with ph = coef * i^{#Y}.  slot = output column (expectation) or 0 (lambda).
+struct alignas(8) DevTerm {
+  uint32_t fmask;
+  uint32_t zmask;
+  float re;
+  float im;
+  int32_t slot;
+  int32_t pad;
+};
+static_assert(sizeof(DevTerm) == 24, "DevTerm layout");
+
+// Wide-mask variant for states larger than one tile (masks over all n bits).
+struct alignas(8) DevTermWide {
+  uint64_t fmask;
+  uint64_t zmask;
+  float re;
+  float im;
+  int32_t slot;
+  int32_t pad;
+};
+
+// One pass over a multi-tile state: tile-local bit j <-> global bit tbit[j].
+constexpr int kMaxTile = 14;
+struct alignas(8) DevPass {
+  int32_t op_begin, op_end;
+  int32_t L;          // tile qubits
+  int32_t nlow;       // tile bits 0..nlow-1 are global bits 0..nlow-1
+  uint8_t tbit[kMaxTile + 2];
+  uint64_t tilemask;  // global bits covered by the tile
+};
+
+// One circuit of a batched launch (LDS-resident: n <= L of the launch).
+struct alignas(8) DevCircuit {
+  int32_t n;
+  int32_t row;            // output row
+  int32_t fwd_begin, fwd_end;
+  int32_t term_begin, term_end;   // expectation terms (slot = column)
+  int32_t lam_begin, lam_end;     // adjoint lambda terms (weights folded in)
+  int32_t rev_begin, rev_end;     // adjoint reverse ops
+  int32_t flags;          // bit0: empty program, bit1: tiled
+  int32_t other_begin;    // inner product: range into the `others` table
+  int32_t other_end;
+  // Tiled circuits (wider than the LDS buffer): the state lives in the
+  // workgroup's global scratch and ops run pass by pass on LDS tiles.
+  int32_t fwd_pass_begin, fwd_pass_end;
+  int32_t rev_pass_begin, rev_pass_end;
+};
+
+// Inner product: one paired ("other") program of a row.
+struct alignas(8) DevOther {
+  int32_t op_begin, op_end;
+  int32_t col;            // output column
+  int32_t pad;
+};
+
+// Kernel modes of the persistent batch kernel.
+enum BatchMode : int {
+  kModeExpectation = 0,
+  kModeAdjoint = 1,
+  kModeState = 2,
+  kModeSamples = 3,
+  kModeSampledExpectation = 4,
+  kModeInnerProduct = 5,
+};
+
+constexpr int kFlagEmpty = 1;
+constexpr int kFlagTiled = 2;
+
+}  // namespace tfqx

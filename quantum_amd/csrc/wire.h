@@ -1,0 +1,56 @@
+// proto3 decoding of TFQ's Program and PauliSum messages.
+//
+// Schema: core/proto/program.proto:20-184, core/proto/pauli_sum.proto:20-35.
+// Replaces ParseProto / ParsePrograms / GetPauliSums
+// (core/ops/parse_context.cc:41-56,60-92,253-289) without protobuf: only the
+// fields the simulation path reads are kept.
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace tfqx {
+
+struct Arg {
+  bool has_value = false;   // arg_value set (oneof)
+  bool has_symbol = false;  // symbol set (oneof)
+  float f = 0.0f;           // arg_value.float_value (or double_value narrowed)
+  std::string str;          // arg_value.string_value
+  std::string symbol;
+};
+
+struct Operation {
+  std::string gate_id;
+  std::vector<std::pair<std::string, Arg>> args;
+  std::vector<std::string> qubits;
+  const Arg* find(const char* name) const;
+};
+
+struct Program {
+  std::vector<std::vector<Operation>> moments;
+};
+
+struct PauliPair {
+  std::string qubit_id;
+  std::string pauli_type;
+};
+
+struct PauliTerm {
+  float coef_real = 0.0f;
+  float coef_imag = 0.0f;
+  std::vector<PauliPair> paulis;
+};
+
+struct PauliSum {
+  std::vector<PauliTerm> terms;
+};
+
+// Return false on malformed input.
+bool DecodeProgram(const uint8_t* data, size_t len, Program* out);
+bool DecodePauliSum(const uint8_t* data, size_t len, PauliSum* out);
+
+}  // namespace tfqx

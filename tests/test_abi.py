@@ -1,0 +1,126 @@
+"""C-ABI checks that need no GPU: the library loads, exports every symbol the
+public header declares, and the host side (decode, resolve, fuse, plan) accepts
+valid inputs and rejects invalid ones with the reference's messages."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from quantum_amd import _lib
+from quantum_amd import circuits as C
+from quantum_amd import serialize as S
+from quantum_amd import workloads as W
+from quantum_amd._lib import InvalidArgumentError, Plan, StringTensor, np_ptr
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, "include", "tfqx.h")).read()
+    declared = set(re.findall(r"\b(tfqx_[a-z_0-9]+)\s*\(", header))
+    assert len(declared) >= 18
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    for name in sorted(declared):
+        assert hasattr(lib, name), name
+    assert set(_lib.EXPORTED_SYMBOLS) == declared
+    assert b"gfx950" in _lib.lib().tfqx_version()
+
+
+def _plan_expectation(circuits, names, values, psums):
+    progs = S.convert_to_tensor(circuits)
+    P = StringTensor(progs)
+    N = StringTensor(np.array(names, dtype=object))
+    vals = np.ascontiguousarray(values, dtype=np.float32)
+    ps = S.convert_to_tensor(psums)
+    Ps = StringTensor(ps)
+    return Plan(_lib.lib().tfqx_plan_simulate_expectation, P.ptr, P.off, P.n, N.ptr, N.off, N.n,
+                np_ptr(vals), Ps.ptr, Ps.off, ps.shape[1])
+
+
+def test_plan_config1_on_host():
+    circuits, names, values, ops = W.config1_line_rotations(batch=4)
+    plan = _plan_expectation(circuits, names, values, ops)
+    assert plan.max_num_qubits == 10
+    assert plan.workspace_bytes > 0
+
+
+def test_plan_large_state_on_host():
+    q = C.LineQubit.range(20)
+    circ = C.Circuit([C.H(x) for x in q] + [C.CNOT(q[i], q[i + 1]) for i in range(19)])
+    plan = _plan_expectation([circ], [], np.zeros((1, 0)), [[C.pauli_z(q[0])]])
+    assert plan.max_num_qubits == 20
+    assert plan.workspace_bytes > 8 * 2 ** 20
+
+
+def test_unparseable_proto():
+    q = C.LineQubit(0)
+    P = StringTensor(np.array([b"junk"], dtype=object))
+    N = StringTensor(np.array([], dtype=object))
+    ps = S.convert_to_tensor([[C.pauli_z(q)]])
+    Ps = StringTensor(ps)
+    with pytest.raises(InvalidArgumentError, match="Unparseable proto"):
+        Plan(_lib.lib().tfqx_plan_simulate_expectation, P.ptr, P.off, 1, N.ptr, N.off, 0, None,
+             Ps.ptr, Ps.off, 1)
+
+
+def test_missing_symbol():
+    q = C.LineQubit(0)
+    circ = C.Circuit(C.X(q) ** C.Symbol("alpha"))
+    with pytest.raises(InvalidArgumentError, match="Could not find symbol in parameter map: alpha"):
+        _plan_expectation([circ], ["beta"], [[0.1]], [[C.pauli_z(q)]])
+
+
+def test_pauli_qubit_not_in_circuit():
+    q = C.LineQubit.range(2)
+    circ = C.Circuit(C.X(q[0]))
+    with pytest.raises(InvalidArgumentError,
+                       match="Found a Pauli sum operating on qubits not found in circuit"):
+        _plan_expectation([circ], [], np.zeros((1, 0)), [[C.pauli_z(q[1])]])
+
+
+def test_empty_program_sentinel_plan():
+    q = C.LineQubit(0)
+    plan = _plan_expectation([C.Circuit()], [], np.zeros((1, 0)), [[C.pauli_z(q)]])
+    assert plan.max_num_qubits == 0
+
+
+def test_num_samples_must_be_positive():
+    from quantum_amd import tfq_simulate_ops as ops
+    q = C.LineQubit(0)
+    with pytest.raises(InvalidArgumentError, match="Each element of num_samples must be greater than 0"):
+        progs = S.convert_to_tensor([C.Circuit(C.X(q))])
+        ps = S.convert_to_tensor([[C.pauli_z(q)]])
+        P, N = StringTensor(progs), StringTensor(np.array([], dtype=object))
+        Ps = StringTensor(ps)
+        ns = np.zeros((1, 1), np.int32)
+        Plan(_lib.lib().tfqx_plan_simulate_sampled_expectation, P.ptr, P.off, 1, N.ptr, N.off, 0,
+             None, Ps.ptr, Ps.off, 1, np_ptr(ns), 1)
+
+
+def test_rank_and_batch_checks():
+    from quantum_amd import tfq_simulate_ops as ops
+    q = C.LineQubit(0)
+    progs = S.convert_to_tensor([C.Circuit(C.X(q))])
+    with pytest.raises(InvalidArgumentError, match="symbol_values must be rank 2"):
+        ops._common(progs, [], np.zeros(3))
+    with pytest.raises(InvalidArgumentError, match="do not match"):
+        ops._common(progs, [], np.zeros((2, 0)))
+    with pytest.raises(InvalidArgumentError, match="programs must be rank 1"):
+        ops._common(progs.reshape(1, 1), [], np.zeros((1, 0)))
+
+
+def test_channel_gate_id_rejected():
+    # A gate id outside the 16 the simulator parses (e.g. a noise channel).
+    q = C.LineQubit(0)
+    good = S.serialize_circuit(C.Circuit(C.X(q)))
+    bad = good.replace(b"\n\x02XP", b"\n\x02DP")
+    P = StringTensor(np.array([bad], dtype=object))
+    N = StringTensor(np.array([], dtype=object))
+    ps = S.convert_to_tensor([[C.pauli_z(q)]])
+    Ps = StringTensor(ps)
+    with pytest.raises(InvalidArgumentError, match="Could not parse gate id: DP"):
+        Plan(_lib.lib().tfqx_plan_simulate_expectation, P.ptr, P.off, 1, N.ptr, N.off, 0, None,
+             Ps.ptr, Ps.off, 1)

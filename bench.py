@@ -147,7 +147,11 @@ def main():
 
     assert torch.cuda.is_available(), "needs a HIP device"
     dev = torch.device("cuda", 0)
-    stream = torch.cuda.current_stream(dev).cuda_stream
+    # A dedicated (default-priority) stream: graph capture is not allowed on
+    # the legacy default stream.
+    side = torch.cuda.Stream(device=dev)
+    torch.cuda.set_stream(side)
+    stream = side.cuda_stream
 
     progs, names, values, ps, weights = build_uneven(args.circuits)
     t = time.time()

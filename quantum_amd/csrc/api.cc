@@ -94,6 +94,7 @@ struct tfqx_plan {
          off_scratch = 0, total = 0;
   std::vector<uint8_t> staging;
   int lds_log2 = 0;
+  int kmax = 2;
   int grid = 0;
   bool with_expectation = false;  // adjoint plan also writes expectations
   // States wider than one LDS tile run on the HBM-streaming path.
@@ -180,7 +181,8 @@ Status Prepare(const uint8_t* programs, const int64_t* poff, int64_t B, const ui
 int AutoFusionWidth(int max_n) {
   const int w = g_fusion_width.load();
   if (w > 0) return std::min(w, kMaxFuse);
-  return max_n <= 6 ? 3 : 2;
+  (void)max_n;
+  return 2;
 }
 
 void Layout(tfqx_plan* p) {
@@ -259,6 +261,8 @@ int BuildPlan(int mode, Inputs& in, int64_t B, int64_t S, int64_t O, int64_t K,
   const int slots = mode == kModeInnerProduct ? int(K) : int(O);
   BuildBatchProgram(jobs, slots, int(S), opt, HostThreads(), &p->prog);
   p->lds_log2 = std::max(1, std::min(p->max_nq, resident));
+  p->kmax = 2;
+  for (const auto& o : p->prog.ops) p->kmax = std::max<int>(p->kmax, o.k);
   Layout(p.get());
   *out = p.release();
   return TFQX_OK;
@@ -270,6 +274,7 @@ BatchArgs MakeArgs(tfqx_plan* p, void* ws, void* output) {
   uint8_t* base = static_cast<uint8_t*>(ws);
   BatchArgs a{};
   a.mode = p->mode;
+  a.kmax = p->kmax;
   a.circuits = reinterpret_cast<const DevCircuit*>(base + p->off_circuits);
   a.ncirc = int(p->prog.circuits.size());
   a.nslots = p->mode == kModeInnerProduct ? int(p->K) : int(p->O);
@@ -495,7 +500,7 @@ int tfqx_plan_launch(tfqx_plan* p, void* ws, void* output, void* stream, char* e
   if (p->large) return LargeLaunch(p->large_plan.get(), ws, output, AsStream(stream), err, len);
   if (p->prog.circuits.empty()) return TFQX_OK;
   if (p->grid == 0) {
-    int maxg = BatchMaxGrid(p->lds_log2, p->mode);
+    int maxg = BatchMaxGrid(p->lds_log2, p->mode, p->kmax);
     if (p->prog.any_tiled) maxg = std::min(maxg, kTiledGridCap);
     p->grid = std::max(1, std::min<int>(maxg, int(p->prog.circuits.size())));
   }
@@ -523,7 +528,7 @@ int tfqx_graph_capture(tfqx_plan* p, void* ws, void* output, void* stream, tfqx_
   hipStream_t s = AsStream(stream);
   // Resolve occupancy and kernel attributes before capture.
   if (!p->large && p->grid == 0 && !p->prog.circuits.empty()) {
-    int maxg = BatchMaxGrid(p->lds_log2, p->mode);
+    int maxg = BatchMaxGrid(p->lds_log2, p->mode, p->kmax);
     if (p->prog.any_tiled) maxg = std::min(maxg, kTiledGridCap);
     p->grid = std::max(1, std::min<int>(maxg, int(p->prog.circuits.size())));
   }

@@ -9,6 +9,7 @@ namespace tfqx {
 
 struct BatchArgs {
   int mode;            // BatchMode
+  int kmax;            // widest fused op in the program (selects the kernel variant)
   const DevCircuit* circuits;
   int ncirc;
   int nslots;          // O: expectation columns (K for inner products)
@@ -54,6 +55,6 @@ constexpr int kTiledGridCap = 512;
 int BatchBuffers(int mode);
 size_t BatchLdsBytes(int lds_log2, int mode);
 hipError_t LaunchBatch(const BatchArgs& args, int grid, hipStream_t stream);
-int BatchMaxGrid(int lds_log2, int mode);
+int BatchMaxGrid(int lds_log2, int mode, int kmax);
 
 }  // namespace tfqx

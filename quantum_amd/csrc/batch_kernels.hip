@@ -26,12 +26,11 @@ namespace tfqx {
 
 namespace {
 
-__device__ void run_ops(float2* s, int n, const DevOp* ops, int b, int e, const float2* mats) {
-  for (int i = b; i < e; ++i) {
-    const DevOp op = ops[i];
-    apply_op<false>(s, n, op, mats);
-    __syncthreads();
-  }
+template <int KMAX>
+__device__ void run_ops(float2* s, int n, const DevOp* ops, int b, int e, const float2* mats,
+                        float2* mbuf) {
+  for_ops(ops, b, e, mats, mbuf,
+          [&](const DevOp& op, const float2* M) { apply_op<false, KMAX>(s, n, op, M); });
 }
 
 __device__ void set_zero_state(float2* s, uint32_t N) {
@@ -115,15 +114,15 @@ __device__ __forceinline__ void build_lambda(const float2* __restrict__ S, float
 }
 
 __device__ __forceinline__ void param_step(float2* A, float2* B, int n, const DevOp& op,
-                                           const float2* mats, double* gacc, double* red) {
+                                           const float2* M, double* gacc, double* red) {
   double acc[kMaxDeriv] = {0.0, 0.0};
   switch (op.k) {
-    case 1: apply_param<1>(A, B, n, op, mats, acc); break;
-    case 2: apply_param<2>(A, B, n, op, mats, acc); break;
-    case 3: apply_param<3>(A, B, n, op, mats, acc); break;
-    default: apply_param<4>(A, B, n, op, mats, acc); break;
+    case 1: apply_param<1>(A, B, n, op, M, acc); break;
+    default: apply_param<2>(A, B, n, op, M, acc); break;
   }
-  for (int d = 0; d < op.nderiv; ++d) {
+#pragma unroll
+  for (int d = 0; d < kMaxDeriv; ++d) {
+    if (d >= op.nderiv) break;
     const double tot = block_sum(acc[d], red);
     if (threadIdx.x == 0) gacc[op.col[d]] += 2.0 * tot;
   }
@@ -131,26 +130,29 @@ __device__ __forceinline__ void param_step(float2* A, float2* B, int n, const De
 
 // Tiled forward: the state lives in global memory G; each pass streams its
 // tiles through LDS buffer A.  The first pass starts from |0>.
-__device__ void tiled_forward(float2* G, int n, const BatchArgs& a, int pb, int pe, float2* A) {
+template <int KMAX>
+__device__ void tiled_forward(float2* G, int n, const BatchArgs& a, int pb, int pe, float2* A,
+                              float2* mbuf, uint64_t* tab) {
   const int tid = threadIdx.x;
   bool first = true;
   for (int p = pb; p < pe; ++p) {
     const DevPass ps = a.passes[p];
+    const bool use_tab = ps.L - ps.nlow <= 8;
+    build_tile_table(ps, tab);
+    __syncthreads();
     const uint32_t T = 1u << ps.L;
     const uint32_t ntiles = 1u << (n - ps.L);
     for (uint32_t t = 0; t < ntiles; ++t) {
       const uint64_t base = tile_base(t, ps.tilemask, n);
       for (uint32_t j = tid; j < T; j += kThreads) {
-        const uint64_t g = tile_index(base, j, ps);
+        const uint64_t g = use_tab ? tile_index_tab(base, j, ps.nlow, tab) : tile_index(base, j, ps);
         A[j] = first ? make_float2(g == 0 ? 1.f : 0.f, 0.f) : G[g];
       }
       __syncthreads();
-      for (int i = ps.op_begin; i < ps.op_end; ++i) {
-        const DevOp op = a.ops[i];
-        apply_op<false>(A, ps.L, op, a.mats);
-        __syncthreads();
-      }
-      for (uint32_t j = tid; j < T; j += kThreads) G[tile_index(base, j, ps)] = A[j];
+      for_ops(a.ops, ps.op_begin, ps.op_end, a.mats, mbuf,
+              [&](const DevOp& op, const float2* M) { apply_op<false, KMAX>(A, ps.L, op, M); });
+      for (uint32_t j = tid; j < T; j += kThreads)
+        G[use_tab ? tile_index_tab(base, j, ps.nlow, tab) : tile_index(base, j, ps)] = A[j];
       __syncthreads();
     }
     first = false;
@@ -162,33 +164,36 @@ __device__ void tiled_forward(float2* G, int n, const BatchArgs& a, int pb, int 
 }
 
 // Tiled adjoint sweep on psi (G) and lambda (Lg) with LDS tiles A and B.
+template <int KMAX>
 __device__ void tiled_reverse(float2* G, float2* Lg, int n, const BatchArgs& a, int pb, int pe,
-                              float2* A, float2* B, double* gacc, double* red) {
+                              float2* A, float2* B, double* gacc, double* red, float2* mbuf,
+                              uint64_t* tab) {
   const int tid = threadIdx.x;
   for (int p = pb; p < pe; ++p) {
     const DevPass ps = a.passes[p];
+    const bool use_tab = ps.L - ps.nlow <= 8;
+    build_tile_table(ps, tab);
+    __syncthreads();
     const uint32_t T = 1u << ps.L;
     const uint32_t ntiles = 1u << (n - ps.L);
     for (uint32_t t = 0; t < ntiles; ++t) {
       const uint64_t base = tile_base(t, ps.tilemask, n);
       for (uint32_t j = tid; j < T; j += kThreads) {
-        const uint64_t g = tile_index(base, j, ps);
+        const uint64_t g = use_tab ? tile_index_tab(base, j, ps.nlow, tab) : tile_index(base, j, ps);
         A[j] = G[g];
         B[j] = Lg[g];
       }
       __syncthreads();
-      for (int i = ps.op_begin; i < ps.op_end; ++i) {
-        const DevOp op = a.ops[i];
+      for_ops(a.ops, ps.op_begin, ps.op_end, a.mats, mbuf, [&](const DevOp& op, const float2* M) {
         if (op.type == kOpParam) {
-          param_step(A, B, ps.L, op, a.mats, gacc, red);
+          param_step(A, B, ps.L, op, M, gacc, red);
         } else {
-          apply_op<true>(A, ps.L, op, a.mats);
-          apply_op<true>(B, ps.L, op, a.mats);
+          apply_op<true, KMAX>(A, ps.L, op, M);
+          apply_op<true, KMAX>(B, ps.L, op, M);
         }
-        __syncthreads();
-      }
+      });
       for (uint32_t j = tid; j < T; j += kThreads) {
-        const uint64_t g = tile_index(base, j, ps);
+        const uint64_t g = use_tab ? tile_index_tab(base, j, ps.nlow, tab) : tile_index(base, j, ps);
         G[g] = A[j];
         Lg[g] = B[j];
       }
@@ -197,11 +202,18 @@ __device__ void tiled_reverse(float2* G, float2* Lg, int n, const BatchArgs& a, 
   }
 }
 
-template <int MODE>
+template <int MODE, int KMAX>
 __global__ __launch_bounds__(kThreads) void batch_kernel(BatchArgs a) {
   extern __shared__ float2 lds[];
   __shared__ double red[kWaves];
   __shared__ int s_ci;
+  __shared__ float2 mbuf[2 * kMatSlot];
+  __shared__ uint64_t tab[kTileTable];
+  __shared__ float2 rotS[8];
+  if (MODE == kModeSampledExpectation && threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) rotS[i] = a.rot[i];
+  }
   float2* psi = lds;
   float2* buf2 = lds + a.lds_amps;
   const int tid = threadIdx.x;
@@ -240,10 +252,11 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(BatchArgs a) {
     }
 
     if (tiled) {
-      tiled_forward(gpsi, n, a, uniform(c.fwd_pass_begin), uniform(c.fwd_pass_end), psi);
+      tiled_forward<KMAX>(gpsi, n, a, uniform(c.fwd_pass_begin), uniform(c.fwd_pass_end), psi, mbuf,
+                    tab);
     } else {
       set_zero_state(psi, N);
-      run_ops(psi, n, a.ops, uniform(c.fwd_begin), uniform(c.fwd_end), a.mats);
+      run_ops<KMAX>(psi, n, a.ops, uniform(c.fwd_begin), uniform(c.fwd_end), a.mats, mbuf);
     }
 
     if (MODE == kModeExpectation || (MODE == kModeAdjoint && a.exp_out)) {
@@ -257,21 +270,20 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(BatchArgs a) {
       const int lb = uniform(c.lam_begin), le = uniform(c.lam_end);
       if (tiled) {
         build_lambda(gpsi, glam, N, a, lb, le);
-        tiled_reverse(gpsi, glam, n, a, uniform(c.rev_pass_begin), uniform(c.rev_pass_end), psi,
-                      buf2, gacc, red);
+        tiled_reverse<KMAX>(gpsi, glam, n, a, uniform(c.rev_pass_begin), uniform(c.rev_pass_end), psi,
+                      buf2, gacc, red, mbuf, tab);
       } else {
         float2* lam = buf2;
         build_lambda(psi, lam, N, a, lb, le);
-        for (int i = uniform(c.rev_begin); i < c.rev_end; ++i) {
-          const DevOp op = a.ops[i];
-          if (op.type == kOpParam) {
-            param_step(psi, lam, n, op, a.mats, gacc, red);
-          } else {
-            apply_op<true>(psi, n, op, a.mats);
-            apply_op<true>(lam, n, op, a.mats);
-          }
-          __syncthreads();
-        }
+        for_ops(a.ops, uniform(c.rev_begin), uniform(c.rev_end), a.mats, mbuf,
+                [&](const DevOp& op, const float2* M) {
+                  if (op.type == kOpParam) {
+                    param_step(psi, lam, n, op, M, gacc, red);
+                  } else {
+                    apply_op<true, KMAX>(psi, n, op, M);
+                    apply_op<true, KMAX>(lam, n, op, M);
+                  }
+                });
       }
       __threadfence_block();
       __syncthreads();
@@ -315,7 +327,7 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(BatchArgs a) {
           for (int q = 0; q < n; ++q) {
             if (!((T.fmask >> q) & 1u)) continue;
             const bool is_y = (T.zmask >> q) & 1u;
-            apply_1q(work, n, q, is_y ? &a.rot[4] : &a.rot[0]);
+            apply_1q(work, n, q, is_y ? &rotS[4] : &rotS[0]);
             __syncthreads();
           }
           const double total = build_cdf(work, N, red);
@@ -343,7 +355,7 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(BatchArgs a) {
       for (int o = uniform(c.other_begin); o < c.other_end; ++o) {
         const DevOther ot = a.others[o];
         set_zero_state(phi, N);
-        run_ops(phi, n, a.ops, uniform(ot.op_begin), uniform(ot.op_end), a.mats);
+        run_ops<KMAX>(phi, n, a.ops, uniform(ot.op_begin), uniform(ot.op_end), a.mats, mbuf);
         double re = 0.0, im = 0.0;
         for (uint32_t x = tid; x < N; x += kThreads) {
           const float2 p = cmul_conj_a(psi[x], phi[x]);
@@ -372,18 +384,19 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(BatchArgs a) {
 }
 
 template <int MODE>
-const void* KernelPtr() {
-  return reinterpret_cast<const void*>(&batch_kernel<MODE>);
+const void* KernelPtr(int kmax) {
+  return kmax <= 2 ? reinterpret_cast<const void*>(&batch_kernel<MODE, 2>)
+                   : reinterpret_cast<const void*>(&batch_kernel<MODE, 4>);
 }
 
-const void* KernelFor(int mode) {
+const void* KernelFor(int mode, int kmax) {
   switch (mode) {
-    case kModeExpectation: return KernelPtr<kModeExpectation>();
-    case kModeAdjoint: return KernelPtr<kModeAdjoint>();
-    case kModeState: return KernelPtr<kModeState>();
-    case kModeSamples: return KernelPtr<kModeSamples>();
-    case kModeSampledExpectation: return KernelPtr<kModeSampledExpectation>();
-    default: return KernelPtr<kModeInnerProduct>();
+    case kModeExpectation: return KernelPtr<kModeExpectation>(kmax);
+    case kModeAdjoint: return KernelPtr<kModeAdjoint>(kmax);
+    case kModeState: return KernelPtr<kModeState>(kmax);
+    case kModeSamples: return KernelPtr<kModeSamples>(kmax);
+    case kModeSampledExpectation: return KernelPtr<kModeSampledExpectation>(kmax);
+    default: return KernelPtr<kModeInnerProduct>(kmax);
   }
 }
 
@@ -402,15 +415,16 @@ hipError_t LaunchBatch(const BatchArgs& args, int grid, hipStream_t stream) {
   // attribute (and setting one above 64 KiB is rejected by the runtime).
   const size_t lds = BatchLdsBytes(args.lds_log2, args.mode);
   void* params[] = {const_cast<BatchArgs*>(&args)};
-  return hipLaunchKernel(KernelFor(args.mode), dim3(grid), dim3(kThreads), params, lds, stream);
+  return hipLaunchKernel(KernelFor(args.mode, args.kmax), dim3(grid), dim3(kThreads), params, lds,
+                         stream);
 }
 
-int BatchMaxGrid(int lds_log2, int mode) {
+int BatchMaxGrid(int lds_log2, int mode, int kmax) {
   int dev = 0, cus = 0, per_cu = 0;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   const size_t lds = BatchLdsBytes(lds_log2, mode);
-  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, KernelFor(mode), kThreads, lds);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, KernelFor(mode, kmax), kThreads, lds);
   if (per_cu < 1) per_cu = 1;
   if (cus < 1) cus = 1;
   return cus * per_cu;

@@ -47,20 +47,25 @@ __global__ __launch_bounds__(kThreads) void pass_kernel(PassArgs a) {
   extern __shared__ float2 lds[];
   __shared__ double red[kWaves];
   __shared__ double pacc[kMaxPassDerivs];
-  const int L = a.pass.L;
+  __shared__ float2 mbuf[2 * kMatSlot];
+  __shared__ uint64_t tab[kTileTable];
+  const DevPass ps = a.pass;
+  const int L = ps.L;
   const uint32_t T = 1u << L;
   float2* A = lds;
   float2* Bf = lds + T;
   const int tid = threadIdx.x;
   const uint64_t ntiles = uint64_t(1) << (a.n - L);
-  if (TWO) {
+  const bool use_tab = ps.L - ps.nlow <= 8;
+  const int nlow = ps.nlow;
+  build_tile_table(ps, tab);
+  if (TWO)
     for (int k = tid; k < kMaxPassDerivs; k += kThreads) pacc[k] = 0.0;
-    __syncthreads();
-  }
+  __syncthreads();
   for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const uint64_t base = tile_base(t, a.pass.tilemask, a.n);
+    const uint64_t base = tile_base(t, ps.tilemask, a.n);
     for (uint32_t j = tid; j < T; j += kThreads) {
-      const uint64_t g = tile_index(base, j, a.pass);
+      const uint64_t g = use_tab ? tile_index_tab(base, j, nlow, tab) : tile_index(base, j, ps);
       if (TWO) {
         A[j] = a.psi[g];
         Bf[j] = a.lam[g];
@@ -70,31 +75,28 @@ __global__ __launch_bounds__(kThreads) void pass_kernel(PassArgs a) {
     }
     __syncthreads();
     int pl = 0;
-    for (int i = a.pass.op_begin; i < a.pass.op_end; ++i) {
-      const DevOp op = a.ops[i];
-      if (!TWO) {
-        apply_op<false>(A, L, op, a.mats);
-      } else if (op.type == kOpParam) {
-        double acc[kMaxDeriv] = {0.0, 0.0};
-        switch (op.k) {
-          case 1: apply_param<1>(A, Bf, L, op, a.mats, acc); break;
-          case 2: apply_param<2>(A, Bf, L, op, a.mats, acc); break;
-          case 3: apply_param<3>(A, Bf, L, op, a.mats, acc); break;
-          default: apply_param<4>(A, Bf, L, op, a.mats, acc); break;
-        }
-        for (int d = 0; d < op.nderiv; ++d) {
-          const double tot = block_sum(acc[d], red);
-          if (tid == 0) pacc[pl + d] += tot;
-        }
-        pl += op.nderiv;
-      } else {
-        apply_op<true>(A, L, op, a.mats);
-        apply_op<true>(Bf, L, op, a.mats);
-      }
-      __syncthreads();
-    }
+    for_ops(a.ops, ps.op_begin, ps.op_end, a.mats, mbuf,
+            [&](const DevOp& op, const float2* M) {
+              if (!TWO) {
+                apply_op<false>(A, L, op, M);
+              } else if (op.type == kOpParam) {
+                double acc[kMaxDeriv] = {0.0, 0.0};
+                if (op.k == 1) apply_param<1>(A, Bf, L, op, M, acc);
+                else apply_param<2>(A, Bf, L, op, M, acc);
+#pragma unroll
+                for (int d = 0; d < kMaxDeriv; ++d) {
+                  if (d >= op.nderiv) break;
+                  const double tot = block_sum(acc[d], red);
+                  if (tid == 0) pacc[pl + d] += tot;
+                }
+                pl += op.nderiv;
+              } else {
+                apply_op<true>(A, L, op, M);
+                apply_op<true>(Bf, L, op, M);
+              }
+            });
     for (uint32_t j = tid; j < T; j += kThreads) {
-      const uint64_t g = tile_index(base, j, a.pass);
+      const uint64_t g = use_tab ? tile_index_tab(base, j, nlow, tab) : tile_index(base, j, ps);
       if (TWO) {
         a.psi[g] = A[j];
         a.lam[g] = Bf[j];
@@ -106,11 +108,11 @@ __global__ __launch_bounds__(kThreads) void pass_kernel(PassArgs a) {
   }
   if (TWO && tid == 0) {
     int pl = 0;
-    for (int i = a.pass.op_begin; i < a.pass.op_end; ++i) {
-      const DevOp op = a.ops[i];
+    for (int i = ps.op_begin; i < ps.op_end; ++i) {
+      const DevOp op = load_op(a.ops + i);
       if (op.type != kOpParam) continue;
-      for (int d = 0; d < op.nderiv; ++d)
-        a.partials[size_t(op.col[d]) * gridDim.x + blockIdx.x] = pacc[pl + d];
+      a.partials[size_t(op.col[0]) * gridDim.x + blockIdx.x] = pacc[pl];
+      if (op.nderiv > 1) a.partials[size_t(op.col[1]) * gridDim.x + blockIdx.x] = pacc[pl + 1];
       pl += op.nderiv;
     }
   }

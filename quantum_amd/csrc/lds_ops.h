@@ -11,8 +11,12 @@
 namespace tfqx {
 namespace {
 
-constexpr int kThreads = 256;
+constexpr int kThreads = 512;
 constexpr int kWaves = kThreads / 64;
+// float2 entries of one matrix staging buffer in LDS: a fused unitary of up
+// to 4 qubits (16x16), or a parameterised gate (<= 2 qubits) with its two
+// derivative matrices.
+constexpr int kMatSlot = 256;
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -27,6 +31,69 @@ __device__ __forceinline__ uint32_t insert_zero(uint32_t x, uint32_t q) {
 }
 
 __device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// Load an op descriptor into scalar registers (all lanes read the same op).
+__device__ __forceinline__ DevOp load_op(const DevOp* p) {
+  // Field-wise from 32-bit words so the descriptor stays in registers.
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(p);
+  uint32_t w[10];
+#pragma unroll
+  for (int i = 0; i < 10; ++i) w[i] = __builtin_amdgcn_readfirstlane(src[i]);
+  DevOp o;
+  o.type = uint8_t(w[0]);
+  o.k = uint8_t(w[0] >> 8);
+  o.nderiv = uint8_t(w[0] >> 16);
+  o.pad0 = 0;
+  o.q[0] = uint8_t(w[1]);
+  o.q[1] = uint8_t(w[1] >> 8);
+  o.q[2] = uint8_t(w[1] >> 16);
+  o.q[3] = uint8_t(w[1] >> 24);
+  o.cmask = w[2];
+  o.cval = w[3];
+  o.mat = w[4];
+  o.dmat[0] = w[5];
+  o.dmat[1] = w[6];
+  o.col[0] = int32_t(w[7]);
+  o.col[1] = int32_t(w[8]);
+  return o;
+}
+
+// Copy an op's matrices into an LDS staging buffer: [G | dG_0 | dG_1].
+__device__ __forceinline__ void stage_op(const DevOp& op, const float2* __restrict__ mats,
+                                         float2* __restrict__ dst) {
+  const int D = 1 << op.k;
+  const int DD = op.type == kOpDiag ? D : D * D;
+  for (int i = threadIdx.x; i < DD; i += kThreads) dst[i] = mats[op.mat + i];
+  if (op.type == kOpParam) {
+#pragma unroll
+    for (int d = 0; d < kMaxDeriv; ++d)
+      if (d < op.nderiv)
+        for (int i = threadIdx.x; i < DD; i += kThreads) dst[DD * (1 + d) + i] = mats[op.dmat[d] + i];
+  }
+}
+
+// Run ops [b, e): f(op, M) with M the op's matrices staged in LDS.  The next
+// op's matrices are staged while the current op runs (two buffers), so each
+// op costs one barrier.
+template <typename F>
+__device__ __forceinline__ void for_ops(const DevOp* ops, int b, int e, const float2* mats,
+                                        float2* mbuf, F&& f) {
+  if (b >= e) return;
+  {
+    const DevOp op0 = load_op(ops + b);
+    stage_op(op0, mats, mbuf);
+  }
+  __syncthreads();
+  for (int i = b; i < e; ++i) {
+    const DevOp op = load_op(ops + i);
+    if (i + 1 < e) {
+      const DevOp nx = load_op(ops + i + 1);
+      stage_op(nx, mats, mbuf + ((i + 1 - b) & 1) * kMatSlot);
+    }
+    f(op, mbuf + ((i - b) & 1) * kMatSlot);
+    __syncthreads();
+  }
+}
 
 __device__ double block_sum(double v, double* red) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -83,6 +150,8 @@ __device__ __forceinline__ double philox_uniform(unsigned long long seed, unsign
 
 // ------------------------------------------------------------------ gate application
 // s <- M s (or M^dagger s) on targets q[0..K-1] of an n-qubit state in LDS.
+// For K >= 3 the matrix entries are re-read from LDS per group instead of
+// being hoisted into registers (a 16x16 complex matrix would take 512 VGPRs).
 template <int K, bool DAG>
 __device__ void apply_dense(float2* __restrict__ s, int n, const DevOp& op,
                             const float2* __restrict__ M) {
@@ -101,6 +170,7 @@ __device__ void apply_dense(float2* __restrict__ s, int n, const DevOp& op,
 #pragma unroll
     for (int j = 0; j < K; ++j) base = insert_zero(base, op.q[j]);
     if ((base & op.cmask) != op.cval) continue;
+    if constexpr (K >= 3) __asm__ volatile("" ::: "memory");  // keep M in LDS, not VGPRs
     float2 v[D];
 #pragma unroll
     for (int m = 0; m < D; ++m) v[m] = s[base | dep[m]];
@@ -126,25 +196,27 @@ __device__ void apply_diag(float2* __restrict__ s, int n, const DevOp& op,
   for (uint32_t x = threadIdx.x; x < N; x += kThreads) {
     if ((x & op.cmask) != op.cval) continue;
     uint32_t m = 0;
-    for (int j = 0; j < k; ++j) m |= ((x >> op.q[j]) & 1u) << j;
+#pragma unroll
+    for (int j = 0; j < kMaxFuse; ++j)
+      if (j < k) m |= ((x >> op.q[j]) & 1u) << j;
     float2 d = Dg[m];
     if (DAG) d.y = -d.y;
     s[x] = cmul(d, s[x]);
   }
 }
 
-template <bool DAG>
-__device__ void apply_op(float2* s, int n, const DevOp& op, const float2* mats) {
-  const float2* M = mats + op.mat;
+// KMAX bounds the fused width the caller may see (fewer template cases keep
+// register pressure low when all ops are 1-2 qubit).
+template <bool DAG, int KMAX = kMaxFuse>
+__device__ void apply_op(float2* s, int n, const DevOp& op, const float2* M) {
   if (op.type == kOpDiag) {
     apply_diag<DAG>(s, n, op, M);
+  } else if (KMAX <= 2 || op.k <= 2) {
+    if (op.k == 1) apply_dense<1, DAG>(s, n, op, M);
+    else apply_dense<2, DAG>(s, n, op, M);
   } else {
-    switch (op.k) {
-      case 1: apply_dense<1, DAG>(s, n, op, M); break;
-      case 2: apply_dense<2, DAG>(s, n, op, M); break;
-      case 3: apply_dense<3, DAG>(s, n, op, M); break;
-      default: apply_dense<4, DAG>(s, n, op, M); break;
-    }
+    if (op.k == 3) apply_dense<(KMAX > 2 ? 3 : 2), DAG>(s, n, op, M);
+    else apply_dense<(KMAX > 2 ? 4 : 2), DAG>(s, n, op, M);
   }
 }
 
@@ -166,9 +238,9 @@ __device__ void apply_1q(float2* __restrict__ s, int n, int q, const float2* M) 
 // (the BulkSetAmpl projection of tfq_adj_grad_op.cc:241-260).
 template <int K>
 __device__ void apply_param(float2* __restrict__ psi, float2* __restrict__ lam, int n,
-                            const DevOp& op, const float2* __restrict__ mats, double* acc) {
+                            const DevOp& op, const float2* __restrict__ M, double* acc) {
   constexpr int D = 1 << K;
-  const float2* G = mats + op.mat;
+  const float2* G = M;
   const uint32_t ngroups = 1u << (n - K);
   const int nd = op.nderiv;
   uint32_t dep[D];
@@ -199,8 +271,10 @@ __device__ void apply_param(float2* __restrict__ psi, float2* __restrict__ lam, 
     }
 #pragma unroll
     for (int m = 0; m < D; ++m) psi[base | dep[m]] = t[m];
-    for (int d = 0; d < nd; ++d) {
-      const float2* dG = mats + op.dmat[d];
+#pragma unroll
+    for (int d = 0; d < kMaxDeriv; ++d) {
+      if (d >= nd) break;
+      const float2* dG = M + D * D * (1 + d);
       float part = 0.f;
 #pragma unroll
       for (int r = 0; r < D; ++r) {
@@ -235,14 +309,34 @@ __device__ __forceinline__ uint64_t tile_base(uint64_t t, uint64_t tm, int n) {
   return base;
 }
 
-__device__ __forceinline__ uint64_t tile_index(uint64_t base, uint32_t j, const DevPass& p) {
-  uint64_t g = base | (j & ((1u << p.nlow) - 1u));
-  uint32_t hj = j >> p.nlow;
-  for (int h = p.nlow; h < p.L; ++h) {
-    g |= uint64_t(hj & 1u) << p.tbit[h];
-    hj >>= 1;
+// Per-workgroup table of the deposit of j's high bits (j >> nlow) into the
+// pass's high tile bits; kTileTable entries cover passes with L - nlow <= 8.
+constexpr int kTileTable = 256;
+
+__device__ __forceinline__ uint64_t deposit_high(uint32_t jh, const DevPass& p) {
+  // Bits of jh go to tbit[nlow], tbit[nlow+1], ...; the loop is unrolled so
+  // tbit is indexed statically (no private-memory copy of the pass).
+  uint64_t g = 0;
+#pragma unroll
+  for (int h = 0; h < kMaxTile; ++h) {
+    if (h >= p.nlow && h < p.L) g |= uint64_t((jh >> (h - p.nlow)) & 1u) << p.tbit[h];
   }
   return g;
+}
+
+__device__ __forceinline__ void build_tile_table(const DevPass& p, uint64_t* tab) {
+  const int hb = p.L - p.nlow;
+  if (hb > 8) return;
+  for (int jh = threadIdx.x; jh < (1 << hb); jh += kThreads) tab[jh] = deposit_high(jh, p);
+}
+
+__device__ __forceinline__ uint64_t tile_index_tab(uint64_t base, uint32_t j, int nlow,
+                                                   const uint64_t* tab) {
+  return base | (j & ((1u << nlow) - 1u)) | tab[j >> nlow];
+}
+
+__device__ __forceinline__ uint64_t tile_index(uint64_t base, uint32_t j, const DevPass& p) {
+  return base | (j & ((1u << p.nlow) - 1u)) | deposit_high(j >> p.nlow, p);
 }
 
 }  // namespace

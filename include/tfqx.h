@@ -134,9 +134,20 @@ int tfqx_plan_expectation_and_gradient(
     const float* downstream_grads, int32_t derivative,
     tfqx_plan** plan, char* err, int64_t err_len);
 
+/* Re-evaluate a plan for new symbol values [batch, num_symbols] without
+ * re-parsing or re-fusing; call tfqx_plan_upload again before launching.  Only
+ * for plans whose programs are one byte-identical LDS-resident program tiled
+ * over the batch (TFQ layers tile one circuit over the batch,
+ * python/layers/circuit_executors/expectation.py:231-390); other plans return
+ * TFQX_INVALID_ARGUMENT.  No reference counterpart: the reference re-parses
+ * every call (core/ops/parse_context.cc:60-92). */
+int tfqx_plan_rebind(tfqx_plan* plan, const float* symbol_values, char* err, int64_t err_len);
+
 /* Plan statistics: out[0] circuits, [1] forward ops, [2] forward HBM passes,
  * [3] max qubits, [4] adjoint ops, [5] adjoint HBM passes, [6] 1 if the
- * HBM-streaming path is used, [7] workspace bytes.  No reference counterpart. */
+ * HBM-streaming path is used, [7] workspace bytes, [8] algorithmic real flops
+ * of one launch (batch path; 0 on the streaming path).  No reference
+ * counterpart. */
 int tfqx_plan_stats(const tfqx_plan* plan, int64_t* out, int64_t n);
 
 /* Largest qubit count in the batch (sizes the samples / state outputs). */
@@ -160,6 +171,36 @@ int tfqx_graph_capture(tfqx_plan* plan, void* workspace, void* output, void* str
 int tfqx_graph_launch(tfqx_graph* graph, void* stream, char* err, int64_t err_len);
 void tfqx_graph_destroy(tfqx_graph* graph);
 void tfqx_plan_destroy(tfqx_plan* plan);
+
+/* ---------------------------------------------------------------- surrogate
+ * MLP surrogate E(x) with input gradient dE/dx (in_dim -> hidden x3 -> 1,
+ * SiLU), trained with the Sobolev loss mean (E-E*)^2 + lam mean ||dE/dx-g*||^2
+ * on BF16 MFMA GEMMs (FP32 accumulate) and Adam.  No reference counterpart:
+ * TFQ has no learned component; this serves VQE loops that call
+ * TfqAdjointGradient repeatedly.  All pointers except host_params are device
+ * pointers; the workspace holds parameters, Adam state and activations. */
+/* out: [workspace bytes, #params, param offset, grad offset, loss offset,
+ * Adam m offset] (byte offsets into the workspace). */
+int tfqx_mlp_layout(int32_t in_dim, int32_t hidden, int32_t max_batch, int64_t* out, int64_t n);
+int tfqx_mlp_set_params(void* workspace, int32_t in_dim, int32_t hidden, int32_t max_batch,
+                        const float* host_params, void* stream, char* err, int64_t err_len);
+/* energy[batch], grad[batch, in_dim] (grad may be null). */
+int tfqx_mlp_predict(void* workspace, int32_t in_dim, int32_t hidden, int32_t max_batch,
+                     const float* x, int32_t batch, float* energy, float* grad, void* stream,
+                     char* err, int64_t err_len);
+/* Loss (2 floats at the loss offset) and parameter gradient (grad offset). */
+int tfqx_mlp_loss_grad(void* workspace, int32_t in_dim, int32_t hidden, int32_t max_batch,
+                       const float* x, const float* energy, const float* grad, int32_t batch,
+                       float lam, void* stream, char* err, int64_t err_len);
+/* loss_grad followed by one Adam step (step >= 1). */
+int tfqx_mlp_train_step(void* workspace, int32_t in_dim, int32_t hidden, int32_t max_batch,
+                        const float* x, const float* energy, const float* grad, int32_t batch,
+                        float lam, float lr, int32_t step, void* stream, char* err,
+                        int64_t err_len);
+/* C = alpha op(A) op(B) + beta C; A, B bf16 row-major, C fp32 (v_mfma_f32_32x32x16_bf16). */
+int tfqx_gemm_bf16(int32_t M, int32_t N, int32_t K, const void* A, int32_t lda, int32_t transA,
+                   const void* B, int32_t ldb, int32_t transB, float* C, int32_t ldc, float alpha,
+                   float beta, void* stream, char* err, int64_t err_len);
 
 /* Fusion width (qubits per fused block) the planner uses; 0 = autotuned. */
 void tfqx_set_fusion_width(int32_t width);

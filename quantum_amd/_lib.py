@@ -48,6 +48,8 @@ def _declare(lib):
     lib.tfqx_plan_inner_product.argtypes = s + [P, P, i64, P, c.c_char_p, i64]
     lib.tfqx_plan_adjoint_gradient.argtypes = s + [P, P, i64, P, i32, P, c.c_char_p, i64]
     lib.tfqx_plan_expectation_and_gradient.argtypes = s + [P, P, i64, P, i32, P, c.c_char_p, i64]
+    lib.tfqx_plan_rebind.argtypes = [P, P, c.c_char_p, i64]
+    lib.tfqx_plan_rebind.restype = c.c_int
     lib.tfqx_plan_stats.argtypes = [P, P, i64]
     lib.tfqx_plan_stats.restype = c.c_int
     for name in ("tfqx_plan_simulate_expectation", "tfqx_plan_simulate_sampled_expectation",
@@ -71,6 +73,17 @@ def _declare(lib):
     lib.tfqx_set_fusion_width.argtypes = [i32]
     lib.tfqx_set_fusion_width.restype = None
     lib.tfqx_version.restype = c.c_char_p
+    f32 = c.c_float
+    E = [c.c_char_p, i64]
+    lib.tfqx_mlp_layout.argtypes = [i32, i32, i32, P, i64]
+    lib.tfqx_mlp_set_params.argtypes = [P, i32, i32, i32, P, P] + E
+    lib.tfqx_mlp_predict.argtypes = [P, i32, i32, i32, P, i32, P, P, P] + E
+    lib.tfqx_mlp_loss_grad.argtypes = [P, i32, i32, i32, P, P, P, i32, f32, P] + E
+    lib.tfqx_mlp_train_step.argtypes = [P, i32, i32, i32, P, P, P, i32, f32, f32, i32, P] + E
+    lib.tfqx_gemm_bf16.argtypes = [i32, i32, i32, P, i32, i32, P, i32, i32, P, i32, f32, f32, P] + E
+    for name in ("tfqx_mlp_layout", "tfqx_mlp_set_params", "tfqx_mlp_predict", "tfqx_mlp_loss_grad",
+                 "tfqx_mlp_train_step", "tfqx_gemm_bf16"):
+        getattr(lib, name).restype = c.c_int
     return lib
 
 
@@ -89,11 +102,13 @@ def lib():
 EXPORTED_SYMBOLS = (
     "tfqx_plan_simulate_expectation", "tfqx_plan_simulate_sampled_expectation",
     "tfqx_plan_simulate_samples", "tfqx_plan_simulate_state", "tfqx_plan_inner_product",
-    "tfqx_plan_adjoint_gradient", "tfqx_plan_expectation_and_gradient", "tfqx_plan_stats",
+    "tfqx_plan_adjoint_gradient", "tfqx_plan_expectation_and_gradient", "tfqx_plan_rebind",
+    "tfqx_plan_stats",
     "tfqx_plan_max_num_qubits", "tfqx_plan_workspace_bytes",
     "tfqx_plan_output_bytes", "tfqx_plan_upload", "tfqx_plan_launch", "tfqx_plan_launch_plain",
     "tfqx_graph_capture", "tfqx_graph_launch", "tfqx_graph_destroy", "tfqx_plan_destroy",
-    "tfqx_set_fusion_width", "tfqx_version")
+    "tfqx_set_fusion_width", "tfqx_version", "tfqx_mlp_layout", "tfqx_mlp_set_params",
+    "tfqx_mlp_predict", "tfqx_mlp_loss_grad", "tfqx_mlp_train_step", "tfqx_gemm_bf16")
 
 
 def check(rc, err):
@@ -142,11 +157,19 @@ class Plan:
         return int(self._lib.tfqx_plan_max_num_qubits(self.handle))
 
     def stats(self):
-        out = (ctypes.c_int64 * 8)()
-        self._lib.tfqx_plan_stats(self.handle, out, 8)
+        out = (ctypes.c_int64 * 9)()
+        self._lib.tfqx_plan_stats(self.handle, out, 9)
         keys = ("circuits", "forward_ops", "forward_passes", "max_qubits", "adjoint_ops",
-                "adjoint_passes", "streaming", "workspace_bytes")
+                "adjoint_passes", "streaming", "workspace_bytes", "flops")
         return dict(zip(keys, [int(x) for x in out]))
+
+    def rebind(self, values):
+        """New symbol values (float32 [batch, n_symbols]) for a plan built from one
+        program tiled over the batch; upload again before launching."""
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        err = ctypes.create_string_buffer(4096)
+        check(self._lib.tfqx_plan_rebind(self.handle, v.ctypes.data if v.size else None, err, 4096),
+              err)
 
     @property
     def workspace_bytes(self):

@@ -16,8 +16,8 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtfqx.so")
 ARCH = os.environ.get("TFQX_OFFLOAD_ARCH", "gfx950")
 
-HOST_SOURCES = ["wire.cc", "circuit.cc", "plan.cc", "large.cc", "api.cc"]
-HIP_SOURCES = ["batch_kernels.hip", "large_kernels.hip"]
+HOST_SOURCES = ["wire.cc", "circuit.cc", "plan.cc", "large.cc", "api.cc", "surrogate_api.cc"]
+HIP_SOURCES = ["batch_kernels.hip", "large_kernels.hip", "surrogate_kernels.hip"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
           # Packed FP32 (v_pk_*_f32) stays off for every device code object.

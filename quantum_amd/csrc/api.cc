@@ -39,7 +39,7 @@ int Fail(char* err, int64_t len, int code, const std::string& msg) {
 
 int HostThreads() {
   unsigned n = std::thread::hardware_concurrency();
-  return int(std::max(1u, std::min(n, 32u)));
+  return int(std::max(1u, std::min(n, 16u)));  // one GPU's share of host cores
 }
 
 template <typename F>
@@ -97,6 +97,12 @@ struct tfqx_plan {
   int kmax = 2;
   int grid = 0;
   bool with_expectation = false;  // adjoint plan also writes expectations
+  bool tfq_fd = true;
+  // All rows share one program: the template circuit and matrix recipes let
+  // tfqx_plan_rebind re-evaluate the matrices for new symbol values.
+  bool shared = false;
+  Circuit tmpl;
+  SharedRecipe recipe;
   // States wider than one LDS tile run on the HBM-streaming path.
   bool large = false;
   std::unique_ptr<LargePlan> large_plan;
@@ -115,7 +121,20 @@ struct Inputs {
   std::vector<Circuit> circuits;
   std::vector<std::vector<std::vector<Pauli>>> sums;   // [B][O]
   std::vector<std::vector<Circuit>> others;            // [B][K]
+  const float* values = nullptr;                       // [B, S]
+  // All programs are byte-identical and LDS-resident: circuits[0] is the
+  // program resolved with row 0's values and stands for every row.
+  bool shared = false;
 };
+
+int LdsLimit(int mode) {
+  return BatchBuffers(mode) == 2 ? kLdsMaxQubitsTwoBuffers : kLdsMaxQubitsOneBuffer;
+}
+
+bool SameBytes(const uint8_t* b, const int64_t* off, int64_t i, int64_t j) {
+  const int64_t n = off[i + 1] - off[i];
+  return n == off[j + 1] - off[j] && memcmp(b + off[i], b + off[j], size_t(n)) == 0;
+}
 
 Status ParsePrograms(const uint8_t* b, const int64_t* off, int64_t n, std::vector<Program>* out) {
   out->assign(size_t(n), Program());
@@ -130,9 +149,14 @@ Status ParsePrograms(const uint8_t* b, const int64_t* off, int64_t n, std::vecto
 Status Prepare(const uint8_t* programs, const int64_t* poff, int64_t B, const uint8_t* names,
                const int64_t* noff, int64_t S, const float* values, const uint8_t* psums,
                const int64_t* psoff, int64_t O, const uint8_t* other, const int64_t* ooff,
-               int64_t K, Inputs* in) {
+               int64_t K, Inputs* in, int shared_max_n = -1) {
   if (B < 0 || S < 0 || O < 0 || K < 0) return Status::Invalid("negative dimension");
-  Status s = ParsePrograms(programs, poff, B, &in->programs);
+  in->values = values;
+  // Byte-identical programs (a circuit tiled over a batch of parameter sets)
+  // are parsed and resolved once.
+  bool same = B >= 1 && !other;
+  for (int64_t i = 1; same && i < B; ++i) same = SameBytes(programs, poff, i, 0);
+  Status s = ParsePrograms(programs, poff, same ? 1 : B, &in->programs);
   if (!s.ok()) return s;
   std::vector<PauliSum> ps;
   if (psums) {
@@ -150,27 +174,45 @@ Status Prepare(const uint8_t* programs, const int64_t* poff, int64_t B, const ui
     if (!s.ok()) return s;
   }
   for (int64_t j = 0; j < S; ++j) in->columns[Bytes(names, noff, j)] = int(j);
-  in->circuits.assign(size_t(B), Circuit());
+  Circuit tmpl;
+  if (same) {
+    SymbolMap sm;
+    sm.columns = &in->columns;
+    sm.values = values;
+    s = BuildCircuit(in->programs[0], sm, &tmpl);
+    if (!s.ok()) return s;
+    in->shared = tmpl.num_qubits <= shared_max_n;
+  }
+  in->circuits.assign(in->shared ? 1 : size_t(B), Circuit());
+  if (in->shared) in->circuits[0] = tmpl;
   in->sums.assign(size_t(B), {});
   in->others.assign(size_t(B), {});
   FirstError fe;
   ParallelFor(int(B), [&](int i) {
-    SymbolMap sm;
-    sm.columns = &in->columns;
-    sm.values = values + size_t(i) * S;
-    Status st = BuildCircuit(in->programs[i], sm, &in->circuits[i]);
-    if (!st.ok()) return fe.record(st);
+    Status st;
+    if (!in->shared) {
+      if (same) {
+        RebindCircuit(tmpl, values + size_t(i) * S, &in->circuits[i]);
+      } else {
+        SymbolMap sm;
+        sm.columns = &in->columns;
+        sm.values = values + size_t(i) * S;
+        st = BuildCircuit(in->programs[i], sm, &in->circuits[i]);
+        if (!st.ok()) return fe.record(st);
+      }
+    }
+    const Circuit& circ = in->circuits[in->shared ? 0 : i];
     if (psums) {
       in->sums[i].resize(size_t(O));
       for (int64_t j = 0; j < O; ++j) {
-        st = ResolvePauliSum(ps[size_t(i * O + j)], in->circuits[i], &in->sums[i][j]);
+        st = ResolvePauliSum(ps[size_t(i * O + j)], circ, &in->sums[i][j]);
         if (!st.ok()) return fe.record(st);
       }
     }
     if (other) {
       in->others[i].resize(size_t(K));
       for (int64_t k = 0; k < K; ++k) {
-        st = BuildPairedCircuit(others[size_t(i * K + k)], in->circuits[i], &in->others[i][k]);
+        st = BuildPairedCircuit(others[size_t(i * K + k)], circ, &in->others[i][k]);
         if (!st.ok()) return fe.record(st);
       }
     }
@@ -217,16 +259,13 @@ void Layout(tfqx_plan* p) {
   put(p->off_passes, g.passes.data(), g.passes.size() * sizeof(DevPass));
 }
 
-int LdsLimit(int mode) {
-  return BatchBuffers(mode) == 2 ? kLdsMaxQubitsTwoBuffers : kLdsMaxQubitsOneBuffer;
-}
-
 int BuildPlan(int mode, Inputs& in, int64_t B, int64_t S, int64_t O, int64_t K,
               const float* downstream, bool tfq_fd, tfqx_plan** out, char* err, int64_t len,
               bool with_expectation = false) {
   auto p = std::make_unique<tfqx_plan>();
   p->mode = mode;
   p->with_expectation = with_expectation;
+  p->tfq_fd = tfq_fd;
   p->B = B;
   p->S = S;
   p->O = O;
@@ -252,14 +291,21 @@ int BuildPlan(int mode, Inputs& in, int64_t B, int64_t S, int64_t O, int64_t K,
   }
   std::vector<CircuitJob> jobs(B);
   for (int64_t i = 0; i < B; ++i) {
-    jobs[i].circuit = &in.circuits[i];
+    jobs[i].circuit = &in.circuits[in.shared ? 0 : i];
     jobs[i].sums = in.sums[i].empty() ? nullptr : &in.sums[i];
     jobs[i].downstream = downstream ? downstream + size_t(i) * O : nullptr;
     jobs[i].others = K > 0 ? &in.others[i] : nullptr;
     jobs[i].row = int(i);
   }
   const int slots = mode == kModeInnerProduct ? int(K) : int(O);
-  BuildBatchProgram(jobs, slots, int(S), opt, HostThreads(), &p->prog);
+  if (in.shared) {
+    p->shared = true;
+    p->tmpl = in.circuits[0];
+    BuildSharedBatchProgram(p->tmpl, jobs, in.values, int(S), slots, opt, HostThreads(), &p->prog,
+                            &p->recipe);
+  } else {
+    BuildBatchProgram(jobs, slots, int(S), opt, HostThreads(), &p->prog);
+  }
   p->lds_log2 = std::max(1, std::min(p->max_nq, resident));
   p->kmax = 2;
   for (const auto& o : p->prog.ops) p->kmax = std::max<int>(p->kmax, o.k);
@@ -269,6 +315,38 @@ int BuildPlan(int mode, Inputs& in, int64_t B, int64_t S, int64_t O, int64_t K,
 }
 
 hipStream_t AsStream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Real floating-point operations one launch of a batch program performs, by
+// the op's algorithm (not the instruction count): a dense k-qubit op is a
+// 2^k x 2^k complex matrix-vector product per 2^k amplitudes, i.e.
+// (8 * 2^k - 2) flops per amplitude; a diagonal op 6; a Pauli term's
+// XOR-mask reduction or lambda accumulation 8.  Adjoint sweeps un-apply each
+// op on psi and lambda; a parameterised op also applies dG (per symbol) and
+// accumulates <lambda|dG psi>.
+double AlgorithmicFlops(const BatchProgram& g) {
+  double f = 0;
+  auto op_flops = [](const DevOp& o) {
+    return o.type == kOpDiag ? 6.0 : 8.0 * double(1 << o.k) - 2.0;
+  };
+  for (const auto& c : g.circuits) {
+    if (c.flags & kFlagEmpty) continue;
+    const double A = std::ldexp(1.0, c.n);
+    for (int i = c.fwd_begin; i < c.fwd_end; ++i) f += A * op_flops(g.ops[size_t(i)]);
+    for (int i = c.rev_begin; i < c.rev_end; ++i) {
+      const DevOp& o = g.ops[size_t(i)];
+      f += 2 * A * op_flops(o);
+      if (o.type == kOpParam) f += o.nderiv * A * (op_flops(o) + 8.0);
+    }
+    f += A * 8.0 * double(c.term_end - c.term_begin);
+    f += A * 8.0 * double(c.lam_end - c.lam_begin);
+    for (int k = c.other_begin; k < c.other_end; ++k) {
+      const DevOther& o = g.others[size_t(k)];
+      for (int i = o.op_begin; i < o.op_end; ++i) f += A * op_flops(g.ops[size_t(i)]);
+      f += A * 8.0;
+    }
+  }
+  return f;
+}
 
 BatchArgs MakeArgs(tfqx_plan* p, void* ws, void* output) {
   uint8_t* base = static_cast<uint8_t*>(ws);
@@ -344,7 +422,7 @@ int tfqx_plan_simulate_expectation(const uint8_t* programs, const int64_t* poff,
                                    int64_t len) {
   Inputs in;
   Status s = Prepare(programs, poff, B, names, noff, S, values, psums, psoff, O, nullptr,
-                     nullptr, 0, &in);
+                     nullptr, 0, &in, LdsLimit(kModeExpectation));
   if (!s.ok()) return Fail(err, len, s.code, s.msg);
   return BuildPlan(kModeExpectation, in, B, S, O, 0, nullptr, true, plan, err, len);
 }
@@ -361,7 +439,7 @@ int tfqx_plan_simulate_sampled_expectation(const uint8_t* programs, const int64_
                   "Each element of num_samples must be greater than 0.");
   Inputs in;
   Status s = Prepare(programs, poff, B, names, noff, S, values, psums, psoff, O, nullptr,
-                     nullptr, 0, &in);
+                     nullptr, 0, &in, LdsLimit(kModeSampledExpectation));
   if (!s.ok()) return Fail(err, len, s.code, s.msg);
   int rc = BuildPlan(kModeSampledExpectation, in, B, S, O, 0, nullptr, true, plan, err, len);
   if (rc != TFQX_OK) return rc;
@@ -384,7 +462,7 @@ int tfqx_plan_simulate_samples(const uint8_t* programs, const int64_t* poff, int
     return Fail(err, len, TFQX_INVALID_ARGUMENT, "num_samples must be non-negative.");
   Inputs in;
   Status s = Prepare(programs, poff, B, names, noff, S, values, nullptr, nullptr, 0, nullptr,
-                     nullptr, 0, &in);
+                     nullptr, 0, &in, LdsLimit(kModeSamples));
   if (!s.ok()) return Fail(err, len, s.code, s.msg);
   int rc = BuildPlan(kModeSamples, in, B, S, 0, 0, nullptr, true, plan, err, len);
   if (rc != TFQX_OK) return rc;
@@ -402,7 +480,7 @@ int tfqx_plan_simulate_state(const uint8_t* programs, const int64_t* poff, int64
                              const float* values, tfqx_plan** plan, char* err, int64_t len) {
   Inputs in;
   Status s = Prepare(programs, poff, B, names, noff, S, values, nullptr, nullptr, 0, nullptr,
-                     nullptr, 0, &in);
+                     nullptr, 0, &in, LdsLimit(kModeState));
   if (!s.ok()) return Fail(err, len, s.code, s.msg);
   return BuildPlan(kModeState, in, B, S, 0, 0, nullptr, true, plan, err, len);
 }
@@ -425,7 +503,7 @@ int tfqx_plan_adjoint_gradient(const uint8_t* programs, const int64_t* poff, int
                                tfqx_plan** plan, char* err, int64_t len) {
   Inputs in;
   Status s = Prepare(programs, poff, B, names, noff, S, values, psums, psoff, O, nullptr,
-                     nullptr, 0, &in);
+                     nullptr, 0, &in, LdsLimit(kModeAdjoint));
   if (!s.ok()) return Fail(err, len, s.code, s.msg);
   return BuildPlan(kModeAdjoint, in, B, S, O, 0, downstream, derivative == TFQX_GRAD_TFQ_FD,
                    plan, err, len);
@@ -439,15 +517,28 @@ int tfqx_plan_expectation_and_gradient(const uint8_t* programs, const int64_t* p
                                        int64_t len) {
   Inputs in;
   Status s = Prepare(programs, poff, B, names, noff, S, values, psums, psoff, O, nullptr,
-                     nullptr, 0, &in);
+                     nullptr, 0, &in, LdsLimit(kModeAdjoint));
   if (!s.ok()) return Fail(err, len, s.code, s.msg);
   return BuildPlan(kModeAdjoint, in, B, S, O, 0, downstream, derivative == TFQX_GRAD_TFQ_FD,
                    plan, err, len, /*with_expectation=*/true);
 }
 
+int tfqx_plan_rebind(tfqx_plan* p, const float* values, char* err, int64_t len) {
+  if (!p || (!values && p->S > 0)) return Fail(err, len, TFQX_INVALID_ARGUMENT, "null argument");
+  if (!p->shared)
+    return Fail(err, len, TFQX_INVALID_ARGUMENT,
+                "Plan cannot be rebound: its programs are not one LDS-resident program tiled over "
+                "the batch.");
+  RebindSharedBatchProgram(p->tmpl, p->recipe, values, int(p->S), p->tfq_fd, HostThreads(),
+                           &p->prog);
+  if (!p->prog.mats.empty())
+    memcpy(p->staging.data() + p->off_mats, p->prog.mats.data(), p->prog.mats.size() * sizeof(cfloat));
+  return TFQX_OK;
+}
+
 int tfqx_plan_stats(const tfqx_plan* p, int64_t* out, int64_t n) {
   if (!p || !out) return TFQX_INVALID_ARGUMENT;
-  int64_t v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int64_t v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   v[3] = p->max_nq;
   if (p->large) {
     const LargePlan& lp = *p->large_plan;
@@ -465,9 +556,10 @@ int tfqx_plan_stats(const tfqx_plan* p, int64_t* out, int64_t n) {
       v[1] += c.fwd_end - c.fwd_begin;
       v[4] += c.rev_end - c.rev_begin;
     }
+    v[8] = int64_t(AlgorithmicFlops(p->prog));
   }
   v[7] = int64_t(p->total);
-  for (int64_t i = 0; i < n && i < 8; ++i) out[i] = v[i];
+  for (int64_t i = 0; i < n && i < 9; ++i) out[i] = v[i];
   return TFQX_OK;
 }
 

@@ -465,6 +465,15 @@ Status BuildCircuit(const Program& prog, const SymbolMap& sym, Circuit* out) {
   return Status();
 }
 
+void RebindCircuit(const Circuit& tmpl, const float* values, Circuit* out) {
+  *out = tmpl;
+  for (auto& g : out->gates) {
+    if (g.nsym == 0) continue;
+    for (int j = 0; j < g.nsym; ++j) g.params[g.sym_pidx[j]] = values[g.sym_col[j]];
+    GateUnitary(g.kind, g.params, g.U);
+  }
+}
+
 Status BuildPairedCircuit(const Program& prog, const Circuit& reference, Circuit* out) {
   out->gates.clear();
   out->num_qubits = reference.num_qubits;

@@ -66,6 +66,9 @@ struct Circuit {
 
 // Parse + resolve one program.  `sym` may have no columns (empty map).
 Status BuildCircuit(const Program& prog, const SymbolMap& sym, Circuit* out);
+// `tmpl` with its symbolic arguments re-read from another row of values
+// (same program bytes, so the same gates, qubits and symbol columns).
+void RebindCircuit(const Circuit& tmpl, const float* values, Circuit* out);
 // Paired resolution for inner products: other programs use the reference
 // program's qubit map and must cover exactly its qubits.
 Status BuildPairedCircuit(const Program& prog, const Circuit& reference, Circuit* out);

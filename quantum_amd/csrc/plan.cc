@@ -78,26 +78,98 @@ void MaskToBits(uint64_t m, int* bits, int* k) {
     if (m >> b & 1) bits[(*k)++] = b;
 }
 
-FusedOp BlockToOp(const Circuit& c, const std::vector<int>& gates, uint64_t qmask) {
+bool GatesDiagonal(const Circuit& c, const std::vector<int>& gates) {
+  for (int gi : gates)
+    if (!KindDiagonal(c.gates[gi].kind)) return false;
+  return true;
+}
+
+// acc <- G * acc for one (controlled) gate acting inside a k-bit block
+// (`bits` ascending, little-endian block index; controls must lie in `bits`).
+// Works on the udim-row groups the gate mixes, with explicit complex
+// arithmetic (std::complex products take the slow C99 Annex G path).
+void ApplyLeft(const Gate& g, const int* bits, int k, double* acc /* re,im interleaved */) {
+  const int dim = 1 << k;
+  int upos[2];
+  for (int j = 0; j < g.nt; ++j)
+    for (int m = 0; m < k; ++m)
+      if (bits[m] == g.bits[j]) upos[j] = m;
+  int cm = 0, cv = 0;
+  for (int m = 0; m < k; ++m)
+    if (g.cmask >> bits[m] & 1) {
+      cm |= 1 << m;
+      if (g.cval >> bits[m] & 1) cv |= 1 << m;
+    }
+  const int udim = 1 << g.nt;
+  int gmask = 0;
+  for (int j = 0; j < g.nt; ++j) gmask |= 1 << upos[j];
+  int rows[4];
+  double ur[16], ui[16];
+  for (int i = 0; i < udim * udim; ++i) {
+    ur[i] = g.U[i].real();
+    ui[i] = g.U[i].imag();
+  }
+  for (int r0 = 0; r0 < dim; ++r0) {
+    if (r0 & gmask) continue;
+    if ((r0 & cm) != cv) continue;
+    // rows[s]: sub-index s (MSB = g.bits[0]) placed on the gate's positions
+    for (int sidx = 0; sidx < udim; ++sidx) {
+      int r = r0;
+      for (int j = 0; j < g.nt; ++j)
+        if (sidx >> (g.nt - 1 - j) & 1) r |= 1 << upos[j];
+      rows[sidx] = r;
+    }
+    for (int c = 0; c < dim; ++c) {
+      double vr[4], vi[4];
+      for (int sidx = 0; sidx < udim; ++sidx) {
+        vr[sidx] = acc[2 * (rows[sidx] * dim + c)];
+        vi[sidx] = acc[2 * (rows[sidx] * dim + c) + 1];
+      }
+      for (int a = 0; a < udim; ++a) {
+        double sr = 0, si = 0;
+        for (int b = 0; b < udim; ++b) {
+          const double xr = ur[a * udim + b], xi = ui[a * udim + b];
+          sr += xr * vr[b] - xi * vi[b];
+          si += xr * vi[b] + xi * vr[b];
+        }
+        acc[2 * (rows[a] * dim + c)] = sr;
+        acc[2 * (rows[a] * dim + c) + 1] = si;
+      }
+    }
+  }
+}
+
+// Product (application order) of the gates' embedded matrices over `bits`.
+void BlockMatrix(const std::vector<Gate>& gs, const int* gates, int ngates, const int* bits, int k,
+                 cdouble* out) {
+  const int dim = 1 << k;
+  double acc[2 * 256];
+  for (int i = 0; i < 2 * dim * dim; ++i) acc[i] = 0;
+  for (int i = 0; i < dim; ++i) acc[2 * (i * dim + i)] = 1.0;
+  for (int t = 0; t < ngates; ++t) ApplyLeft(gs[gates[t]], bits, k, acc);
+  for (int i = 0; i < dim * dim; ++i) out[i] = cdouble(acc[2 * i], acc[2 * i + 1]);
+}
+
+FusedOp BlockToOp(const Circuit& c, const std::vector<int>& gates, uint64_t qmask,
+                  bool structural_diag) {
   FusedOp op;
   MaskToBits(qmask, op.bits, &op.k);
   const int dim = 1 << op.k;
-  std::vector<cdouble> acc(dim * dim, 0.0), e(dim * dim), tmp(dim * dim);
-  for (int i = 0; i < dim; ++i) acc[i * dim + i] = 1.0;
-  for (int gi : gates) {
-    const Gate& g = c.gates[gi];
-    Embed(g.U, g.bits, g.nt, g.cmask, g.cval, op.bits, op.k, e.data());
-    MatMul(e.data(), acc.data(), dim, tmp.data());
-    acc.swap(tmp);
-  }
-  op.M = std::move(acc);
-  op.diag = IsDiagonal(op.M, dim);
+  op.M.assign(size_t(dim) * dim, 0.0);
+  BlockMatrix(c.gates, gates.data(), int(gates.size()), op.bits, op.k, op.M.data());
+  op.diag = structural_diag ? GatesDiagonal(c, gates) : IsDiagonal(op.M, dim);
+  op.gates = gates;
   return op;
 }
 
 }  // namespace
 
-FusedOp GateToOp(const Gate& g, bool keep_controls_outside) {
+bool KindDiagonal(GateKind kind) {
+  return kind == kGateI1 || kind == kGateI2 || kind == kGateZ || kind == kGateZZ ||
+         kind == kGateCZ;
+}
+
+FusedOp GateToOp(const Gate& g, bool keep_controls_outside, bool structural_diag) {
   FusedOp op;
   uint64_t m = 0;
   for (int j = 0; j < g.nt; ++j) m |= uint64_t(1) << g.bits[j];
@@ -112,7 +184,7 @@ FusedOp GateToOp(const Gate& g, bool keep_controls_outside) {
   } else {
     Embed(g.U, g.bits, g.nt, g.cmask, g.cval, op.bits, op.k, op.M.data());
   }
-  op.diag = IsDiagonal(op.M, dim);
+  op.diag = structural_diag ? KindDiagonal(g.kind) : IsDiagonal(op.M, dim);
   return op;
 }
 
@@ -132,7 +204,7 @@ void FuseGates(const Circuit& c, int begin, int end, const FuseOptions& opt,
     b.open = false;
     for (int q = 0; q < 64; ++q)
       if (b.qmask >> q & 1) owner[q] = -1;
-    out->push_back(BlockToOp(c, b.gates, b.qmask));
+    out->push_back(BlockToOp(c, b.gates, b.qmask, opt.structural_diag));
   };
   for (int gi = begin; gi < end; ++gi) {
     const Gate& g = c.gates[gi];
@@ -146,7 +218,8 @@ void FuseGates(const Circuit& c, int begin, int end, const FuseOptions& opt,
     if (Popc(Q) > opt.max_fuse) {
       // Unfusable: flush what it touches, emit alone with controls outside.
       for (int bi : touching) close(bi);
-      out->push_back(GateToOp(g, true));
+      out->push_back(GateToOp(g, true, opt.structural_diag));
+      out->back().gates.assign(1, gi);
       continue;
     }
     uint64_t U = Q;
@@ -206,7 +279,18 @@ uint32_t PushMatrix(std::vector<cfloat>* mats, const std::vector<cdouble>& M, bo
   return off;
 }
 
-DevOp MakeDevOp(const FusedOp& f, std::vector<cfloat>* mats) {
+void WriteMatrix(cfloat* dst, const cdouble* M, bool diag_only, int dim) {
+  if (diag_only) {
+    for (int i = 0; i < dim; ++i)
+      dst[i] = cfloat(float(M[i * dim + i].real()), float(M[i * dim + i].imag()));
+  } else {
+    for (int i = 0; i < dim * dim; ++i) dst[i] = cfloat(float(M[i].real()), float(M[i].imag()));
+  }
+}
+
+// `c` and `rec` (both or neither): record how to rebuild this op's matrices.
+DevOp MakeDevOp(const FusedOp& f, std::vector<cfloat>* mats, const Circuit* c = nullptr,
+                SharedRecipe* rec = nullptr) {
   DevOp d{};
   d.k = uint8_t(f.k);
   for (int j = 0; j < f.k; ++j) d.q[j] = uint8_t(f.bits[j]);
@@ -224,6 +308,23 @@ DevOp MakeDevOp(const FusedOp& f, std::vector<cfloat>* mats) {
   } else {
     d.type = f.diag ? kOpDiag : kOpUnitary;
     d.mat = PushMatrix(mats, f.M, f.diag, dim);
+  }
+  if (rec) {
+    OpRecipe r;
+    r.kind = f.param ? 2 : (f.cmask ? 1 : 0);
+    r.k = uint8_t(f.k);
+    r.diag = !f.param && f.diag;
+    for (int j = 0; j < f.k; ++j) r.bits[j] = f.bits[j];
+    r.gate_begin = int(rec->gates.size());
+    for (int gi : f.gates) {
+      rec->gates.push_back(gi);
+      if (c->gates[gi].nsym > 0) r.symbolic = true;
+    }
+    r.gate_end = int(rec->gates.size());
+    r.mat = d.mat;
+    r.nderiv = f.param ? f.nderiv : 0;
+    for (int i = 0; i < r.nderiv; ++i) r.dmat[i] = d.dmat[i];
+    rec->ops.push_back(r);
   }
   return d;
 }
@@ -248,31 +349,15 @@ DevTerm MakeTerm(const Pauli& p, float w, int slot, bool phase_y = true, int ind
   return t;
 }
 
-void BuildPiece(const CircuitJob& job, int num_slots, const PlanOptions& opt, Piece* pc) {
-  const Circuit& c = *job.circuit;
+// Expectation terms (ComputeExpectationQsim, util_qsim.h:152-186; sampled:
+// ComputeSampledExpectationQsim, util_qsim.h:199-270) and, for the adjoint, the
+// lambda terms (AccumulateOperators, util_qsim.h:363-414: w = g_j * c_t in
+// float32, terms with |w| < 1e-5 dropped).  Returns their cost.
+double AddTerms(const CircuitJob& job, int num_slots, const PlanOptions& opt, Piece* pc) {
   DevCircuit& dc = pc->dc;
-  dc = DevCircuit{};
-  dc.n = c.num_qubits;
-  dc.row = job.row;
+  const bool empty = (dc.flags & kFlagEmpty) != 0;
+  pc->terms.clear();
   pc->ident.assign(num_slots, 0.0f);
-  const bool empty = c.gates.empty();
-  if (empty) dc.flags |= kFlagEmpty;
-  // Forward program.
-  std::vector<FusedOp> fused;
-  if (!empty) FuseGates(c, 0, int(c.gates.size()), opt.fuse, &fused);
-  const bool tiled = !empty && dc.n > opt.tile_log2;
-  if (tiled) dc.flags |= kFlagTiled;
-  dc.fwd_begin = 0;
-  dc.fwd_pass_begin = 0;
-  if (tiled) {
-    SchedulePasses(fused, dc.n, opt.tile_log2, &pc->passes, &pc->ops, &pc->mats);
-  } else {
-    for (const auto& f : fused) pc->ops.push_back(MakeDevOp(f, &pc->mats));
-  }
-  dc.fwd_end = int(pc->ops.size());
-  dc.fwd_pass_end = int(pc->passes.size());
-  // Expectation terms (ComputeExpectationQsim, util_qsim.h:152-186; sampled:
-  // ComputeSampledExpectationQsim, util_qsim.h:199-270).
   dc.term_begin = 0;
   const bool sampled = opt.mode == kModeSampledExpectation;
   if (!empty && job.sums) {
@@ -290,14 +375,7 @@ void BuildPiece(const CircuitJob& job, int num_slots, const PlanOptions& opt, Pi
   }
   dc.term_end = int(pc->terms.size());
   dc.lam_begin = dc.lam_end = dc.term_end;
-  dc.rev_begin = dc.rev_end = dc.fwd_end;
-  double cost = 0;
-  const double amps = std::ldexp(1.0, std::max(dc.n, 0));
-  for (const auto& f : fused) cost += amps * (f.diag ? 1 : (1 << f.k)) + 2048.0;
-  cost += amps * (dc.term_end - dc.term_begin);
   if (opt.adjoint && !empty) {
-    // Lambda terms (AccumulateOperators, util_qsim.h:363-414): w = g_j * c_t in
-    // float32, terms with |w| < 1e-5 dropped.
     for (int j = 0; j < num_slots; ++j) {
       const float gj = job.downstream[j];
       for (const auto& p : (*job.sums)[j]) {
@@ -307,7 +385,41 @@ void BuildPiece(const CircuitJob& job, int num_slots, const PlanOptions& opt, Pi
       }
     }
     dc.lam_end = int(pc->terms.size());
-    cost += amps * (dc.lam_end - dc.lam_begin);
+  }
+  const double amps = std::ldexp(1.0, std::max(dc.n, 0));
+  return amps * double(pc->terms.size());
+}
+
+// `rec`: record the matrix recipes (LDS-resident circuits only).
+void BuildPiece(const CircuitJob& job, int num_slots, const PlanOptions& opt, Piece* pc,
+                SharedRecipe* rec = nullptr) {
+  const Circuit& c = *job.circuit;
+  DevCircuit& dc = pc->dc;
+  dc = DevCircuit{};
+  dc.n = c.num_qubits;
+  dc.row = job.row;
+  const bool empty = c.gates.empty();
+  if (empty) dc.flags |= kFlagEmpty;
+  const Circuit* rc = rec ? &c : nullptr;
+  // Forward program.
+  std::vector<FusedOp> fused;
+  if (!empty) FuseGates(c, 0, int(c.gates.size()), opt.fuse, &fused);
+  const bool tiled = !empty && dc.n > opt.tile_log2;
+  if (tiled) dc.flags |= kFlagTiled;
+  dc.fwd_begin = 0;
+  dc.fwd_pass_begin = 0;
+  if (tiled) {
+    SchedulePasses(fused, dc.n, opt.tile_log2, &pc->passes, &pc->ops, &pc->mats);
+  } else {
+    for (const auto& f : fused) pc->ops.push_back(MakeDevOp(f, &pc->mats, rc, rec));
+  }
+  dc.fwd_end = int(pc->ops.size());
+  dc.fwd_pass_end = int(pc->passes.size());
+  double cost = AddTerms(job, num_slots, opt, pc);
+  dc.rev_begin = dc.rev_end = dc.fwd_end;
+  const double amps = std::ldexp(1.0, std::max(dc.n, 0));
+  for (const auto& f : fused) cost += amps * (f.diag ? 1 : (1 << f.k)) + 2048.0;
+  if (opt.adjoint && !empty) {
     // Reverse program: segments between parameterised gates, last first.
     std::vector<int> pg;
     for (int i = 0; i < int(c.gates.size()); ++i)
@@ -326,7 +438,8 @@ void BuildPiece(const CircuitJob& job, int num_slots, const PlanOptions& opt, Pi
       }
       if (p == 0) break;
       const Gate& g = c.gates[pg[p - 1]];
-      FusedOp f = GateToOp(g, true);
+      FusedOp f = GateToOp(g, true, opt.fuse.structural_diag);
+      f.gates.assign(1, pg[p - 1]);
       f.param = true;
       f.nderiv = g.nsym;
       for (int d = 0; d < g.nsym; ++d) {
@@ -343,7 +456,7 @@ void BuildPiece(const CircuitJob& job, int num_slots, const PlanOptions& opt, Pi
     if (tiled) {
       SchedulePasses(rev_ops, dc.n, opt.tile_log2, &pc->passes, &pc->ops, &pc->mats);
     } else {
-      for (const auto& f : rev_ops) pc->ops.push_back(MakeDevOp(f, &pc->mats));
+      for (const auto& f : rev_ops) pc->ops.push_back(MakeDevOp(f, &pc->mats, rc, rec));
     }
     dc.rev_end = int(pc->ops.size());
     dc.rev_pass_end = int(pc->passes.size());
@@ -370,34 +483,67 @@ void BuildPiece(const CircuitJob& job, int num_slots, const PlanOptions& opt, Pi
   pc->cost = cost;
 }
 
-}  // namespace
-
-double CircuitCost(const DevCircuit& c, const BatchProgram& p) {
-  (void)p;
-  return std::ldexp(1.0, c.n) * double((c.fwd_end - c.fwd_begin) + (c.rev_end - c.rev_begin));
+// Gates of `tmpl` with one row's symbol values (ParseArg resolves a symbolic
+// argument to the row's value; the unitary follows from the arguments).
+void RowGates(const Circuit& tmpl, const float* values, std::vector<Gate>* gates) {
+  *gates = tmpl.gates;
+  for (auto& g : *gates) {
+    if (g.nsym == 0) continue;
+    for (int j = 0; j < g.nsym; ++j) g.params[g.sym_pidx[j]] = values[g.sym_col[j]];
+    GateUnitary(g.kind, g.params, g.U);
+  }
 }
 
-void BuildBatchProgram(const std::vector<CircuitJob>& jobs, int num_slots, int num_symbols,
-                       const PlanOptions& opt, int threads, BatchProgram* out) {
-  const int B = int(jobs.size());
-  std::vector<Piece> pieces(B);
+// Rewrite the matrices of symbolic ops for one row (others keep `base`'s).
+void RebindRow(const std::vector<Gate>& gs, const SharedRecipe& rec, bool tfq_fd, cfloat* mats) {
+  cdouble M[256];
+  for (const auto& r : rec.ops) {
+    if (!r.symbolic) continue;
+    const int dim = 1 << r.k;
+    const int* gl = rec.gates.data() + r.gate_begin;
+    const int ng = r.gate_end - r.gate_begin;
+    if (r.kind == 0) {
+      BlockMatrix(gs, gl, ng, r.bits, r.k, M);
+      WriteMatrix(mats + r.mat, M, r.diag, dim);
+      continue;
+    }
+    const Gate& g = gs[gl[0]];
+    Embed(g.U, g.bits, g.nt, 0, 0, r.bits, r.k, M);
+    WriteMatrix(mats + r.mat, M, r.diag, dim);
+    for (int d = 0; d < r.nderiv; ++d) {
+      cdouble dU[16];
+      GateDerivative(g.kind, g.params, g.sym_pidx[d], tfq_fd, dU);
+      Embed(dU, g.bits, g.nt, 0, 0, r.bits, r.k, M);
+      WriteMatrix(mats + r.dmat[d], M, false, dim);
+    }
+  }
+}
+
+template <typename F>
+void RunParallel(int n, int threads, F f) {
+  threads = std::max(1, std::min(threads, n));
   std::atomic<int> next(0);
   auto work = [&]() {
     while (true) {
       int i = next.fetch_add(1);
-      if (i >= B) break;
-      BuildPiece(jobs[i], num_slots, opt, &pieces[i]);
+      if (i >= n) break;
+      f(i);
     }
   };
-  threads = std::max(1, std::min(threads, B));
   if (threads == 1) {
     work();
-  } else {
-    std::vector<std::thread> pool;
-    for (int t = 0; t < threads; ++t) pool.emplace_back(work);
-    for (auto& th : pool) th.join();
+    return;
   }
-  // Longest-first launch order (LPT) so the persistent kernel's tail is short.
+  std::vector<std::thread> pool;
+  for (int t = 0; t < threads; ++t) pool.emplace_back(work);
+  for (auto& th : pool) th.join();
+}
+
+// Concatenate pieces in longest-first order (LPT) so the persistent kernel's
+// tail is short.
+void Concatenate(std::vector<Piece>& pieces, int num_slots, int num_symbols, const PlanOptions& opt,
+                 BatchProgram* out) {
+  const int B = int(pieces.size());
   std::vector<int> order(B);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(),
@@ -409,6 +555,7 @@ void BuildBatchProgram(const std::vector<CircuitJob>& jobs, int num_slots, int n
   out->ops.clear();
   out->mats.clear();
   out->terms.clear();
+  out->row_mat_base.assign(size_t(B), 0);
   out->ident.assign(size_t(B) * num_slots, 0.0f);
   out->num_rows = B;
   out->num_slots = num_slots;
@@ -430,6 +577,7 @@ void BuildBatchProgram(const std::vector<CircuitJob>& jobs, int num_slots, int n
     const uint32_t mat_base = uint32_t(out->mats.size());
     const int term_base = int(out->terms.size());
     DevCircuit dc = p.dc;
+    out->row_mat_base[size_t(dc.row)] = mat_base;
     const int pass_base = int(out->passes.size());
     for (DevPass ps : p.passes) {
       ps.op_begin += op_base;
@@ -461,6 +609,61 @@ void BuildBatchProgram(const std::vector<CircuitJob>& jobs, int num_slots, int n
     out->circuits.push_back(dc);
     out->max_n = std::max(out->max_n, dc.n);
   }
+}
+
+}  // namespace
+
+double CircuitCost(const DevCircuit& c, const BatchProgram& p) {
+  (void)p;
+  return std::ldexp(1.0, c.n) * double((c.fwd_end - c.fwd_begin) + (c.rev_end - c.rev_begin));
+}
+
+void BuildBatchProgram(const std::vector<CircuitJob>& jobs, int num_slots, int num_symbols,
+                       const PlanOptions& opt, int threads, BatchProgram* out) {
+  const int B = int(jobs.size());
+  std::vector<Piece> pieces(B);
+  RunParallel(B, threads, [&](int i) { BuildPiece(jobs[i], num_slots, opt, &pieces[i]); });
+  Concatenate(pieces, num_slots, num_symbols, opt, out);
+}
+
+void BuildSharedBatchProgram(const Circuit& tmpl, const std::vector<CircuitJob>& jobs,
+                             const float* values, int num_symbols, int num_slots,
+                             const PlanOptions& opt_in, int threads, BatchProgram* out,
+                             SharedRecipe* recipe) {
+  PlanOptions opt = opt_in;
+  opt.fuse.structural_diag = true;
+  const int B = int(jobs.size());
+  *recipe = SharedRecipe();
+  Piece base;
+  CircuitJob j0 = jobs[0];
+  j0.circuit = &tmpl;
+  BuildPiece(j0, num_slots, opt, &base, recipe);
+  recipe->mats_per_row = base.mats.size();
+  std::vector<Piece> pieces(B);
+  RunParallel(B, threads, [&](int i) {
+    Piece& pc = pieces[i];
+    pc.dc = base.dc;
+    pc.dc.row = jobs[i].row;
+    pc.ops = base.ops;
+    pc.mats = base.mats;
+    std::vector<Gate> gs;
+    RowGates(tmpl, values + size_t(jobs[i].row) * num_symbols, &gs);
+    RebindRow(gs, *recipe, opt.tfq_fd, pc.mats.data());
+    const double term_cost = AddTerms(jobs[i], num_slots, opt, &pc);
+    pc.cost = base.cost - std::ldexp(1.0, std::max(base.dc.n, 0)) * double(base.terms.size()) +
+              term_cost;
+  });
+  Concatenate(pieces, num_slots, num_symbols, opt, out);
+}
+
+void RebindSharedBatchProgram(const Circuit& tmpl, const SharedRecipe& recipe, const float* values,
+                              int num_symbols, bool tfq_fd, int threads, BatchProgram* prog) {
+  const int B = prog->num_rows;
+  RunParallel(B, threads, [&](int row) {
+    std::vector<Gate> gs;
+    RowGates(tmpl, values + size_t(row) * num_symbols, &gs);
+    RebindRow(gs, recipe, tfq_fd, prog->mats.data() + prog->row_mat_base[size_t(row)]);
+  });
 }
 
 }  // namespace tfqx

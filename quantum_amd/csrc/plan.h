@@ -32,10 +32,35 @@ struct FusedOp {
   int nderiv = 0;
   std::vector<cdouble> dM[kMaxDeriv];
   int col[kMaxDeriv];
+  // Gates (circuit indices, application order) the matrix is the product of.
+  std::vector<int> gates;
 };
 
 struct FuseOptions {
   int max_fuse = 2;   // qubits per fused block
+  // Decide diagonal storage from the gate kinds only (Z, ZZ, CZ, I) instead of
+  // the numeric matrix, so the op list does not depend on symbol values.
+  bool structural_diag = false;
+};
+
+// How one device op's matrices are rebuilt from its gates for another row
+// that shares the program (only symbol values differ).
+struct OpRecipe {
+  uint8_t kind = 0;      // 0 fused block, 1 single gate (controls outside), 2 parameterised
+  uint8_t k = 0;
+  bool diag = false;
+  bool symbolic = false; // contains a symbolic gate (else the matrix is value-independent)
+  int bits[kMaxFuse];
+  int gate_begin = 0, gate_end = 0;  // range in SharedRecipe::gates
+  uint32_t mat = 0;                  // offsets into the row's matrices
+  uint32_t dmat[kMaxDeriv];
+  int nderiv = 0;
+};
+
+struct SharedRecipe {
+  std::vector<OpRecipe> ops;
+  std::vector<int> gates;
+  size_t mats_per_row = 0;
 };
 
 // Fuse gates [begin, end) of a circuit into blocks (deterministic greedy
@@ -47,7 +72,9 @@ void FuseGates(const Circuit& c, int begin, int end, const FuseOptions& opt,
                std::vector<FusedOp>* out);
 
 // Expand one gate into kernel convention (sorted bits, little-endian index).
-FusedOp GateToOp(const Gate& g, bool keep_controls_outside);
+FusedOp GateToOp(const Gate& g, bool keep_controls_outside, bool structural_diag = false);
+// Gate kinds whose matrix is diagonal for every argument value.
+bool KindDiagonal(GateKind kind);
 
 // A batch program for the LDS-resident kernel.
 struct BatchProgram {
@@ -58,6 +85,7 @@ struct BatchProgram {
   std::vector<cfloat> mats;
   std::vector<DevTerm> terms;
   std::vector<float> ident;           // [B * O] identity-term constants
+  std::vector<uint32_t> row_mat_base; // shared programs: row -> first matrix of the row
   int max_n = 0;
   int num_rows = 0, num_slots = 0, num_symbols = 0;
   bool adjoint = false;
@@ -83,6 +111,18 @@ struct PlanOptions {
 // Build the per-circuit program pieces and concatenate them.
 void BuildBatchProgram(const std::vector<CircuitJob>& jobs, int num_slots, int num_symbols,
                        const PlanOptions& opt, int threads, BatchProgram* out);
+
+// Rows whose programs are byte-identical: `tmpl` is the program resolved with
+// row 0's values; the op list is built once and each row re-evaluates only the
+// matrices of ops that contain symbolic gates (from values[row * S ...]).
+// Requires tmpl.num_qubits <= opt.tile_log2 (LDS-resident).
+void BuildSharedBatchProgram(const Circuit& tmpl, const std::vector<CircuitJob>& jobs,
+                             const float* values, int num_symbols, int num_slots,
+                             const PlanOptions& opt, int threads, BatchProgram* out,
+                             SharedRecipe* recipe);
+// Re-evaluate every row's matrices of a shared program for new values.
+void RebindSharedBatchProgram(const Circuit& tmpl, const SharedRecipe& recipe, const float* values,
+                              int num_symbols, bool tfq_fd, int threads, BatchProgram* prog);
 
 // Cost estimate used for longest-first ordering and the autotuner.
 double CircuitCost(const DevCircuit& c, const BatchProgram& p);

@@ -124,3 +124,24 @@ def test_channel_gate_id_rejected():
     with pytest.raises(InvalidArgumentError, match="Could not parse gate id: DP"):
         Plan(_lib.lib().tfqx_plan_simulate_expectation, P.ptr, P.off, 1, N.ptr, N.off, 0, None,
              Ps.ptr, Ps.off, 1)
+
+
+def test_rebind_only_for_one_program_tiled_over_the_batch():
+    circuit, names, q = W.hea_ring(6, 2)
+    ham = W.tfim(q, 0.7)
+    rng = np.random.default_rng(0)
+    vals = rng.uniform(-3, 3, (5, len(names)))
+    shared = _plan_expectation([circuit] * 5, names, vals, [[ham]] * 5)
+    stats = shared.stats()
+    shared.rebind(rng.uniform(-3, 3, (5, len(names))))
+    assert shared.stats() == stats
+    other = W.hea_ring(6, 1)[0]
+    mixed = _plan_expectation([circuit, other] * 2, names, vals[:4], [[ham]] * 4)
+    with pytest.raises(InvalidArgumentError, match="cannot be rebound"):
+        mixed.rebind(vals[:4])
+    # Same program but wider than one workgroup's LDS: planned per row, not rebindable.
+    wide, wnames, wq = W.hea_ring(16, 1)
+    big = _plan_expectation([wide] * 2, wnames, rng.uniform(-3, 3, (2, len(wnames))),
+                            [[W.tfim(wq, 1.0)]] * 2)
+    with pytest.raises(InvalidArgumentError, match="cannot be rebound"):
+        big.rebind(np.zeros((2, len(wnames))))

@@ -356,3 +356,41 @@ def test_streaming_path_19_qubits():
     ref_g = O.adjoint_gradient(list(progs), names, vals, ps, g)
     np.testing.assert_allclose(_np(grad), ref_g, atol=TOL * max(1, np.abs(ref_g).max()))
     np.testing.assert_allclose(_np(exp), ref, atol=TOL * max(1, np.abs(ref).max()))
+
+
+# ----------------------------------------------------------------------------- shared programs
+@pytest.mark.parametrize("derivative", ["tfq", "analytic"])
+def test_rebound_plan_matches_oracle_and_fresh_plan(derivative):
+    """One program tiled over the batch: the plan is built once (structural
+    diagonal detection, per-row matrices) and rebound to new values."""
+    import torch
+    _, adj, _ = ops()
+    circuit, names, q = W.hea_ring(8, 3)
+    circuit = circuit + C.Circuit([C.Z(q[0]) ** C.Symbol(names[0]), C.CZ(q[1], q[2])])
+    ham = W.tfim(q, 0.6)
+    rng = np.random.default_rng(11)
+    B = 6
+    progs = _ser([circuit] * B)
+    ps = _ser([[ham]] * B)
+    v1 = rng.uniform(-3, 3, (B, len(names))).astype(np.float32)
+    v1[0, :] = 0.0  # ry(0) is numerically diagonal; the shared plan keeps it dense
+    v2 = rng.uniform(-3, 3, (B, len(names))).astype(np.float32)
+    dg = rng.uniform(-1, 1, (B, 1)).astype(np.float32)
+    plan, shape = adj.plan_adj_grad(progs, names, v1, ps, dg, derivative, with_expectation=True)
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    outs = []
+    for vals in (v1, v2):
+        _say(f"rebound plan derivative={derivative}")
+        plan.rebind(vals)
+        out = torch.empty(shape, dtype=torch.float32, device=dev)
+        plan.upload(dev, stream)
+        plan.launch(out, stream)
+        outs.append(_np(out))
+        e, g = adj.expectation_and_gradient(progs, names, vals, ps, dg, derivative)
+        fresh = np.concatenate([_np(e).reshape(-1), _np(g).reshape(-1)])
+        np.testing.assert_array_equal(outs[-1], fresh)
+        eo = O.simulate_expectation(list(progs), names, vals, ps)
+        go = O.adjoint_gradient(list(progs), names, vals, ps, dg, mode=derivative)
+        np.testing.assert_allclose(outs[-1][:B], eo[:, 0], atol=TOL)
+        np.testing.assert_allclose(outs[-1][B:].reshape(B, -1), go, atol=TOL)

@@ -7,9 +7,18 @@ roofline of the 30-qubit streaming passes.
 
 Prints one JSON line.  A step is one launch over the whole batch with the
 device program already resident in HBM (host parse/fuse and upload happen
-before timing).  `roofline` is the 30-qubit gate pass (config: 5x6 grid,
-14 cycles): algorithmic bytes 16 B x 2^30 per pass / average pass time,
-measured with HIP events on the launch stream.
+before timing).
+
+`roofline` is the batch kernel (the only kernel of a step): its algorithmic
+real flops per launch (from the plan: (8*2^k - 2) per amplitude per dense
+k-qubit op, 6 per diagonal op, 8 per Pauli term, twice per op in the adjoint
+sweep, plus the derivative products) divided by its average launch time,
+measured with HIP events on the launch stream, against the FP32 peak.
+`traffic` is the kernel's HBM bytes per launch from rocprofv3 FETCH_SIZE and
+WRITE_SIZE counters, read from profiles/batch_kernel_uneven4096_traffic.json
+when that file matches this workload, else null.  `hbm_pass` is the 30-qubit
+streaming gate pass (5x6 grid, 14 cycles): 16 B x 2^30 per pass / average
+pass time, against 8 TB/s.
 """
 
 import argparse
@@ -25,6 +34,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = ("expectation+adjoint-grad circuits/s, batched vs plain GPU, err≤1e-5; HBM GB/s")
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "batch_kernel_uneven4096_traffic.json")
 
 
 def log(msg):
@@ -100,9 +111,22 @@ def hbm_roofline(stream):
     del plan
     torch.cuda.empty_cache()
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
             "kernel": "pass_kernel (tile pass, 30-qubit state)", "passes": passes,
             "ms_per_pass": round(per_pass * 1e3, 4), "bytes_per_pass": bytes_per_pass}
+
+
+def measured_traffic(circuits, flops):
+    """HBM bytes per launch from the committed rocprofv3 counter summary, if it
+    was taken on this same workload (same circuit count and flop count)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("circuits") != circuits or t.get("flops") != flops:
+        return None
+    return t.get("bytes_per_launch")
 
 
 def cpu_baseline(progs, names, values, ps, weights, gpu_exp, gpu_grad, budget_s):
@@ -138,6 +162,10 @@ def main():
     ap.add_argument("--circuits", type=int, default=4096)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--only", choices=["all", "batch", "hbm", "copy"], default="all",
+                    help="profiling runs: 'batch' launches only the batched kernel (graph "
+                         "replays, no plain launches, no CPU baseline); 'hbm' only the 30-qubit "
+                         "streaming passes; 'copy' a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py measures one GPU: this build has no multi-GPU path (each box has one "
@@ -153,6 +181,17 @@ def main():
     torch.cuda.set_stream(side)
     stream = side.cuda_stream
 
+    if args.only == "copy":
+        src = torch.ones(1 << 27, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        t = time_launch(lambda: dst.copy_(src), stream, reps=3)
+        print(json.dumps({"copy_bytes_read": 1 << 30, "copy_bytes_written": 1 << 30,
+                          "ms": round(t * 1e3, 4), "GB/s": round(2 * (1 << 30) / t / 1e9, 1)}),
+              flush=True)
+        return
+    if args.only == "hbm":
+        print(json.dumps({"hbm_pass": hbm_roofline(stream)}), flush=True)
+        return
     progs, names, values, ps, weights = build_uneven(args.circuits)
     t = time.time()
     plan, shape = adj.plan_adj_grad(progs, names, values, ps, weights, with_expectation=True)
@@ -162,6 +201,7 @@ def main():
     torch.cuda.synchronize()
 
     B, O, S = len(progs), ps.shape[1], len(names)
+    flops = plan.stats()["flops"]
     elapsed, wall, graph = time_graph(plan, out, stream, args.steps, args.warmup)
     value = B * args.steps / elapsed
     log(f"batched: {elapsed / args.steps * 1e3:.3f} ms/step ({value:.0f} circuits/s)")
@@ -169,6 +209,10 @@ def main():
     gpu_exp = res[:B * O].reshape(B, O)
     gpu_grad = res[B * O:].reshape(B, S)
 
+    if args.only == "batch":
+        print(json.dumps({"batch_ms_per_launch": round(elapsed / args.steps * 1e3, 4),
+                          "flops_per_launch": flops, "circuits": B}), flush=True)
+        return
     out_plain = torch.empty_like(out)
     t_plain = time_launch(lambda: plan.launch(out_plain, stream, plain=True), stream, reps=1)
     plain_value = B / t_plain
@@ -176,10 +220,19 @@ def main():
     log(f"plain (one launch per circuit): {t_plain * 1e3:.1f} ms ({plain_value:.0f} circuits/s),"
         f" identical={same}")
 
-    roof = None
+    per_launch = elapsed / args.steps
+    tflops = flops / per_launch / 1e12
+    log(f"batch kernel: {flops / 1e12:.3f} TFLOP per launch, {tflops:.2f} TFLOP/s")
+    roof = {"bound": "mfma", "achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(tflops / FP32_PEAK_TFLOPS, 4),
+            "traffic": measured_traffic(B, flops),
+            "kernel": "batch_kernel<adjoint> (LDS-resident persistent queue, FP32 VALU)",
+            "flops_per_launch": flops, "ms_per_launch": round(per_launch * 1e3, 4),
+            "peak_note": "MI355X FP32 peak (vector and matrix: 157.3 TFLOP/s)"}
+    hbm = None
     if not args.no_roofline:
         del graph
-        roof = hbm_roofline(stream)
+        hbm = hbm_roofline(stream)
 
     done, cpu_dt, err = cpu_baseline(progs, names, values, ps, weights, gpu_exp, gpu_grad,
                                      args.cpu_budget)
@@ -208,6 +261,7 @@ def main():
         "max_abs_err_vs_oracle": err,
         "oracle_rows_checked": done,
         "roofline": roof,
+        "hbm_pass": hbm,
         "cpu_baseline": {"value": round(done / cpu_dt, 3), "unit": "circuits/s", "cores": 1,
                          "kind": "port",
                          "sample": f"first {done} circuits of the same batch (expectation + "

@@ -21,6 +21,8 @@ The encoder writes proto3 wire format directly (no generated code).
 
 from __future__ import annotations
 
+import functools
+import math
 import struct
 from typing import Iterable, List
 
@@ -32,6 +34,7 @@ GATE_SET_NAME = "tfq_gate_set"
 
 
 # --------------------------------------------------------------------------- wire helpers
+@functools.lru_cache(maxsize=1 << 16)
 def _varint(v: int) -> bytes:
     out = bytearray()
     v &= (1 << 64) - 1
@@ -45,6 +48,7 @@ def _varint(v: int) -> bytes:
             return bytes(out)
 
 
+@functools.lru_cache(maxsize=256)
 def _key(field: int, wire: int) -> bytes:
     return _varint((field << 3) | wire)
 
@@ -67,8 +71,12 @@ def _float_field(field: int, x: float, always: bool = False) -> bytes:
 
 # --------------------------------------------------------------------------- values
 def round6(x: float) -> float:
-    """``serializer._round``: ``np.round(x, 6)``."""
-    return float(np.round(float(x), 6))
+    """``serializer._round``: ``np.round(x, 6)``, i.e. rint(x * 1e6) / 1e6 in
+    float64 (Python's ``round`` of a float rounds half to even like rint)."""
+    x = float(x)
+    if not math.isfinite(x):
+        return float(np.round(x, 6))
+    return round(x * 1e6) / 1e6
 
 
 def symbol_part(x):
@@ -99,8 +107,25 @@ def _arg(value) -> bytes:
     return _len_field(1, arg_value)
 
 
+@functools.lru_cache(maxsize=1 << 16)
+def _map_entry_cached(name: str, kind: str, value, sign: float = 1.0) -> bytes:
+    v = C.Symbol(value) if kind == "sym" else value
+    return _len_field(2, _str_field(1, name, always=True) + _len_field(2, _arg(v)))
+
+
 def _map_entry(name: str, value) -> bytes:
-    return _len_field(2, _str_field(1, name, always=True) + _len_field(2, _arg(value)))
+    if isinstance(value, C.Symbol):
+        return _map_entry_cached(name, "sym", value.name)
+    if isinstance(value, str):
+        return _map_entry_cached(name, "str", value)
+    x = float(value)
+    # 0.0 and -0.0 hash alike but encode differently.
+    return _map_entry_cached(name, "f", x, math.copysign(1.0, x))
+
+
+@functools.lru_cache(maxsize=1 << 12)
+def _qubit_field(proto_id: str) -> bytes:
+    return _len_field(3, _str_field(2, proto_id, always=True))
 
 
 def _op_args(op: C.Operation) -> List:
@@ -133,7 +158,7 @@ def _operation(op: C.Operation) -> bytes:
     for name, value in _op_args(op):
         body += _map_entry(name, value)
     for q in op.target_qubits:
-        body += _len_field(3, _str_field(2, q.proto_id(), always=True))
+        body += _qubit_field(q.proto_id())
     return body
 
 

@@ -9,8 +9,10 @@ Contents:
 * ``tfq_oracle.py``  complex128 numpy restatement of TFQ's six ops;
 * ``wire.py``        an independent proto3 decoder for Program / PauliSum;
 * ``philox.py``      host replica of the device sampler's Philox4x32-10 stream;
-* ``cpu_tfq.cc``     C++ restatement of TFQ's CPU (qsim) schedule, timed as the
-                     CPU baseline by ``bench.py``.
+* ``mlp.py``         float64 restatement of the VQE energy surrogate network;
+* ``cpu_tfq.cc``     float32 C++ restatement of TFQ's CPU (qsim) schedule with
+  ``cpu_tfq.py``     std::threads, timed as the CPU baseline by ``bench.py``
+                     (built by ``__graft_entry__.build()`` into libcpu_tfq.so).
 
 Parity status of each function (how it is pinned) is recorded in
 ``tests/test_oracle.py``; no function is left "parity unpinned" except the

@@ -1,0 +1,205 @@
+// CPU restatement of TFQ's qsim path (test and measurement infrastructure
+// only: the reported CPU baseline of bench.py; the product never links it).
+//
+// Follows the structure of TFQ's CPU ops, in float32 complex like qsim's
+// state space:
+//   expectation   simulate the (fused) gate list, then per Pauli term copy the
+//                 state, apply the term's Paulis and take the real inner
+//                 product (ComputeExpectationQsim, core/src/util_qsim.h:142-188);
+//                 identity terms add their coefficient.
+//   adjoint       simulate, lambda = sum_j g_j sum_t c_t P_t psi with
+//                 |g_j c_t| < 1e-5 dropped (AccumulateOperators,
+//                 util_qsim.h:363-414), then for gates in reverse:
+//                 psi <- G^dag psi; per symbol: scratch = dG psi (control-
+//                 satisfied amplitudes only), grad += 2 Re<scratch|lambda>;
+//                 lambda <- G^dag lam (core/ops/tfq_adj_grad_op.cc:214-276).
+// Circuits run in parallel over std::threads (the ComputeSmall schedule of
+// util_qsim.h:46-131, one circuit per worker).
+//
+// Gate conventions: bits[0] is the matrix-index MSB; matrices are row-major
+// complex64.  The Python side (oracle/cpu_tfq.py) lowers programs with the
+// oracle's parser and supplies matrices and derivative matrices.
+#include <stdint.h>
+
+#include <atomic>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+namespace {
+
+using c64 = std::complex<float>;
+
+struct Gate {
+  int32_t nt, b0, b1, mat, nderiv, dmat0, dmat1, col0, col1;
+  uint64_t cmask, cval;
+};
+
+inline c64 mulc(c64 a, c64 b) {
+  return c64(a.real() * b.real() - a.imag() * b.imag(), a.real() * b.imag() + a.imag() * b.real());
+}
+
+inline uint64_t ins0(uint64_t x, int b) { return ((x >> b) << (b + 1)) | (x & ((uint64_t(1) << b) - 1)); }
+
+void load_matrix(const c64* M, int d, bool dag, c64* out) {
+  for (int r = 0; r < d; ++r)
+    for (int c = 0; c < d; ++c) out[r * d + c] = dag ? std::conj(M[c * d + r]) : M[r * d + c];
+}
+
+// s <- U s on the gate's targets (controls: amplitudes with (x & cmask) == cval).
+// zero_others: amplitudes failing the control are set to zero (projection).
+void apply(c64* s, int n, const Gate& g, const c64* U, bool zero_others = false) {
+  const uint64_t half = uint64_t(1) << (n - g.nt);
+  if (g.nt == 1) {
+    const uint64_t bit = uint64_t(1) << g.b0;
+    for (uint64_t i = 0; i < half; ++i) {
+      const uint64_t x0 = ins0(i, g.b0), x1 = x0 | bit;
+      if ((x0 & g.cmask) != g.cval) {
+        if (zero_others) s[x0] = s[x1] = c64(0, 0);
+        continue;
+      }
+      const c64 a0 = s[x0], a1 = s[x1];
+      s[x0] = mulc(U[0], a0) + mulc(U[1], a1);
+      s[x1] = mulc(U[2], a0) + mulc(U[3], a1);
+    }
+    return;
+  }
+  const int lo = g.b0 < g.b1 ? g.b0 : g.b1, hi = g.b0 < g.b1 ? g.b1 : g.b0;
+  const uint64_t m0 = uint64_t(1) << g.b0, m1 = uint64_t(1) << g.b1;
+  for (uint64_t i = 0; i < half; ++i) {
+    const uint64_t base = ins0(ins0(i, lo), hi);
+    uint64_t x[4];
+    for (int m = 0; m < 4; ++m) x[m] = base | ((m >> 1) ? m0 : 0) | ((m & 1) ? m1 : 0);
+    if ((base & g.cmask) != g.cval) {
+      if (zero_others)
+        for (int m = 0; m < 4; ++m) s[x[m]] = c64(0, 0);
+      continue;
+    }
+    c64 a[4];
+    for (int m = 0; m < 4; ++m) a[m] = s[x[m]];
+    for (int r = 0; r < 4; ++r) {
+      c64 acc(0, 0);
+      for (int c = 0; c < 4; ++c) acc += mulc(U[r * 4 + c], a[c]);
+      s[x[r]] = acc;
+    }
+  }
+}
+
+// s <- P s for one Pauli string (types 0 X, 1 Y, 2 Z on bits).
+void apply_paulis(c64* s, int n, const int32_t* paulis, int np) {
+  static const c64 X[4] = {c64(0, 0), c64(1, 0), c64(1, 0), c64(0, 0)};
+  static const c64 Y[4] = {c64(0, 0), c64(0, -1), c64(0, 1), c64(0, 0)};
+  static const c64 Z[4] = {c64(1, 0), c64(0, 0), c64(0, 0), c64(-1, 0)};
+  for (int k = 0; k < np; ++k) {
+    Gate g{};
+    g.nt = 1;
+    g.b0 = paulis[2 * k];
+    const int t = paulis[2 * k + 1];
+    apply(s, n, g, t == 0 ? X : (t == 1 ? Y : Z));
+  }
+}
+
+double real_inner(const c64* a, const c64* b, uint64_t N) {
+  double acc = 0;
+  for (uint64_t i = 0; i < N; ++i) acc += double(a[i].real()) * b[i].real() + double(a[i].imag()) * b[i].imag();
+  return acc;
+}
+
+}  // namespace
+
+extern "C" {
+
+// circ[B][8]: n, gate_begin, gate_end, fgate_begin, fgate_end, term_begin, term_end, row.
+// gates_i[G][9]: nt, b0, b1, mat, nderiv, dmat0, dmat1, col0, col1; gates_m[G][2]: cmask, cval.
+// terms_i[T][3]: slot, pauli_begin, pauli_end; terms_c[T]: coefficient; paulis[P][2]: bit, type.
+// mode 0: expectation -> exp_out[B][O] (fused gates); mode 1: adjoint -> grad_out[B][S]
+// (unfused gates, downstream[B][O]).  Returns 0.
+int cpu_tfq_run(int mode, int64_t B, const int32_t* circ, const int32_t* gates_i,
+                const uint64_t* gates_m, const float* mats, const int32_t* terms_i,
+                const float* terms_c, const int32_t* paulis, int64_t O, int64_t S,
+                const float* downstream, int threads, float* exp_out, float* grad_out) {
+  const c64* M = reinterpret_cast<const c64*>(mats);
+  std::atomic<int64_t> next(0);
+  auto gate_at = [&](int64_t k) {
+    Gate g;
+    const int32_t* a = gates_i + 9 * k;
+    g.nt = a[0]; g.b0 = a[1]; g.b1 = a[2]; g.mat = a[3]; g.nderiv = a[4];
+    g.dmat0 = a[5]; g.dmat1 = a[6]; g.col0 = a[7]; g.col1 = a[8];
+    g.cmask = gates_m[2 * k]; g.cval = gates_m[2 * k + 1];
+    return g;
+  };
+  auto work = [&]() {
+    std::vector<c64> psi, lam, scratch;
+    while (true) {
+      const int64_t i = next.fetch_add(1);
+      if (i >= B) break;
+      const int32_t* c = circ + 8 * i;
+      const int n = c[0];
+      const int64_t row = c[7];
+      const uint64_t N = uint64_t(1) << n;
+      psi.assign(N, c64(0, 0));
+      psi[0] = c64(1, 0);
+      const int gb = mode == 0 ? c[3] : c[1], ge = mode == 0 ? c[4] : c[2];
+      c64 U[16];
+      for (int k = gb; k < ge; ++k) {
+        const Gate g = gate_at(k);
+        load_matrix(M + g.mat, 1 << g.nt, false, U);
+        apply(psi.data(), n, g, U);
+      }
+      if (mode == 0) {
+        for (int64_t j = 0; j < O; ++j) exp_out[row * O + j] = 0.f;
+        std::vector<double> acc(size_t(O), 0.0);
+        for (int t = c[5]; t < c[6]; ++t) {
+          const int32_t* ti = terms_i + 3 * t;
+          const int np = ti[2] - ti[1];
+          if (np == 0) {
+            acc[size_t(ti[0])] += terms_c[t];
+            continue;
+          }
+          scratch = psi;
+          apply_paulis(scratch.data(), n, paulis + 2 * ti[1], np);
+          acc[size_t(ti[0])] += double(terms_c[t]) * real_inner(psi.data(), scratch.data(), N);
+        }
+        for (int64_t j = 0; j < O; ++j) exp_out[row * O + j] = float(acc[size_t(j)]);
+        continue;
+      }
+      // Adjoint.
+      lam.assign(N, c64(0, 0));
+      for (int t = c[5]; t < c[6]; ++t) {
+        const int32_t* ti = terms_i + 3 * t;
+        const float w = downstream[row * O + ti[0]] * terms_c[t];
+        if (std::fabs(w) < 1e-5f) continue;
+        scratch = psi;
+        apply_paulis(scratch.data(), n, paulis + 2 * ti[1], ti[2] - ti[1]);
+        for (uint64_t x = 0; x < N; ++x) lam[x] += w * scratch[x];
+      }
+      std::vector<double> grad(size_t(S), 0.0);
+      for (int k = ge - 1; k >= gb; --k) {
+        const Gate g = gate_at(k);
+        load_matrix(M + g.mat, 1 << g.nt, true, U);
+        apply(psi.data(), n, g, U);
+        for (int d = 0; d < g.nderiv; ++d) {
+          scratch = psi;
+          c64 dU[16];
+          load_matrix(M + (d == 0 ? g.dmat0 : g.dmat1), 1 << g.nt, false, dU);
+          apply(scratch.data(), n, g, dU, /*zero_others=*/true);
+          grad[size_t(d == 0 ? g.col0 : g.col1)] += 2.0 * real_inner(scratch.data(), lam.data(), N);
+        }
+        apply(lam.data(), n, g, U);
+      }
+      for (int64_t s = 0; s < S; ++s) grad_out[row * S + s] = float(grad[size_t(s)]);
+    }
+  };
+  if (threads <= 1) {
+    work();
+  } else {
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; ++t) pool.emplace_back(work);
+    for (auto& th : pool) th.join();
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,160 @@
+"""Driver of ``oracle/cpu_tfq.cc``: TFQ's CPU (qsim) schedule restated in
+float32 C++ with std::threads -- the reported CPU baseline of ``bench.py``.
+
+TEST AND MEASUREMENT INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
+Programs are lowered with the oracle's parser (``tfq_oracle.Program``): gate
+matrices in complex64, TFQ's float32 finite-difference gradient gates
+(``adj_util.cc:175-302``), Pauli sums resolved to (bit, type).  The
+expectation path applies qsim-style basic fusion (single-qubit gates
+absorbed into the next two-qubit gate on their qubit); the adjoint path
+applies gates one by one, as ``tfq_adj_grad_op.cc:214-276`` does between
+parameterised gates.
+"""
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from . import tfq_oracle as T
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "cpu_tfq.cc")
+LIB = os.path.join(HERE, "libcpu_tfq.so")
+_lib = None
+_PT = {"X": 0, "Y": 1, "Z": 2}
+
+
+def build():
+    """Compile the restatement (g++, x86-64-v3, -O3)."""
+    cmd = ["g++", "-O3", "-march=x86-64-v3", "-std=c++17", "-fPIC", "-shared", "-pthread", SRC,
+           "-o", LIB + ".tmp"]
+    subprocess.run(cmd, check=True, capture_output=True, text=True)
+    os.replace(LIB + ".tmp", LIB)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        P = ctypes.c_void_p
+        L.cpu_tfq_run.argtypes = [ctypes.c_int, ctypes.c_int64, P, P, P, P, P, P, P,
+                                  ctypes.c_int64, ctypes.c_int64, P, ctypes.c_int, P, P]
+        L.cpu_tfq_run.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _basic_fuse(gates):
+    """[(U, bits, cmask, cval)] -> fused list: uncontrolled single-qubit gates
+    are absorbed into the next uncontrolled two-qubit gate on their qubit."""
+    pending, out = {}, []
+
+    def flush(b):
+        if b in pending:
+            out.append((pending.pop(b), [b], 0, 0))
+
+    for U, bits, cmask, cval in gates:
+        if len(bits) == 1 and cmask == 0:
+            b = bits[0]
+            pending[b] = U @ pending.get(b, np.eye(2))
+        elif len(bits) == 2 and cmask == 0:
+            A = pending.pop(bits[0], np.eye(2))
+            Bm = pending.pop(bits[1], np.eye(2))
+            out.append((U @ np.kron(A, Bm), list(bits), 0, 0))
+        else:
+            for b in list(bits) + [q for q in range(64) if (cmask >> q) & 1]:
+                flush(b)
+            out.append((U, list(bits), cmask, cval))
+    for b in sorted(pending):
+        flush(b)
+    return out
+
+
+class Lowered:
+    """Flat arrays of a batch for ``cpu_tfq_run``."""
+
+    def __init__(self, programs, symbol_names, symbol_values, pauli_sums):
+        names = [T._name(s) for s in symbol_names]
+        col = {s: j for j, s in enumerate(names)}
+        maps = T._symbol_maps(names, np.asarray(symbol_values, dtype=np.float32).reshape(len(programs), -1))
+        self.B, self.S = len(programs), len(names)
+        ps = np.asarray(pauli_sums, dtype=object)
+        self.O = ps.shape[1] if self.B else 0
+        circ, gi, gm, mats, ti, tc, pl = [], [], [], [], [], [], []
+        moff = [0]
+        empty = []
+
+        def push_mat(U):
+            off = moff[0]
+            m = np.asarray(U, dtype=np.complex64).reshape(-1)
+            mats.append(m)
+            moff[0] += m.size
+            return off
+
+        def push_gate(U, bits, cmask, cval, derivs=()):
+            mo = push_mat(U)
+            d = [(push_mat(dU), col[s]) for dU, s in derivs]
+            d += [(0, 0)] * (2 - len(d))
+            gi.append([len(bits), bits[0], bits[1] if len(bits) > 1 else 0, mo, len(derivs),
+                       d[0][0], d[1][0], d[0][1], d[1][1]])
+            gm.append([cmask, cval])
+
+        for i in range(self.B):
+            prog = T.Program(programs[i], maps[i])
+            n = prog.n
+            empty.append(not prog.gates)
+            gb = len(gi)
+            plain = []
+            for g in prog.gates:
+                U = g.unitary()
+                cmask = sum(1 << b for b in g.cbits)
+                cval = sum(int(v) << b for b, v in zip(g.cbits, g.cvals))
+                derivs = [(T._fd_matrix(g, pidx), sym) for sym, pidx in g.symbols]
+                push_gate(U, list(g.bits), cmask, cval, derivs)
+                plain.append((U, list(g.bits), cmask, cval))
+            ge = len(gi)
+            for U, bits, cmask, cval in _basic_fuse(plain):
+                push_gate(U, bits, cmask, cval)
+            fe = len(gi)
+            tb = len(ti)
+            if prog.gates:
+                for j in range(self.O):
+                    for c, paulis in T.parse_paulisum(ps[i][j], prog.qmap, n):
+                        ti.append([j, len(pl), len(pl) + len(paulis)])
+                        tc.append(c)
+                        pl.extend([b, _PT[t]] for b, t in paulis)
+            circ.append([n, gb, ge, ge, fe, tb, len(ti), i])
+        self.empty = np.array(empty, dtype=bool)
+        self.circ = np.array(circ, np.int32).reshape(-1, 8)
+        self.gi = np.array(gi, np.int32).reshape(-1, 9)
+        self.gm = np.array(gm, np.uint64).reshape(-1, 2)
+        self.mats = np.concatenate(mats) if mats else np.zeros(1, np.complex64)
+        self.ti = np.array(ti, np.int32).reshape(-1, 3)
+        self.tc = np.array(tc, np.float32)
+        self.pl = np.array(pl, np.int32).reshape(-1, 2)
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data if a.size else 0)
+
+
+def run(low, mode, downstream=None, threads=1):
+    """mode 0: expectations [B, O]; mode 1: adjoint gradients [B, S]."""
+    exp = np.zeros((low.B, max(low.O, 1)), np.float32)
+    grad = np.zeros((low.B, max(low.S, 1)), np.float32)
+    ds = np.ascontiguousarray(downstream if downstream is not None else np.zeros((low.B, low.O)),
+                              dtype=np.float32)
+    mats = np.ascontiguousarray(low.mats.view(np.float32))
+    rc = lib().cpu_tfq_run(mode, low.B, _p(low.circ), _p(low.gi), _p(low.gm), _p(mats), _p(low.ti),
+                           _p(low.tc), _p(low.pl), low.O, low.S, _p(ds), int(threads), _p(exp),
+                           _p(grad))
+    if rc != 0:
+        raise RuntimeError("cpu_tfq_run failed")
+    exp = exp[:, :low.O]
+    exp[low.empty] = -2.0
+    return exp if mode == 0 else grad[:, :low.S]

@@ -1,0 +1,57 @@
+"""The C++ CPU baseline (oracle/cpu_tfq.cc, float32, TFQ's qsim schedule)
+against the complex128 oracle on random circuits: expectations and TFQ-mode
+adjoint gradients agree to float32 accuracy (1e-4), with and without the
+basic fusion, threaded and single-threaded."""
+
+import numpy as np
+
+from oracle import cpu_tfq
+from oracle import tfq_oracle as O
+from quantum_amd import circuits as C
+from quantum_amd import serialize as S
+from quantum_amd import workloads as W
+
+
+def _batch(seed, count=6):
+    rng = np.random.default_rng(seed)
+    names = ["a", "b", "c", "d"]
+    circs, sums = [], []
+    for i in range(count):
+        q = C.LineQubit.range(int(rng.integers(2, 7)))
+        circs.append(W.random_circuit(rng, q, 25, symbols=names, p_symbol=0.5, p_control=0.2))
+        sums.append([W.random_pauli_sum(rng, q, 3), W.random_pauli_sum(rng, q, 2)])
+    circs.append(C.Circuit())
+    sums.append([C.PauliSum([]), C.PauliSum([])])
+    vals = rng.uniform(-2, 2, (len(circs), len(names))).astype(np.float32)
+    grads = rng.uniform(-1, 1, (len(circs), 2)).astype(np.float32)
+    return S.convert_to_tensor(circs), names, vals, S.convert_to_tensor(sums), grads
+
+
+def test_cpu_baseline_matches_oracle():
+    progs, names, vals, ps, grads = _batch(3)
+    low = cpu_tfq.Lowered(list(progs), names, vals, ps)
+    for threads in (1, 4):
+        e = cpu_tfq.run(low, 0, threads=threads)
+        g = cpu_tfq.run(low, 1, downstream=grads, threads=threads)
+        np.testing.assert_allclose(e, O.simulate_expectation(list(progs), names, vals, ps), atol=1e-4)
+        np.testing.assert_allclose(g, O.adjoint_gradient(list(progs), names, vals, ps, grads),
+                                   atol=1e-4)
+
+
+def test_basic_fusion_preserves_the_state():
+    rng = np.random.default_rng(5)
+    q = C.LineQubit.range(4)
+    circ = W.random_circuit(rng, q, 40, p_control=0.3)
+    prog = O.Program(S.serialize_circuit(circ), {})
+    plain = [(g.unitary(), list(g.bits), sum(1 << b for b in g.cbits),
+              sum(int(v) << b for b, v in zip(g.cbits, g.cvals))) for g in prog.gates]
+    fused = cpu_tfq._basic_fuse(plain)
+    assert len(fused) < len(plain)
+
+    def run(gl):
+        s = O.zero_state(prog.n)
+        for U, bits, cm, cv in gl:
+            cb = [b for b in range(prog.n) if (cm >> b) & 1]
+            s = O.apply_matrix(s, U, bits, prog.n, cb, [(cv >> b) & 1 for b in cb])
+        return s
+    np.testing.assert_allclose(run(fused), run(plain), atol=1e-12)

@@ -143,6 +143,21 @@ int tfqx_plan_expectation_and_gradient(
  * every call (core/ops/parse_context.cc:60-92). */
 int tfqx_plan_rebind(tfqx_plan* plan, const float* symbol_values, char* err, int64_t err_len);
 
+/* Fusion decisions of the planner for one program (host only): the fused ops
+ * of the forward program at the given width (1..4).  ops_out[i][8] = {k,
+ * bit0..bit3 (global bit positions, ascending, -1 unused), 1 if the op keeps
+ * controls outside its matrix, first, end} with [first, end) indexing gates_out,
+ * the op's gates (circuit order of the program's operations) in application
+ * order.  *n_ops / *n_gates receive the sizes; nothing is written when they
+ * exceed the capacities.  No reference counterpart: exposes qsim-fuser-like
+ * decisions (qsim BasicGateFuser, called at core/src/circuit_parser_qsim.cc:
+ * 857-859) so a host replica can check them. */
+int tfqx_debug_fuse(const uint8_t* program, int64_t program_len, const uint8_t* symbol_names,
+                    const int64_t* symbol_offsets, int64_t num_symbols, const float* symbol_values,
+                    int32_t width, int32_t* ops_out, int64_t ops_cap, int32_t* gates_out,
+                    int64_t gates_cap, int64_t* n_ops, int64_t* n_gates, char* err,
+                    int64_t err_len);
+
 /* Plan statistics: out[0] circuits, [1] forward ops, [2] forward HBM passes,
  * [3] max qubits, [4] adjoint ops, [5] adjoint HBM passes, [6] 1 if the
  * HBM-streaming path is used, [7] workspace bytes, [8] algorithmic real flops
@@ -201,6 +216,12 @@ int tfqx_mlp_train_step(void* workspace, int32_t in_dim, int32_t hidden, int32_t
 int tfqx_gemm_bf16(int32_t M, int32_t N, int32_t K, const void* A, int32_t lda, int32_t transA,
                    const void* B, int32_t ldb, int32_t transB, float* C, int32_t ldc, float alpha,
                    float beta, void* stream, char* err, int64_t err_len);
+
+/* Planner tuning for later plans: register engine on (1) / off (0) for the
+ * batched kernel's tiles of 12-13 qubits, and the streaming path's tile
+ * qubits for one-state and two-state passes (0 = defaults 13 and 11; at most
+ * 14 and 13).  No reference counterpart (qsim's CPU path has no tiles). */
+void tfqx_set_tuning(int32_t register_engine, int32_t tile_single, int32_t tile_two);
 
 /* Fusion width (qubits per fused block) the planner uses; 0 = autotuned. */
 void tfqx_set_fusion_width(int32_t width);

@@ -48,6 +48,9 @@ def _declare(lib):
     lib.tfqx_plan_inner_product.argtypes = s + [P, P, i64, P, c.c_char_p, i64]
     lib.tfqx_plan_adjoint_gradient.argtypes = s + [P, P, i64, P, i32, P, c.c_char_p, i64]
     lib.tfqx_plan_expectation_and_gradient.argtypes = s + [P, P, i64, P, i32, P, c.c_char_p, i64]
+    lib.tfqx_debug_fuse.argtypes = [P, i64, P, P, i64, P, i32, P, i64, P, i64, P, P,
+                                    c.c_char_p, i64]
+    lib.tfqx_debug_fuse.restype = c.c_int
     lib.tfqx_plan_rebind.argtypes = [P, P, c.c_char_p, i64]
     lib.tfqx_plan_rebind.restype = c.c_int
     lib.tfqx_plan_stats.argtypes = [P, P, i64]
@@ -70,6 +73,8 @@ def _declare(lib):
     lib.tfqx_graph_destroy.restype = None
     lib.tfqx_plan_destroy.argtypes = [P]
     lib.tfqx_plan_destroy.restype = None
+    lib.tfqx_set_tuning.argtypes = [i32, i32, i32]
+    lib.tfqx_set_tuning.restype = None
     lib.tfqx_set_fusion_width.argtypes = [i32]
     lib.tfqx_set_fusion_width.restype = None
     lib.tfqx_version.restype = c.c_char_p
@@ -103,11 +108,12 @@ EXPORTED_SYMBOLS = (
     "tfqx_plan_simulate_expectation", "tfqx_plan_simulate_sampled_expectation",
     "tfqx_plan_simulate_samples", "tfqx_plan_simulate_state", "tfqx_plan_inner_product",
     "tfqx_plan_adjoint_gradient", "tfqx_plan_expectation_and_gradient", "tfqx_plan_rebind",
+    "tfqx_debug_fuse",
     "tfqx_plan_stats",
     "tfqx_plan_max_num_qubits", "tfqx_plan_workspace_bytes",
     "tfqx_plan_output_bytes", "tfqx_plan_upload", "tfqx_plan_launch", "tfqx_plan_launch_plain",
     "tfqx_graph_capture", "tfqx_graph_launch", "tfqx_graph_destroy", "tfqx_plan_destroy",
-    "tfqx_set_fusion_width", "tfqx_version", "tfqx_mlp_layout", "tfqx_mlp_set_params",
+    "tfqx_set_fusion_width", "tfqx_set_tuning", "tfqx_version", "tfqx_mlp_layout", "tfqx_mlp_set_params",
     "tfqx_mlp_predict", "tfqx_mlp_loss_grad", "tfqx_mlp_train_step", "tfqx_gemm_bf16")
 
 
@@ -224,6 +230,35 @@ class Graph:
                 self.handle = ctypes.c_void_p()
         except Exception:
             pass
+
+
+def fusion_decisions(program, symbol_names=(), symbol_values=(), width=2):
+    """The planner's fused ops for one serialized program: a list of
+    (bits tuple, controls_outside, gate index tuple)."""
+    L = lib()
+    names = StringTensor(np.array(list(symbol_names), dtype=object))
+    vals = np.ascontiguousarray(np.asarray(symbol_values, dtype=np.float32).reshape(-1))
+    buf = ctypes.create_string_buffer(program, max(1, len(program)))
+    nops, ngates = ctypes.c_int64(), ctypes.c_int64()
+    err = ctypes.create_string_buffer(4096)
+    args = [ctypes.addressof(buf), len(program), names.ptr, names.off, names.n, np_ptr(vals),
+            int(width)]
+    check(L.tfqx_debug_fuse(*args, None, 0, None, 0, ctypes.byref(nops), ctypes.byref(ngates),
+                            err, 4096), err)
+    ops = np.zeros((max(1, nops.value), 8), np.int32)
+    gates = np.zeros(max(1, ngates.value), np.int32)
+    check(L.tfqx_debug_fuse(*args, ops.ctypes.data, nops.value, gates.ctypes.data, ngates.value,
+                            ctypes.byref(nops), ctypes.byref(ngates), err, 4096), err)
+    out = []
+    for r in ops[:nops.value]:
+        bits = tuple(int(b) for b in r[1:1 + r[0]])
+        out.append((bits, bool(r[5]), tuple(int(g) for g in gates[r[6]:r[7]])))
+    return out
+
+
+def set_tuning(register_engine=True, tile_single=0, tile_two=0):
+    """Planner tuning for later plans (see include/tfqx.h: tfqx_set_tuning)."""
+    lib().tfqx_set_tuning(int(bool(register_engine)), int(tile_single), int(tile_two))
 
 
 def set_fusion_width(width):

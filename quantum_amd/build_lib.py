@@ -33,10 +33,16 @@ def _hipcc():
     raise RuntimeError("hipcc not found")
 
 
+# Device code: keep SimplifyCFG from sinking the stores of the register
+# engine's per-qubit-position variants into one store through a PHI of
+# addresses, which would demote the register arrays to scratch memory.
+HIP_ONLY = ["-mllvm", "-simplifycfg-sink-common=false"]
+
+
 def _compile(src):
     path = os.path.join(CSRC, src)
     obj = os.path.join(BUILD, src + ".o")
-    lang = ["-x", "hip"] if src.endswith(".hip") else []
+    lang = ["-x", "hip"] + HIP_ONLY if src.endswith(".hip") else []
     cmd = [_hipcc()] + COMMON + lang + ["-c", path, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

@@ -326,6 +326,7 @@ hipStream_t AsStream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 double AlgorithmicFlops(const BatchProgram& g) {
   double f = 0;
   auto op_flops = [](const DevOp& o) {
+    if (o.type == kOpRemap) return 0.0;
     return o.type == kOpDiag ? 6.0 : 8.0 * double(1 << o.k) - 2.0;
   };
   for (const auto& c : g.circuits) {
@@ -414,6 +415,14 @@ extern "C" {
 const char* tfqx_version(void) { return "tfqx 0.1 (gfx950)"; }
 
 void tfqx_set_fusion_width(int32_t width) { g_fusion_width.store(width); }
+
+void tfqx_set_tuning(int32_t register_engine, int32_t tile_single, int32_t tile_two) {
+  Tuning t;
+  t.reg_engine = register_engine != 0;
+  t.tile_single = std::max(0, int(tile_single));
+  t.tile_two = std::max(0, int(tile_two));
+  SetTuning(t);
+}
 
 int tfqx_plan_simulate_expectation(const uint8_t* programs, const int64_t* poff, int64_t B,
                                    const uint8_t* names, const int64_t* noff, int64_t S,
@@ -523,6 +532,48 @@ int tfqx_plan_expectation_and_gradient(const uint8_t* programs, const int64_t* p
                    plan, err, len, /*with_expectation=*/true);
 }
 
+int tfqx_debug_fuse(const uint8_t* program, int64_t program_len, const uint8_t* names,
+                    const int64_t* noff, int64_t S, const float* values, int32_t width,
+                    int32_t* ops_out, int64_t ops_cap, int32_t* gates_out, int64_t gates_cap,
+                    int64_t* n_ops, int64_t* n_gates, char* err, int64_t len) {
+  if (!program || !n_ops || !n_gates || width < 1 || width > kMaxFuse || S < 0)
+    return Fail(err, len, TFQX_INVALID_ARGUMENT, "invalid argument");
+  Program prog;
+  if (!DecodeProgram(program, size_t(program_len), &prog))
+    return Fail(err, len, TFQX_INVALID_ARGUMENT,
+                "Unparseable proto: " + std::string(reinterpret_cast<const char*>(program),
+                                                    size_t(program_len)));
+  std::unordered_map<std::string, int> columns;
+  for (int64_t j = 0; j < S; ++j) columns[Bytes(names, noff, j)] = int(j);
+  SymbolMap sm;
+  sm.columns = &columns;
+  sm.values = values;
+  Circuit c;
+  Status s = BuildCircuit(prog, sm, &c);
+  if (!s.ok()) return Fail(err, len, s.code, s.msg);
+  std::vector<FusedOp> fused;
+  FuseOptions fo;
+  fo.max_fuse = width;
+  if (!c.gates.empty()) FuseGates(c, 0, int(c.gates.size()), fo, &fused);
+  *n_ops = int64_t(fused.size());
+  int64_t ng = 0;
+  for (const auto& f : fused) ng += int64_t(f.gates.size());
+  *n_gates = ng;
+  if (int64_t(fused.size()) > ops_cap || ng > gates_cap) return TFQX_OK;  // sizes only
+  int64_t gpos = 0;
+  for (size_t i = 0; i < fused.size(); ++i) {
+    const FusedOp& f = fused[i];
+    int32_t* r = ops_out + 8 * i;
+    r[0] = f.k;
+    for (int j = 0; j < 4; ++j) r[1 + j] = j < f.k ? f.bits[j] : -1;
+    r[5] = f.cmask != 0;
+    r[6] = int32_t(gpos);
+    for (int g : f.gates) gates_out[gpos++] = g;
+    r[7] = int32_t(gpos);
+  }
+  return TFQX_OK;
+}
+
 int tfqx_plan_rebind(tfqx_plan* p, const float* values, char* err, int64_t len) {
   if (!p || (!values && p->S > 0)) return Fail(err, len, TFQX_INVALID_ARGUMENT, "null argument");
   if (!p->shared)
@@ -543,18 +594,24 @@ int tfqx_plan_stats(const tfqx_plan* p, int64_t* out, int64_t n) {
   if (p->large) {
     const LargePlan& lp = *p->large_plan;
     v[0] = int64_t(lp.circuits.size());
+    auto real_ops = [&](const DevPass& ps) {
+      int64_t k = 0;
+      for (int i = ps.op_begin; i < ps.op_end; ++i) k += lp.ops[size_t(i)].type != kOpRemap;
+      return k;
+    };
     for (const auto& c : lp.circuits) {
       v[2] += int64_t(c.fwd.passes.size());
       v[5] += int64_t(c.rev.passes.size());
-      for (const auto& ps : c.fwd.passes) v[1] += ps.op_end - ps.op_begin;
-      for (const auto& ps : c.rev.passes) v[4] += ps.op_end - ps.op_begin;
+      for (const auto& ps : c.fwd.passes) v[1] += real_ops(ps);
+      for (const auto& ps : c.rev.passes) v[4] += real_ops(ps);
     }
     v[6] = 1;
   } else {
     v[0] = int64_t(p->prog.circuits.size());
+    const auto& ops = p->prog.ops;
     for (const auto& c : p->prog.circuits) {
-      v[1] += c.fwd_end - c.fwd_begin;
-      v[4] += c.rev_end - c.rev_begin;
+      for (int i = c.fwd_begin; i < c.fwd_end; ++i) v[1] += ops[size_t(i)].type != kOpRemap;
+      for (int i = c.rev_begin; i < c.rev_end; ++i) v[4] += ops[size_t(i)].type != kOpRemap;
     }
     v[8] = int64_t(AlgorithmicFlops(p->prog));
   }

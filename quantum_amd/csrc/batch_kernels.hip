@@ -21,25 +21,37 @@
 
 #include "batch_kernels.h"
 #include "lds_ops.h"
+#include "reg_engine.h"
 
 namespace tfqx {
 
 namespace {
 
+struct NoAcc {
+  __device__ void operator()(int, int, double) const {}
+};
+
+// Forward ops [b, e) on an LDS tile of n qubits (register engine when planned).
 template <int KMAX>
-__device__ void run_ops(float2* s, int n, const DevOp* ops, int b, int e, const float2* mats,
+__device__ __forceinline__ void run_ops(float2* s, int n, const DevOp* ops, int b, int e, const float2* mats,
                         float2* mbuf) {
+  if (reg_run_any<false>(s, s, n, ops, b, e, mats, NoAcc())) return;
   for_ops(ops, b, e, mats, mbuf,
           [&](const DevOp& op, const float2* M) { apply_op<false, KMAX>(s, n, op, M); });
 }
 
-__device__ void set_zero_state(float2* s, uint32_t N) {
+// Adjoint sweep of ops [b, e) on psi (A) and lambda (B), gradients into gacc.
+template <int KMAX>
+__device__ __forceinline__ void run_reverse(float2* A, float2* B, int n, const DevOp* ops, int b, int e,
+                            const float2* mats, float2* mbuf, double* gacc, double* red);
+
+__device__ __forceinline__ void set_zero_state(float2* s, uint32_t N) {
   for (uint32_t x = threadIdx.x; x < N; x += kThreads) s[x] = make_float2(x == 0 ? 1.f : 0.f, 0.f);
   __syncthreads();
 }
 
 // In-place: float2 amplitudes -> FP64 inclusive prefix of |a|^2 (same bytes).
-__device__ double build_cdf(float2* s, uint32_t N, double* red) {
+__device__ __forceinline__ double build_cdf(float2* s, uint32_t N, double* red) {
   double* cdf = reinterpret_cast<double*>(s);
   const uint32_t chunk = (N + kThreads - 1) / kThreads;
   const uint32_t b = threadIdx.x * chunk;
@@ -128,10 +140,25 @@ __device__ __forceinline__ void param_step(float2* A, float2* B, int n, const De
   }
 }
 
+template <int KMAX>
+__device__ __forceinline__ void run_reverse(float2* A, float2* B, int n, const DevOp* ops, int b, int e,
+                            const float2* mats, float2* mbuf, double* gacc, double* red) {
+  auto acc = [gacc](int, int col, double w) { atomicAdd(&gacc[col], 2.0 * w); };
+  if (reg_run_any<true>(A, B, n, ops, b, e, mats, acc)) return;
+  for_ops(ops, b, e, mats, mbuf, [&](const DevOp& op, const float2* M) {
+    if (op.type == kOpParam) {
+      param_step(A, B, n, op, M, gacc, red);
+    } else {
+      apply_op<true, KMAX>(A, n, op, M);
+      apply_op<true, KMAX>(B, n, op, M);
+    }
+  });
+}
+
 // Tiled forward: the state lives in global memory G; each pass streams its
 // tiles through LDS buffer A.  The first pass starts from |0>.
 template <int KMAX>
-__device__ void tiled_forward(float2* G, int n, const BatchArgs& a, int pb, int pe, float2* A,
+__device__ __forceinline__ void tiled_forward(float2* G, int n, const BatchArgs& a, int pb, int pe, float2* A,
                               float2* mbuf, uint64_t* tab) {
   const int tid = threadIdx.x;
   bool first = true;
@@ -144,15 +171,17 @@ __device__ void tiled_forward(float2* G, int n, const BatchArgs& a, int pb, int 
     const uint32_t ntiles = 1u << (n - ps.L);
     for (uint32_t t = 0; t < ntiles; ++t) {
       const uint64_t base = tile_base(t, ps.tilemask, n);
-      for (uint32_t j = tid; j < T; j += kThreads) {
-        const uint64_t g = use_tab ? tile_index_tab(base, j, ps.nlow, tab) : tile_index(base, j, ps);
-        A[j] = first ? make_float2(g == 0 ? 1.f : 0.f, 0.f) : G[g];
+      auto gidx = [&](uint32_t j) -> uint64_t {
+        return use_tab ? tile_index_tab(base, j, ps.nlow, tab) : tile_index(base, j, ps);
+      };
+      if (first) {
+        for (uint32_t j = tid; j < T; j += kThreads) A[j] = make_float2(gidx(j) == 0 ? 1.f : 0.f, 0.f);
+      } else {
+        tile_gather(A, G, T, gidx);
       }
       __syncthreads();
-      for_ops(a.ops, ps.op_begin, ps.op_end, a.mats, mbuf,
-              [&](const DevOp& op, const float2* M) { apply_op<false, KMAX>(A, ps.L, op, M); });
-      for (uint32_t j = tid; j < T; j += kThreads)
-        G[use_tab ? tile_index_tab(base, j, ps.nlow, tab) : tile_index(base, j, ps)] = A[j];
+      run_ops<KMAX>(A, ps.L, a.ops, ps.op_begin, ps.op_end, a.mats, mbuf);
+      tile_scatter(G, A, T, gidx);
       __syncthreads();
     }
     first = false;
@@ -165,7 +194,7 @@ __device__ void tiled_forward(float2* G, int n, const BatchArgs& a, int pb, int 
 
 // Tiled adjoint sweep on psi (G) and lambda (Lg) with LDS tiles A and B.
 template <int KMAX>
-__device__ void tiled_reverse(float2* G, float2* Lg, int n, const BatchArgs& a, int pb, int pe,
+__device__ __forceinline__ void tiled_reverse(float2* G, float2* Lg, int n, const BatchArgs& a, int pb, int pe,
                               float2* A, float2* B, double* gacc, double* red, float2* mbuf,
                               uint64_t* tab) {
   const int tid = threadIdx.x;
@@ -178,25 +207,14 @@ __device__ void tiled_reverse(float2* G, float2* Lg, int n, const BatchArgs& a, 
     const uint32_t ntiles = 1u << (n - ps.L);
     for (uint32_t t = 0; t < ntiles; ++t) {
       const uint64_t base = tile_base(t, ps.tilemask, n);
-      for (uint32_t j = tid; j < T; j += kThreads) {
-        const uint64_t g = use_tab ? tile_index_tab(base, j, ps.nlow, tab) : tile_index(base, j, ps);
-        A[j] = G[g];
-        B[j] = Lg[g];
-      }
+      auto gidx = [&](uint32_t j) -> uint64_t {
+        return use_tab ? tile_index_tab(base, j, ps.nlow, tab) : tile_index(base, j, ps);
+      };
+      tile_gather2(A, B, G, Lg, T, gidx);
       __syncthreads();
-      for_ops(a.ops, ps.op_begin, ps.op_end, a.mats, mbuf, [&](const DevOp& op, const float2* M) {
-        if (op.type == kOpParam) {
-          param_step(A, B, ps.L, op, M, gacc, red);
-        } else {
-          apply_op<true, KMAX>(A, ps.L, op, M);
-          apply_op<true, KMAX>(B, ps.L, op, M);
-        }
-      });
-      for (uint32_t j = tid; j < T; j += kThreads) {
-        const uint64_t g = use_tab ? tile_index_tab(base, j, ps.nlow, tab) : tile_index(base, j, ps);
-        G[g] = A[j];
-        Lg[g] = B[j];
-      }
+      run_reverse<KMAX>(A, B, ps.L, a.ops, ps.op_begin, ps.op_end, a.mats, mbuf, gacc, red);
+      tile_scatter(G, A, T, gidx);
+      tile_scatter(Lg, B, T, gidx);
       __syncthreads();
     }
   }
@@ -275,20 +293,16 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(BatchArgs a) {
       } else {
         float2* lam = buf2;
         build_lambda(psi, lam, N, a, lb, le);
-        for_ops(a.ops, uniform(c.rev_begin), uniform(c.rev_end), a.mats, mbuf,
-                [&](const DevOp& op, const float2* M) {
-                  if (op.type == kOpParam) {
-                    param_step(psi, lam, n, op, M, gacc, red);
-                  } else {
-                    apply_op<true, KMAX>(psi, n, op, M);
-                    apply_op<true, KMAX>(lam, n, op, M);
-                  }
-                });
+        run_reverse<KMAX>(psi, lam, n, a.ops, uniform(c.rev_begin), uniform(c.rev_end), a.mats,
+                          mbuf, gacc, red);
       }
-      __threadfence_block();
+      // Gradients arrive by atomics (register engine) or plain stores; read them
+      // back from L2.
+      __threadfence();
       __syncthreads();
       for (int j = tid; j < a.nsym; j += kThreads)
-        a.grad_out[size_t(row) * a.nsym + j] = float(gacc[j]);
+        a.grad_out[size_t(row) * a.nsym + j] =
+            float(__hip_atomic_load(&gacc[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     }
 
     if (MODE == kModeState) {

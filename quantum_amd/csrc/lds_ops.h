@@ -95,7 +95,7 @@ __device__ __forceinline__ void for_ops(const DevOp* ops, int b, int e, const fl
   }
 }
 
-__device__ double block_sum(double v, double* red) {
+__device__ __forceinline__ double block_sum(double v, double* red) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   __syncthreads();
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
@@ -107,7 +107,7 @@ __device__ double block_sum(double v, double* red) {
 }
 
 // Exclusive block scan of one double per thread.
-__device__ double block_exclusive_scan(double v, double* red, double* total) {
+__device__ __forceinline__ double block_exclusive_scan(double v, double* red, double* total) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   double inc = v;
   for (int off = 1; off < 64; off <<= 1) {
@@ -153,7 +153,7 @@ __device__ __forceinline__ double philox_uniform(unsigned long long seed, unsign
 // For K >= 3 the matrix entries are re-read from LDS per group instead of
 // being hoisted into registers (a 16x16 complex matrix would take 512 VGPRs).
 template <int K, bool DAG>
-__device__ void apply_dense(float2* __restrict__ s, int n, const DevOp& op,
+__device__ __forceinline__ void apply_dense(float2* __restrict__ s, int n, const DevOp& op,
                             const float2* __restrict__ M) {
   constexpr int D = 1 << K;
   const uint32_t ngroups = 1u << (n - K);
@@ -189,7 +189,7 @@ __device__ void apply_dense(float2* __restrict__ s, int n, const DevOp& op,
 }
 
 template <bool DAG>
-__device__ void apply_diag(float2* __restrict__ s, int n, const DevOp& op,
+__device__ __forceinline__ void apply_diag(float2* __restrict__ s, int n, const DevOp& op,
                            const float2* __restrict__ Dg) {
   const uint32_t N = 1u << n;
   const int k = op.k;
@@ -208,7 +208,7 @@ __device__ void apply_diag(float2* __restrict__ s, int n, const DevOp& op,
 // KMAX bounds the fused width the caller may see (fewer template cases keep
 // register pressure low when all ops are 1-2 qubit).
 template <bool DAG, int KMAX = kMaxFuse>
-__device__ void apply_op(float2* s, int n, const DevOp& op, const float2* M) {
+__device__ __forceinline__ void apply_op(float2* s, int n, const DevOp& op, const float2* M) {
   if (op.type == kOpDiag) {
     apply_diag<DAG>(s, n, op, M);
   } else if (KMAX <= 2 || op.k <= 2) {
@@ -221,7 +221,7 @@ __device__ void apply_op(float2* s, int n, const DevOp& op, const float2* M) {
 }
 
 // One-qubit matrix on bit q (Z-basis rotations of the sampled expectation).
-__device__ void apply_1q(float2* __restrict__ s, int n, int q, const float2* M) {
+__device__ __forceinline__ void apply_1q(float2* __restrict__ s, int n, int q, const float2* M) {
   const uint32_t ngroups = 1u << (n - 1);
   const float2 m00 = M[0], m01 = M[1], m10 = M[2], m11 = M[3];
   for (uint32_t g = threadIdx.x; g < ngroups; g += kThreads) {
@@ -237,7 +237,7 @@ __device__ void apply_1q(float2* __restrict__ s, int n, int q, const float2* M) 
 // Control-mismatched amplitudes are untouched and contribute nothing
 // (the BulkSetAmpl projection of tfq_adj_grad_op.cc:241-260).
 template <int K>
-__device__ void apply_param(float2* __restrict__ psi, float2* __restrict__ lam, int n,
+__device__ __forceinline__ void apply_param(float2* __restrict__ psi, float2* __restrict__ lam, int n,
                             const DevOp& op, const float2* __restrict__ M, double* acc) {
   constexpr int D = 1 << K;
   const float2* G = M;
@@ -337,6 +337,87 @@ __device__ __forceinline__ uint64_t tile_index_tab(uint64_t base, uint32_t j, in
 
 __device__ __forceinline__ uint64_t tile_index(uint64_t base, uint32_t j, const DevPass& p) {
   return base | (j & ((1u << p.nlow) - 1u)) | deposit_high(j >> p.nlow, p);
+}
+
+// ------------------------------------------------------------------ tile transfer
+// Global <-> LDS tile copies with kGatherBatch independent global accesses in
+// flight per thread (a one-at-a-time loop leaves the copy latency-bound).
+// idx(j) maps tile-local amplitude j to its global index.
+constexpr int kGatherBatch = 8;
+
+template <typename IndexFn>
+__device__ __forceinline__ void tile_gather(float2* __restrict__ A, const float2* __restrict__ G,
+                                            uint32_t T, IndexFn&& idx) {
+  for (uint32_t j0 = threadIdx.x; j0 < T; j0 += kThreads * kGatherBatch) {
+    float v[2 * kGatherBatch];
+#pragma unroll
+    for (int i = 0; i < kGatherBatch; ++i) {
+      const uint32_t j = j0 + uint32_t(i) * kThreads;
+      if (j < T) {
+        const float2 x = G[idx(j)];
+        v[2 * i] = x.x;
+        v[2 * i + 1] = x.y;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < kGatherBatch; ++i) {
+      const uint32_t j = j0 + uint32_t(i) * kThreads;
+      if (j < T) A[j] = make_float2(v[2 * i], v[2 * i + 1]);
+    }
+  }
+}
+
+// Two states through the same index map (psi and lambda of the adjoint).
+template <typename IndexFn>
+__device__ __forceinline__ void tile_gather2(float2* __restrict__ A, float2* __restrict__ B,
+                                             const float2* __restrict__ GA,
+                                             const float2* __restrict__ GB, uint32_t T,
+                                             IndexFn&& idx) {
+  for (uint32_t j0 = threadIdx.x; j0 < T; j0 += kThreads * kGatherBatch) {
+    float va[2 * kGatherBatch], vb[2 * kGatherBatch];
+#pragma unroll
+    for (int i = 0; i < kGatherBatch; ++i) {
+      const uint32_t j = j0 + uint32_t(i) * kThreads;
+      if (j < T) {
+        const uint64_t g = idx(j);
+        const float2 x = GA[g], y = GB[g];
+        va[2 * i] = x.x;
+        va[2 * i + 1] = x.y;
+        vb[2 * i] = y.x;
+        vb[2 * i + 1] = y.y;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < kGatherBatch; ++i) {
+      const uint32_t j = j0 + uint32_t(i) * kThreads;
+      if (j < T) {
+        A[j] = make_float2(va[2 * i], va[2 * i + 1]);
+        B[j] = make_float2(vb[2 * i], vb[2 * i + 1]);
+      }
+    }
+  }
+}
+
+template <typename IndexFn>
+__device__ __forceinline__ void tile_scatter(float2* __restrict__ G, const float2* __restrict__ A,
+                                             uint32_t T, IndexFn&& idx) {
+  for (uint32_t j0 = threadIdx.x; j0 < T; j0 += kThreads * kGatherBatch) {
+    float v[2 * kGatherBatch];
+#pragma unroll
+    for (int i = 0; i < kGatherBatch; ++i) {
+      const uint32_t j = j0 + uint32_t(i) * kThreads;
+      if (j < T) {
+        const float2 x = A[j];
+        v[2 * i] = x.x;
+        v[2 * i + 1] = x.y;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < kGatherBatch; ++i) {
+      const uint32_t j = j0 + uint32_t(i) * kThreads;
+      if (j < T) G[idx(j)] = make_float2(v[2 * i], v[2 * i + 1]);
+    }
+  }
 }
 
 }  // namespace

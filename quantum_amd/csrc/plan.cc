@@ -164,6 +164,54 @@ FusedOp BlockToOp(const Circuit& c, const std::vector<int>& gates, uint64_t qmas
 
 }  // namespace
 
+namespace {
+std::atomic<int> g_reg_engine(1), g_tile_single(0), g_tile_two(0);
+}  // namespace
+
+Tuning GetTuning() {
+  Tuning t;
+  t.reg_engine = g_reg_engine.load();
+  t.tile_single = g_tile_single.load();
+  t.tile_two = g_tile_two.load();
+  return t;
+}
+
+void SetTuning(const Tuning& t) {
+  g_reg_engine.store(t.reg_engine);
+  g_tile_single.store(t.tile_single);
+  g_tile_two.store(t.tile_two);
+}
+
+void InsertRemaps(std::vector<DevOp>* ops, size_t begin, int L) {
+  if (!g_reg_engine.load()) return;
+  const int R = RegBitsForTile(L);
+  if (R == 0 || begin >= ops->size()) return;
+  for (size_t i = begin; i < ops->size(); ++i)
+    if ((*ops)[i].k > 2) return;
+  std::vector<DevOp> in(ops->begin() + long(begin), ops->end());
+  ops->resize(begin);
+  size_t i = 0;
+  while (i < in.size()) {
+    uint32_t S = 0;
+    size_t j = i;
+    for (; j < in.size(); ++j) {
+      const DevOp& o = in[j];
+      if (o.type == kOpDiag) continue;
+      uint32_t T = 0;
+      for (int t = 0; t < o.k; ++t) T |= 1u << o.q[t];
+      if (Popc(S | T) > R) break;
+      S |= T;
+    }
+    for (int b = 0; b < L && Popc(S) < R; ++b) S |= 1u << b;
+    DevOp rm{};
+    rm.type = kOpRemap;
+    rm.cmask = S;
+    ops->push_back(rm);
+    for (size_t t = i; t < j; ++t) ops->push_back(in[t]);
+    i = j;
+  }
+}
+
 bool KindDiagonal(GateKind kind) {
   return kind == kGateI1 || kind == kGateI2 || kind == kGateZ || kind == kGateZZ ||
          kind == kGateCZ;
@@ -409,9 +457,10 @@ void BuildPiece(const CircuitJob& job, int num_slots, const PlanOptions& opt, Pi
   dc.fwd_begin = 0;
   dc.fwd_pass_begin = 0;
   if (tiled) {
-    SchedulePasses(fused, dc.n, opt.tile_log2, &pc->passes, &pc->ops, &pc->mats);
+    SchedulePasses(fused, dc.n, opt.tile_log2, &pc->passes, &pc->ops, &pc->mats, true);
   } else {
     for (const auto& f : fused) pc->ops.push_back(MakeDevOp(f, &pc->mats, rc, rec));
+    InsertRemaps(&pc->ops, size_t(dc.fwd_begin), dc.n);
   }
   dc.fwd_end = int(pc->ops.size());
   dc.fwd_pass_end = int(pc->passes.size());
@@ -454,9 +503,10 @@ void BuildPiece(const CircuitJob& job, int num_slots, const PlanOptions& opt, Pi
       seg_end = pg[p - 1];
     }
     if (tiled) {
-      SchedulePasses(rev_ops, dc.n, opt.tile_log2, &pc->passes, &pc->ops, &pc->mats);
+      SchedulePasses(rev_ops, dc.n, opt.tile_log2, &pc->passes, &pc->ops, &pc->mats, true);
     } else {
       for (const auto& f : rev_ops) pc->ops.push_back(MakeDevOp(f, &pc->mats, rc, rec));
+      InsertRemaps(&pc->ops, size_t(dc.rev_begin), dc.n);
     }
     dc.rev_end = int(pc->ops.size());
     dc.rev_pass_end = int(pc->passes.size());
@@ -473,6 +523,7 @@ void BuildPiece(const CircuitJob& job, int num_slots, const PlanOptions& opt, Pi
         pc->ops.push_back(MakeDevOp(f, &pc->mats));
         cost += amps * (f.diag ? 1 : (1 << f.k)) + 2048.0;
       }
+      InsertRemaps(&pc->ops, size_t(o.op_begin), dc.n);
       o.op_end = int(pc->ops.size());
       o.col = int(k);
       pc->others.push_back(o);

@@ -71,6 +71,23 @@ struct SharedRecipe {
 void FuseGates(const Circuit& c, int begin, int end, const FuseOptions& opt,
                std::vector<FusedOp>* out);
 
+// Register-engine segments: rewrite ops[begin, end) of an op list that runs on
+// a tile of L qubits so that it starts with a kOpRemap and every non-diagonal
+// op's targets lie in the register qubits of the latest remap (greedy segments
+// of at most RegBitsForTile(L) target qubits).  No-op when the tile size has no
+// register engine or an op is wider than 2 qubits.
+void InsertRemaps(std::vector<DevOp>* ops, size_t begin, int L);
+
+// Planner tuning (process-wide; 0 = default): register engine on/off and the
+// streaming path's tile qubits for one- and two-state passes.
+struct Tuning {
+  int reg_engine = 1;
+  int tile_single = 0;
+  int tile_two = 0;
+};
+Tuning GetTuning();
+void SetTuning(const Tuning& t);
+
 // Expand one gate into kernel convention (sorted bits, little-endian index).
 FusedOp GateToOp(const Gate& g, bool keep_controls_outside, bool structural_diag = false);
 // Gate kinds whose matrix is diagonal for every argument value.

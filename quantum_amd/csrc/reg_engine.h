@@ -1,0 +1,421 @@
+// Register-resident gate engine for a 2^L-amplitude state tile held in LDS.
+//
+// A workgroup of 512 threads keeps the tile in VGPRs: each thread holds 2^R
+// amplitudes (R = L - 9, 3..4) whose indices differ only in R "register
+// qubits"; the thread index fills the other L - R bits.  Ops whose targets
+// are register qubits (and diagonal ops on any qubits, with controls on any
+// qubits) are applied in registers with no LDS traffic and no barrier.  The
+// host planner splits the op list into segments whose non-diagonal targets
+// fit R qubits and starts each with a kOpRemap op carrying the segment's
+// register-qubit mask; a remap writes the registers back to LDS and reloads
+// them in the new layout (two barriers).  Between remaps LDS holds the tile in
+// a swizzled layout (x ^ ((x >> 4) & 15)) so that the strided register loads
+// and stores of a wavefront are spread over the LDS banks; on entry and exit
+// the layout is the natural one.
+//
+// Op matrices are read straight from global memory with wave-uniform
+// addresses.  Parameterised ops (reverse sweep) accumulate their FP64
+// gradient partials per thread, reduce them across the wavefront and hand the
+// sum to the caller's accumulator.
+#pragma once
+
+#include "lds_ops.h"
+
+namespace tfqx {
+namespace {
+
+static_assert(kThreads == (1 << kRegThreadBits), "register engine assumes 512 threads");
+
+// Register-resident complex amplitude.  A plain two-float struct (not the HIP
+// vector type, whose union members turn register arrays into scratch memory).
+struct cf {
+  float x, y;
+};
+__device__ __forceinline__ cf cmulr(cf a, cf b) {
+  return cf{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+__device__ __forceinline__ cf caddr(cf a, cf b) { return cf{a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ cf cconj(cf a) { return cf{a.x, -a.y}; }
+__device__ __forceinline__ cf ldm(const float2* __restrict__ M, int i) {
+  const float* f = reinterpret_cast<const float*>(M);
+  return cf{f[2 * i], f[2 * i + 1]};
+}
+
+__device__ __forceinline__ uint32_t swz(uint32_t x) { return x ^ ((x >> 4) & 15u); }
+
+__host__ __device__ constexpr int ins0(int g, int j) {
+  return ((g >> j) << (j + 1)) | (g & ((1 << j) - 1));
+}
+
+template <int R>
+struct RegMap {
+  uint32_t rpos[R];  // register qubit positions, ascending (wave-uniform)
+  uint32_t mask;     // wave-uniform
+  uint32_t tbase;    // this thread's bits on the non-register positions
+};
+
+// Tile index of register i (compile-time i) relative to tbase.
+template <int R>
+__device__ __forceinline__ uint32_t rdep(const RegMap<R>& m, int i) {
+  uint32_t o = 0;
+#pragma unroll
+  for (int j = 0; j < R; ++j)
+    if ((i >> j) & 1) o |= 1u << m.rpos[j];
+  return o;
+}
+
+template <int R>
+__device__ __forceinline__ void make_map(RegMap<R>& m, uint32_t mask, int L) {
+  mask = uint32_t(uniform(int(mask)));
+  m.mask = mask;
+  uint32_t mm = mask;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    m.rpos[j] = mm ? uint32_t(__builtin_ctz(mm)) : 0u;  // the planner always sets R bits
+    mm &= mm - 1;
+  }
+  const uint32_t tid = threadIdx.x;
+  uint32_t tb = 0;
+  int k = 0;
+  for (int p = 0; p < L; ++p) {
+    if (!((mask >> p) & 1u)) {
+      tb |= ((tid >> k) & 1u) << p;
+      ++k;
+    }
+  }
+  m.tbase = tb;
+}
+
+template <int R>
+__device__ __forceinline__ void reg_load(cf (&a)[1 << R], const float2* __restrict__ S,
+                                         const RegMap<R>& m, bool swizzled) {
+  const float* Sf = reinterpret_cast<const float*>(S);
+#pragma unroll
+  for (int i = 0; i < (1 << R); ++i) {
+    const uint32_t x = m.tbase | rdep(m, i);
+    const uint32_t p = 2u * (swizzled ? swz(x) : x);
+    a[i] = cf{Sf[p], Sf[p + 1]};
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void reg_store(const cf (&a)[1 << R], float2* __restrict__ S,
+                                          const RegMap<R>& m, bool swizzled) {
+  float* Sf = reinterpret_cast<float*>(S);
+#pragma unroll
+  for (int i = 0; i < (1 << R); ++i) {
+    const uint32_t x = m.tbase | rdep(m, i);
+    const uint32_t p = 2u * (swizzled ? swz(x) : x);
+    Sf[p] = a[i].x;
+    Sf[p + 1] = a[i].y;
+  }
+}
+
+// ------------------------------------------------------------------ dense ops
+template <int R, int J, bool DAG, bool BOTH>
+__device__ __forceinline__ void reg_dense1(cf (&a)[1 << R], cf (&b)[1 << R],
+                                           const float2* __restrict__ M, const RegMap<R>& m,
+                                           uint32_t cmask, uint32_t cval) {
+  cf u00 = ldm(M, 0), u01 = ldm(M, 1), u10 = ldm(M, 2), u11 = ldm(M, 3);
+  if (DAG) {
+    const cf t = u01;
+    u00 = cconj(u00);
+    u01 = cconj(u10);
+    u10 = cconj(t);
+    u11 = cconj(u11);
+  }
+#pragma unroll
+  for (int g = 0; g < (1 << (R - 1)); ++g) {
+    constexpr int one = 1 << J;
+    const int i0 = ins0(g, J);
+    if (cmask && (((m.tbase | rdep(m, i0)) & cmask) != cval)) continue;
+    const cf v0 = a[i0], v1 = a[i0 | one];
+    a[i0] = caddr(cmulr(u00, v0), cmulr(u01, v1));
+    a[i0 | one] = caddr(cmulr(u10, v0), cmulr(u11, v1));
+    if (BOTH) {
+      const cf w0 = b[i0], w1 = b[i0 | one];
+      b[i0] = caddr(cmulr(u00, w0), cmulr(u01, w1));
+      b[i0 | one] = caddr(cmulr(u10, w0), cmulr(u11, w1));
+    }
+  }
+}
+
+template <int R, int J0, int J1, bool DAG, bool BOTH>
+__device__ __forceinline__ void reg_dense2(cf (&a)[1 << R], cf (&b)[1 << R],
+                                           const float2* __restrict__ M, const RegMap<R>& m,
+                                           uint32_t cmask, uint32_t cval) {
+  cf u[16];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) u[r * 4 + c] = DAG ? cconj(ldm(M, c * 4 + r)) : ldm(M, r * 4 + c);
+#pragma unroll
+  for (int g = 0; g < (1 << (R - 2)); ++g) {
+    const int i0 = ins0(ins0(g, J0), J1);
+    if (cmask && (((m.tbase | rdep(m, i0)) & cmask) != cval)) continue;
+    int idx[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) idx[s] = i0 | ((s & 1) << J0) | ((s >> 1) << J1);
+    cf v[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) v[s] = a[idx[s]];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      cf acc = cmulr(u[r * 4], v[0]);
+#pragma unroll
+      for (int c = 1; c < 4; ++c) acc = caddr(acc, cmulr(u[r * 4 + c], v[c]));
+      a[idx[r]] = acc;
+    }
+    if (BOTH) {
+#pragma unroll
+      for (int s = 0; s < 4; ++s) v[s] = b[idx[s]];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        cf acc = cmulr(u[r * 4], v[0]);
+#pragma unroll
+        for (int c = 1; c < 4; ++c) acc = caddr(acc, cmulr(u[r * 4 + c], v[c]));
+        b[idx[r]] = acc;
+      }
+    }
+  }
+}
+
+template <int R, bool DAG, bool BOTH, int J = 0>
+__device__ __forceinline__ void dispatch1(int j, cf (&a)[1 << R], cf (&b)[1 << R], const float2* M,
+                                          const RegMap<R>& m, uint32_t cmask, uint32_t cval) {
+  if constexpr (J < R) {
+    if (j == J) reg_dense1<R, J, DAG, BOTH>(a, b, M, m, cmask, cval);
+    else dispatch1<R, DAG, BOTH, J + 1>(j, a, b, M, m, cmask, cval);
+  }
+}
+
+template <int R, bool DAG, bool BOTH, int J0 = 0, int J1 = 1>
+__device__ __forceinline__ void dispatch2(int j0, int j1, cf (&a)[1 << R], cf (&b)[1 << R],
+                                          const float2* M, const RegMap<R>& m, uint32_t cmask,
+                                          uint32_t cval) {
+  if constexpr (J1 < R) {
+    if (j0 == J0 && j1 == J1) {
+      reg_dense2<R, J0, J1, DAG, BOTH>(a, b, M, m, cmask, cval);
+    } else if constexpr (J1 + 1 < R) {
+      dispatch2<R, DAG, BOTH, J0, J1 + 1>(j0, j1, a, b, M, m, cmask, cval);
+    } else if constexpr (J0 + 2 < R) {
+      dispatch2<R, DAG, BOTH, J0 + 1, J0 + 2>(j0, j1, a, b, M, m, cmask, cval);
+    }
+  }
+}
+
+template <int R>
+__device__ __forceinline__ int reg_slot(const RegMap<R>& m, uint32_t q) {
+  return __builtin_popcount(m.mask & ((1u << q) - 1u));
+}
+
+// Dense op on a (and, BOTH, on b with the same matrix).
+template <int R, bool DAG, bool BOTH>
+__device__ __forceinline__ void reg_dense(cf (&a)[1 << R], cf (&b)[1 << R], const DevOp& op,
+                                          const float2* M, const RegMap<R>& m) {
+  if (op.k == 1) {
+    dispatch1<R, DAG, BOTH>(reg_slot(m, op.q[0]), a, b, M, m, op.cmask, op.cval);
+  } else {
+    dispatch2<R, DAG, BOTH>(reg_slot(m, op.q[0]), reg_slot(m, op.q[1]), a, b, M, m, op.cmask,
+                            op.cval);
+  }
+}
+
+// ------------------------------------------------------------------ diagonal ops
+// Targets and controls may be any tile qubits; k <= 2.
+template <int R, bool DAG, bool BOTH>
+__device__ __forceinline__ void reg_diag(cf (&a)[1 << R], cf (&b)[1 << R], const DevOp& op,
+                                         const float2* __restrict__ Dg, const RegMap<R>& m) {
+  const bool two = op.k == 2;
+  cf d0 = ldm(Dg, 0), d1 = ldm(Dg, 1);
+  cf d2 = two ? ldm(Dg, 2) : d0, d3 = two ? ldm(Dg, 3) : d1;
+  if (DAG) {
+    d0 = cconj(d0);
+    d1 = cconj(d1);
+    d2 = cconj(d2);
+    d3 = cconj(d3);
+  }
+  const uint32_t q0 = op.q[0], q1 = two ? op.q[1] : op.q[0];
+#pragma unroll
+  for (int i = 0; i < (1 << R); ++i) {
+    const uint32_t x = m.tbase | rdep(m, i);
+    if ((x & op.cmask) != op.cval) continue;
+    const uint32_t b0 = (x >> q0) & 1u, b1 = two ? ((x >> q1) & 1u) : 0u;
+    const cf d = b1 ? (b0 ? d3 : d2) : (b0 ? d1 : d0);
+    a[i] = cmulr(d, a[i]);
+    if (BOTH) b[i] = cmulr(d, b[i]);
+  }
+}
+
+// ------------------------------------------------------------------ parameterised ops
+// psi <- G^dag psi;  acc_d += Re<lam| dG_d psi>;  lam <- G^dag lam  (see apply_param).
+// Three sweeps over the registers so that only one matrix is live at a time.
+template <int R, int K, int J0, int J1>
+__device__ __forceinline__ int param_idx(int g, int s) {
+  const int i0 = K == 1 ? ins0(g, J0) : ins0(ins0(g, J0), J1);
+  return K == 1 ? (i0 | (s << J0)) : (i0 | ((s & 1) << J0) | ((s >> 1) << J1));
+}
+
+template <int R, int K, int J0, int J1>
+__device__ __forceinline__ void reg_dag_apply(cf (&a)[1 << R], const float2* __restrict__ G,
+                                              const DevOp& op, const RegMap<R>& m) {
+  constexpr int D = 1 << K;
+  cf gd[D * D];
+#pragma unroll
+  for (int r = 0; r < D; ++r)
+#pragma unroll
+    for (int c = 0; c < D; ++c) gd[r * D + c] = cconj(ldm(G, c * D + r));
+#pragma unroll
+  for (int g = 0; g < (1 << (R - K)); ++g) {
+    const int i0 = param_idx<R, K, J0, J1>(g, 0);
+    if (op.cmask && (((m.tbase | rdep(m, i0)) & op.cmask) != op.cval)) continue;
+    cf v[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) v[c] = a[param_idx<R, K, J0, J1>(g, c)];
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      cf s = cmulr(gd[r * D], v[0]);
+#pragma unroll
+      for (int c = 1; c < D; ++c) s = caddr(s, cmulr(gd[r * D + c], v[c]));
+      a[param_idx<R, K, J0, J1>(g, r)] = s;
+    }
+  }
+}
+
+template <int R, int K, int J0, int J1>
+__device__ __forceinline__ void reg_param_k(cf (&a)[1 << R], cf (&b)[1 << R], const DevOp& op,
+                                            const float2* __restrict__ mats, const RegMap<R>& m,
+                                            double* acc) {
+  constexpr int D = 1 << K;
+  reg_dag_apply<R, K, J0, J1>(a, mats + op.mat, op, m);
+  for (int d = 0; d < op.nderiv; ++d) {
+    const float2* dG = mats + (d == 0 ? op.dmat[0] : op.dmat[1]);  // static field access
+    cf dg[D * D];
+#pragma unroll
+    for (int i = 0; i < D * D; ++i) dg[i] = ldm(dG, i);
+    float part = 0.f;
+#pragma unroll
+    for (int g = 0; g < (1 << (R - K)); ++g) {
+      const int i0 = param_idx<R, K, J0, J1>(g, 0);
+      if (op.cmask && (((m.tbase | rdep(m, i0)) & op.cmask) != op.cval)) continue;
+#pragma unroll
+      for (int r = 0; r < D; ++r) {
+        cf s = cmulr(dg[r * D], a[param_idx<R, K, J0, J1>(g, 0)]);
+#pragma unroll
+        for (int c = 1; c < D; ++c) s = caddr(s, cmulr(dg[r * D + c], a[param_idx<R, K, J0, J1>(g, c)]));
+        const cf l = b[param_idx<R, K, J0, J1>(g, r)];
+        part += s.x * l.x + s.y * l.y;
+      }
+    }
+    if (d == 0) acc[0] += double(part);
+    else acc[1] += double(part);
+  }
+  reg_dag_apply<R, K, J0, J1>(b, mats + op.mat, op, m);
+}
+
+template <int R, int J0 = 0>
+__device__ __forceinline__ void param_dispatch1(int j, cf (&a)[1 << R], cf (&b)[1 << R],
+                                                const DevOp& op, const float2* mats,
+                                                const RegMap<R>& m, double* acc) {
+  if constexpr (J0 < R) {
+    if (j == J0) reg_param_k<R, 1, J0, 0>(a, b, op, mats, m, acc);
+    else param_dispatch1<R, J0 + 1>(j, a, b, op, mats, m, acc);
+  }
+}
+
+template <int R, int J0 = 0, int J1 = 1>
+__device__ __forceinline__ void param_dispatch2(int j0, int j1, cf (&a)[1 << R],
+                                                cf (&b)[1 << R], const DevOp& op,
+                                                const float2* mats, const RegMap<R>& m,
+                                                double* acc) {
+  if constexpr (J1 < R) {
+    if (j0 == J0 && j1 == J1) {
+      reg_param_k<R, 2, J0, J1>(a, b, op, mats, m, acc);
+    } else if constexpr (J1 + 1 < R) {
+      param_dispatch2<R, J0, J1 + 1>(j0, j1, a, b, op, mats, m, acc);
+    } else if constexpr (J0 + 2 < R) {
+      param_dispatch2<R, J0 + 1, J0 + 2>(j0, j1, a, b, op, mats, m, acc);
+    }
+  }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// ------------------------------------------------------------------ driver
+// Apply ops [b, e) to the LDS tile A (and B, reverse sweep) of 2^L amplitudes.
+// REV = false: apply each op's matrix to A.  REV = true: un-apply (M^dagger)
+// non-parameterised ops on A and B; parameterised ops run the adjoint step
+// and call acc_fn(slot, column, wave_partial) from lane 0 of each wavefront
+// per derivative, slot counting the derivative matrices of earlier
+// parameterised ops and column being the derivative's symbol column.
+// Ops [b, e) must start with a kOpRemap.  The caller synchronises before
+// (tile written) and the engine synchronises before returning.
+template <int R, bool REV, typename AccFn>
+__device__ __forceinline__ void reg_run(float2* __restrict__ A, float2* __restrict__ B, int L,
+                        const DevOp* __restrict__ ops, int b, int e,
+                        const float2* __restrict__ mats, AccFn&& acc_fn) {
+  cf ra[1 << R];
+  cf rb[1 << R];  // unused when !REV
+  RegMap<R> m;
+  bool have = false, swizzled = false;
+  int slot = 0;
+  for (int i = b; i < e; ++i) {
+    const DevOp op = load_op(ops + i);
+    if (op.type == kOpRemap) {
+      if (have) {
+        __syncthreads();
+        reg_store<R>(ra, A, m, true);
+        if (REV) reg_store<R>(rb, B, m, true);
+        swizzled = true;
+      }
+      __syncthreads();
+      make_map<R>(m, op.cmask, L);
+      reg_load<R>(ra, A, m, swizzled);
+      if (REV) reg_load<R>(rb, B, m, swizzled);
+      have = true;
+      continue;
+    }
+    if (op.type == kOpDiag) {
+      reg_diag<R, REV, REV>(ra, rb, op, mats + op.mat, m);
+    } else if (REV && op.type == kOpParam) {
+      double acc[kMaxDeriv] = {0.0, 0.0};
+      if (op.k == 1) param_dispatch1<R>(reg_slot(m, op.q[0]), ra, rb, op, mats, m, acc);
+      else param_dispatch2<R>(reg_slot(m, op.q[0]), reg_slot(m, op.q[1]), ra, rb, op, mats, m, acc);
+      const double w0 = wave_sum(acc[0]);
+      if ((threadIdx.x & 63) == 0) acc_fn(slot, op.col[0], w0);
+      if (op.nderiv > 1) {
+        const double w1 = wave_sum(acc[1]);
+        if ((threadIdx.x & 63) == 0) acc_fn(slot + 1, op.col[1], w1);
+      }
+      slot += op.nderiv;
+    } else {
+      reg_dense<R, REV, REV>(ra, rb, op, mats + op.mat, m);
+    }
+  }
+  if (have) {
+    __syncthreads();
+    reg_store<R>(ra, A, m, false);
+    if (REV) reg_store<R>(rb, B, m, false);
+    __syncthreads();
+  }
+}
+
+// Run ops [b, e) on a tile of L qubits with the register engine when the op
+// list was planned for it (starts with a remap); returns false otherwise.
+template <bool REV, typename AccFn>
+__device__ __forceinline__ bool reg_run_any(float2* A, float2* B, int L, const DevOp* ops, int b,
+                                            int e, const float2* mats, AccFn&& acc_fn) {
+  if (b >= e) return false;
+  if (uniform(int(ops[b].type)) != kOpRemap) return false;
+  switch (RegBitsForTile(L)) {
+    case 3: reg_run<3, REV>(A, B, L, ops, b, e, mats, acc_fn); return true;
+    case 4: reg_run<4, REV>(A, B, L, ops, b, e, mats, acc_fn); return true;
+    default: return false;
+  }
+}
+
+}  // namespace
+}  // namespace tfqx

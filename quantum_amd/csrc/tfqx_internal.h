@@ -21,7 +21,15 @@ enum OpType : uint8_t {
   kOpUnitary = 0,   // apply M (forward) or M^dagger (adjoint sweep) on targets
   kOpParam = 1,     // adjoint: psi <- G^dag psi; grad += 2Re<lam|dG psi>; lam <- G^dag lam
   kOpDiag = 2,      // diagonal M: only the 2^k diagonal entries are stored
+  kOpRemap = 3,     // register engine: reload the tile with cmask = register-qubit mask
 };
+
+// Register engine (reg_engine.h): 2^9 threads per workgroup, each holding
+// 2^R amplitudes of a 2^L tile in registers, R = L - 9 for tiles of 12..13
+// qubits; other tile sizes use the LDS path.  The planner inserts kOpRemap
+// ops exactly when this returns nonzero and every op of the list has k <= 2.
+constexpr int kRegThreadBits = 9;
+constexpr int RegBitsForTile(int L) { return (L >= 12 && L <= 13) ? L - kRegThreadBits : 0; }
 
 // One op on a tile.  Qubit positions are tile-local bit indices (ascending).
 // Matrix layout: row-major 2^k x 2^k complex64, index bit j <-> q[j].

@@ -145,3 +145,32 @@ def test_rebind_only_for_one_program_tiled_over_the_batch():
                             [[W.tfim(wq, 1.0)]] * 2)
     with pytest.raises(InvalidArgumentError, match="cannot be rebound"):
         big.rebind(np.zeros((2, len(wnames))))
+
+
+@pytest.mark.parametrize("width", [1, 2, 3, 4])
+def test_fuser_decisions_match_host_replica(width):
+    """The planner's fusion decisions equal the oracle's restatement exactly."""
+    from oracle import fuser
+    rng = np.random.default_rng(100 + width)
+    for trial in range(12):
+        q = C.LineQubit.range(int(rng.integers(2, 9)))
+        circ = W.random_circuit(rng, q, int(rng.integers(5, 60)), p_control=0.25)
+        data = S.serialize_circuit(circ)
+        got = _lib.fusion_decisions(data, width=width)
+        want = fuser.fuse(fuser.program_masks(data), width)
+        assert got == want, (trial, width)
+
+
+def test_fuser_replica_pinned_by_hand_example():
+    """A hand-worked case: H(0) CZ(0,1) H(2) CNOT(1,2) X(3) at width 2."""
+    from oracle import fuser
+    q = C.LineQubit.range(4)
+    circ = C.Circuit.from_moments([[C.H(q[0])], [C.CZ(q[0], q[1])], [C.H(q[2])],
+                                   [C.CNOT(q[1], q[2])], [C.X(q[3])]])
+    data = S.serialize_circuit(circ)
+    # qubit q -> bit 3 - q.  H0 opens {3}; CZ joins -> {2,3}; H2 opens {1};
+    # CNOT(1,2) touches {2,3} and {1}: union 3 qubits > 2 -> both close, new
+    # block {1,2}; X3 opens {0}; end: {1,2} then {0}.
+    want = [((2, 3), False, (0, 1)), ((1,), False, (2,)), ((1, 2), False, (3,)), ((0,), False, (4,))]
+    assert fuser.fuse(fuser.program_masks(data), 2) == want
+    assert _lib.fusion_decisions(data, width=2) == want

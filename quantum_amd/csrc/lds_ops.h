@@ -25,6 +25,15 @@ __device__ __forceinline__ float2 cmul_conj_a(float2 a, float2 b) {  // conj(a) 
   return make_float2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
 }
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+// acc + m * v as four FMAs (a product followed by an add leaves separate
+// multiplies and adds).
+__device__ __forceinline__ float2 cmacf(float2 acc, float2 m, float2 v) {
+  float re = __builtin_fmaf(m.x, v.x, acc.x);
+  re = __builtin_fmaf(-m.y, v.y, re);
+  float im = __builtin_fmaf(m.x, v.y, acc.y);
+  im = __builtin_fmaf(m.y, v.x, im);
+  return make_float2(re, im);
+}
 
 __device__ __forceinline__ uint32_t insert_zero(uint32_t x, uint32_t q) {
   return ((x >> q) << (q + 1)) | (x & ((1u << q) - 1u));
@@ -181,7 +190,7 @@ __device__ __forceinline__ void apply_dense(float2* __restrict__ s, int n, const
       for (int c = 0; c < D; ++c) {
         float2 m = DAG ? M[c * D + r] : M[r * D + c];
         if (DAG) m.y = -m.y;
-        acc = cadd(acc, cmul(m, v[c]));
+        acc = c == 0 ? cmul(m, v[c]) : cmacf(acc, m, v[c]);
       }
       s[base | dep[r]] = acc;
     }
@@ -227,8 +236,8 @@ __device__ __forceinline__ void apply_1q(float2* __restrict__ s, int n, int q, c
   for (uint32_t g = threadIdx.x; g < ngroups; g += kThreads) {
     const uint32_t b = insert_zero(g, q);
     const float2 a0 = s[b], a1 = s[b | (1u << q)];
-    s[b] = cadd(cmul(m00, a0), cmul(m01, a1));
-    s[b | (1u << q)] = cadd(cmul(m10, a0), cmul(m11, a1));
+    s[b] = cmacf(cmul(m00, a0), m01, a1);
+    s[b | (1u << q)] = cmacf(cmul(m10, a0), m11, a1);
   }
 }
 
@@ -266,7 +275,11 @@ __device__ __forceinline__ void apply_param(float2* __restrict__ psi, float2* __
     for (int r = 0; r < D; ++r) {
       float2 s = make_float2(0.f, 0.f);
 #pragma unroll
-      for (int c = 0; c < D; ++c) s = cadd(s, cmul_conj_a(G[c * D + r], a[c]));
+      for (int c = 0; c < D; ++c) {
+        float2 g = G[c * D + r];
+        g.y = -g.y;
+        s = c == 0 ? cmul(g, a[c]) : cmacf(s, g, a[c]);
+      }
       t[r] = s;
     }
 #pragma unroll
@@ -280,7 +293,7 @@ __device__ __forceinline__ void apply_param(float2* __restrict__ psi, float2* __
       for (int r = 0; r < D; ++r) {
         float2 s = make_float2(0.f, 0.f);
 #pragma unroll
-        for (int c = 0; c < D; ++c) s = cadd(s, cmul(dG[r * D + c], t[c]));
+        for (int c = 0; c < D; ++c) s = c == 0 ? cmul(dG[r * D], t[0]) : cmacf(s, dG[r * D + c], t[c]);
         part += s.x * b[r].x + s.y * b[r].y;  // Re(conj(s) * lam)
       }
       acc[d] += double(part);
@@ -289,7 +302,11 @@ __device__ __forceinline__ void apply_param(float2* __restrict__ psi, float2* __
     for (int r = 0; r < D; ++r) {
       float2 s = make_float2(0.f, 0.f);
 #pragma unroll
-      for (int c = 0; c < D; ++c) s = cadd(s, cmul_conj_a(G[c * D + r], b[c]));
+      for (int c = 0; c < D; ++c) {
+        float2 g = G[c * D + r];
+        g.y = -g.y;
+        s = c == 0 ? cmul(g, b[c]) : cmacf(s, g, b[c]);
+      }
       lam[base | dep[r]] = s;
     }
   }

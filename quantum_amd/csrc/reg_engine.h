@@ -35,6 +35,14 @@ __device__ __forceinline__ cf cmulr(cf a, cf b) {
   return cf{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
 __device__ __forceinline__ cf caddr(cf a, cf b) { return cf{a.x + b.x, a.y + b.y}; }
+// acc + u * v as four FMAs.
+__device__ __forceinline__ cf cmac(cf acc, cf u, cf v) {
+  acc.x = __builtin_fmaf(u.x, v.x, acc.x);
+  acc.x = __builtin_fmaf(-u.y, v.y, acc.x);
+  acc.y = __builtin_fmaf(u.x, v.y, acc.y);
+  acc.y = __builtin_fmaf(u.y, v.x, acc.y);
+  return acc;
+}
 __device__ __forceinline__ cf cconj(cf a) { return cf{a.x, -a.y}; }
 __device__ __forceinline__ cf ldm(const float2* __restrict__ M, int i) {
   const float* f = reinterpret_cast<const float*>(M);
@@ -130,12 +138,12 @@ __device__ __forceinline__ void reg_dense1(cf (&a)[1 << R], cf (&b)[1 << R],
     const int i0 = ins0(g, J);
     if (cmask && (((m.tbase | rdep(m, i0)) & cmask) != cval)) continue;
     const cf v0 = a[i0], v1 = a[i0 | one];
-    a[i0] = caddr(cmulr(u00, v0), cmulr(u01, v1));
-    a[i0 | one] = caddr(cmulr(u10, v0), cmulr(u11, v1));
+    a[i0] = cmac(cmulr(u00, v0), u01, v1);
+    a[i0 | one] = cmac(cmulr(u10, v0), u11, v1);
     if (BOTH) {
       const cf w0 = b[i0], w1 = b[i0 | one];
-      b[i0] = caddr(cmulr(u00, w0), cmulr(u01, w1));
-      b[i0 | one] = caddr(cmulr(u10, w0), cmulr(u11, w1));
+      b[i0] = cmac(cmulr(u00, w0), u01, w1);
+      b[i0 | one] = cmac(cmulr(u10, w0), u11, w1);
     }
   }
 }
@@ -163,7 +171,7 @@ __device__ __forceinline__ void reg_dense2(cf (&a)[1 << R], cf (&b)[1 << R],
     for (int r = 0; r < 4; ++r) {
       cf acc = cmulr(u[r * 4], v[0]);
 #pragma unroll
-      for (int c = 1; c < 4; ++c) acc = caddr(acc, cmulr(u[r * 4 + c], v[c]));
+      for (int c = 1; c < 4; ++c) acc = cmac(acc, u[r * 4 + c], v[c]);
       a[idx[r]] = acc;
     }
     if (BOTH) {
@@ -173,7 +181,7 @@ __device__ __forceinline__ void reg_dense2(cf (&a)[1 << R], cf (&b)[1 << R],
       for (int r = 0; r < 4; ++r) {
         cf acc = cmulr(u[r * 4], v[0]);
 #pragma unroll
-        for (int c = 1; c < 4; ++c) acc = caddr(acc, cmulr(u[r * 4 + c], v[c]));
+        for (int c = 1; c < 4; ++c) acc = cmac(acc, u[r * 4 + c], v[c]);
         b[idx[r]] = acc;
       }
     }
@@ -276,7 +284,7 @@ __device__ __forceinline__ void reg_dag_apply(cf (&a)[1 << R], const float2* __r
     for (int r = 0; r < D; ++r) {
       cf s = cmulr(gd[r * D], v[0]);
 #pragma unroll
-      for (int c = 1; c < D; ++c) s = caddr(s, cmulr(gd[r * D + c], v[c]));
+      for (int c = 1; c < D; ++c) s = cmac(s, gd[r * D + c], v[c]);
       a[param_idx<R, K, J0, J1>(g, r)] = s;
     }
   }
@@ -302,7 +310,7 @@ __device__ __forceinline__ void reg_param_k(cf (&a)[1 << R], cf (&b)[1 << R], co
       for (int r = 0; r < D; ++r) {
         cf s = cmulr(dg[r * D], a[param_idx<R, K, J0, J1>(g, 0)]);
 #pragma unroll
-        for (int c = 1; c < D; ++c) s = caddr(s, cmulr(dg[r * D + c], a[param_idx<R, K, J0, J1>(g, c)]));
+        for (int c = 1; c < D; ++c) s = cmac(s, dg[r * D + c], a[param_idx<R, K, J0, J1>(g, c)]);
         const cf l = b[param_idx<R, K, J0, J1>(g, r)];
         part += s.x * l.x + s.y * l.y;
       }

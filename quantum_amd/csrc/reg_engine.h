@@ -120,10 +120,10 @@ __device__ __forceinline__ void reg_store(const cf (&a)[1 << R], float2* __restr
 }
 
 // ------------------------------------------------------------------ dense ops
-template <int R, int J, bool DAG, bool BOTH>
-__device__ __forceinline__ void reg_dense1(cf (&a)[1 << R], cf (&b)[1 << R],
-                                           const float2* __restrict__ M, const RegMap<R>& m,
-                                           uint32_t cmask, uint32_t cval) {
+template <int R, int J, bool DAG, bool BOTH, bool CTRL>
+__device__ __forceinline__ void reg_dense1_c(cf (&a)[1 << R], cf (&b)[1 << R],
+                                             const float2* __restrict__ M, const RegMap<R>& m,
+                                             uint32_t cmask, uint32_t cval) {
   cf u00 = ldm(M, 0), u01 = ldm(M, 1), u10 = ldm(M, 2), u11 = ldm(M, 3);
   if (DAG) {
     const cf t = u01;
@@ -136,7 +136,7 @@ __device__ __forceinline__ void reg_dense1(cf (&a)[1 << R], cf (&b)[1 << R],
   for (int g = 0; g < (1 << (R - 1)); ++g) {
     constexpr int one = 1 << J;
     const int i0 = ins0(g, J);
-    if (cmask && (((m.tbase | rdep(m, i0)) & cmask) != cval)) continue;
+    if (CTRL && (((m.tbase | rdep(m, i0)) & cmask) != cval)) continue;
     const cf v0 = a[i0], v1 = a[i0 | one];
     a[i0] = cmac(cmulr(u00, v0), u01, v1);
     a[i0 | one] = cmac(cmulr(u10, v0), u11, v1);
@@ -148,10 +148,10 @@ __device__ __forceinline__ void reg_dense1(cf (&a)[1 << R], cf (&b)[1 << R],
   }
 }
 
-template <int R, int J0, int J1, bool DAG, bool BOTH>
-__device__ __forceinline__ void reg_dense2(cf (&a)[1 << R], cf (&b)[1 << R],
-                                           const float2* __restrict__ M, const RegMap<R>& m,
-                                           uint32_t cmask, uint32_t cval) {
+template <int R, int J0, int J1, bool DAG, bool BOTH, bool CTRL>
+__device__ __forceinline__ void reg_dense2_c(cf (&a)[1 << R], cf (&b)[1 << R],
+                                             const float2* __restrict__ M, const RegMap<R>& m,
+                                             uint32_t cmask, uint32_t cval) {
   cf u[16];
 #pragma unroll
   for (int r = 0; r < 4; ++r)
@@ -160,7 +160,7 @@ __device__ __forceinline__ void reg_dense2(cf (&a)[1 << R], cf (&b)[1 << R],
 #pragma unroll
   for (int g = 0; g < (1 << (R - 2)); ++g) {
     const int i0 = ins0(ins0(g, J0), J1);
-    if (cmask && (((m.tbase | rdep(m, i0)) & cmask) != cval)) continue;
+    if (CTRL && (((m.tbase | rdep(m, i0)) & cmask) != cval)) continue;
     int idx[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) idx[s] = i0 | ((s & 1) << J0) | ((s >> 1) << J1);
@@ -186,6 +186,24 @@ __device__ __forceinline__ void reg_dense2(cf (&a)[1 << R], cf (&b)[1 << R],
       }
     }
   }
+}
+
+// Gates without control qubits (the common case) run a check-free loop: a
+// per-group predicate makes the compiler copy every output register first.
+template <int R, int J, bool DAG, bool BOTH>
+__device__ __forceinline__ void reg_dense1(cf (&a)[1 << R], cf (&b)[1 << R],
+                                           const float2* __restrict__ M, const RegMap<R>& m,
+                                           uint32_t cmask, uint32_t cval) {
+  if (cmask) reg_dense1_c<R, J, DAG, BOTH, true>(a, b, M, m, cmask, cval);
+  else reg_dense1_c<R, J, DAG, BOTH, false>(a, b, M, m, cmask, cval);
+}
+
+template <int R, int J0, int J1, bool DAG, bool BOTH>
+__device__ __forceinline__ void reg_dense2(cf (&a)[1 << R], cf (&b)[1 << R],
+                                           const float2* __restrict__ M, const RegMap<R>& m,
+                                           uint32_t cmask, uint32_t cval) {
+  if (cmask) reg_dense2_c<R, J0, J1, DAG, BOTH, true>(a, b, M, m, cmask, cval);
+  else reg_dense2_c<R, J0, J1, DAG, BOTH, false>(a, b, M, m, cmask, cval);
 }
 
 template <int R, bool DAG, bool BOTH, int J = 0>
@@ -244,10 +262,11 @@ __device__ __forceinline__ void reg_diag(cf (&a)[1 << R], cf (&b)[1 << R], const
     d3 = cconj(d3);
   }
   const uint32_t q0 = op.q[0], q1 = two ? op.q[1] : op.q[0];
+  const bool ctrl = op.cmask != 0;
 #pragma unroll
   for (int i = 0; i < (1 << R); ++i) {
     const uint32_t x = m.tbase | rdep(m, i);
-    if ((x & op.cmask) != op.cval) continue;
+    if (ctrl && (x & op.cmask) != op.cval) continue;
     const uint32_t b0 = (x >> q0) & 1u, b1 = two ? ((x >> q1) & 1u) : 0u;
     const cf d = b1 ? (b0 ? d3 : d2) : (b0 ? d1 : d0);
     a[i] = cmulr(d, a[i]);
@@ -264,7 +283,7 @@ __device__ __forceinline__ int param_idx(int g, int s) {
   return K == 1 ? (i0 | (s << J0)) : (i0 | ((s & 1) << J0) | ((s >> 1) << J1));
 }
 
-template <int R, int K, int J0, int J1>
+template <int R, int K, int J0, int J1, bool CTRL>
 __device__ __forceinline__ void reg_dag_apply(cf (&a)[1 << R], const float2* __restrict__ G,
                                               const DevOp& op, const RegMap<R>& m) {
   constexpr int D = 1 << K;
@@ -276,7 +295,7 @@ __device__ __forceinline__ void reg_dag_apply(cf (&a)[1 << R], const float2* __r
 #pragma unroll
   for (int g = 0; g < (1 << (R - K)); ++g) {
     const int i0 = param_idx<R, K, J0, J1>(g, 0);
-    if (op.cmask && (((m.tbase | rdep(m, i0)) & op.cmask) != op.cval)) continue;
+    if (CTRL && (((m.tbase | rdep(m, i0)) & op.cmask) != op.cval)) continue;
     cf v[D];
 #pragma unroll
     for (int c = 0; c < D; ++c) v[c] = a[param_idx<R, K, J0, J1>(g, c)];
@@ -290,12 +309,12 @@ __device__ __forceinline__ void reg_dag_apply(cf (&a)[1 << R], const float2* __r
   }
 }
 
-template <int R, int K, int J0, int J1>
-__device__ __forceinline__ void reg_param_k(cf (&a)[1 << R], cf (&b)[1 << R], const DevOp& op,
-                                            const float2* __restrict__ mats, const RegMap<R>& m,
-                                            double* acc) {
+template <int R, int K, int J0, int J1, bool CTRL>
+__device__ __forceinline__ void reg_param_kc(cf (&a)[1 << R], cf (&b)[1 << R], const DevOp& op,
+                                             const float2* __restrict__ mats, const RegMap<R>& m,
+                                             double* acc) {
   constexpr int D = 1 << K;
-  reg_dag_apply<R, K, J0, J1>(a, mats + op.mat, op, m);
+  reg_dag_apply<R, K, J0, J1, CTRL>(a, mats + op.mat, op, m);
   for (int d = 0; d < op.nderiv; ++d) {
     const float2* dG = mats + (d == 0 ? op.dmat[0] : op.dmat[1]);  // static field access
     cf dg[D * D];
@@ -305,7 +324,7 @@ __device__ __forceinline__ void reg_param_k(cf (&a)[1 << R], cf (&b)[1 << R], co
 #pragma unroll
     for (int g = 0; g < (1 << (R - K)); ++g) {
       const int i0 = param_idx<R, K, J0, J1>(g, 0);
-      if (op.cmask && (((m.tbase | rdep(m, i0)) & op.cmask) != op.cval)) continue;
+      if (CTRL && (((m.tbase | rdep(m, i0)) & op.cmask) != op.cval)) continue;
 #pragma unroll
       for (int r = 0; r < D; ++r) {
         cf s = cmulr(dg[r * D], a[param_idx<R, K, J0, J1>(g, 0)]);
@@ -318,7 +337,15 @@ __device__ __forceinline__ void reg_param_k(cf (&a)[1 << R], cf (&b)[1 << R], co
     if (d == 0) acc[0] += double(part);
     else acc[1] += double(part);
   }
-  reg_dag_apply<R, K, J0, J1>(b, mats + op.mat, op, m);
+  reg_dag_apply<R, K, J0, J1, CTRL>(b, mats + op.mat, op, m);
+}
+
+template <int R, int K, int J0, int J1>
+__device__ __forceinline__ void reg_param_k(cf (&a)[1 << R], cf (&b)[1 << R], const DevOp& op,
+                                            const float2* __restrict__ mats, const RegMap<R>& m,
+                                            double* acc) {
+  if (op.cmask) reg_param_kc<R, K, J0, J1, true>(a, b, op, mats, m, acc);
+  else reg_param_kc<R, K, J0, J1, false>(a, b, op, mats, m, acc);
 }
 
 template <int R, int J0 = 0>

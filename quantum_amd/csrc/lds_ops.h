@@ -41,10 +41,21 @@ __device__ __forceinline__ uint32_t insert_zero(uint32_t x, uint32_t q) {
 
 __device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
+// Read-only plan data (op descriptors, gate matrices) is not written while a
+// kernel runs, so it can be read through the constant address space: with a
+// wave-uniform address the compiler then emits scalar loads into SGPRs
+// instead of per-lane vector loads into VGPRs.
+template <typename T>
+using kconst = const __attribute__((address_space(4))) T;
+template <typename T>
+__device__ __forceinline__ kconst<T>* as_const(const T* p) {
+  return (kconst<T>*)(p);
+}
+
 // Load an op descriptor into scalar registers (all lanes read the same op).
 __device__ __forceinline__ DevOp load_op(const DevOp* p) {
   // Field-wise from 32-bit words so the descriptor stays in registers.
-  const uint32_t* src = reinterpret_cast<const uint32_t*>(p);
+  kconst<uint32_t>* src = as_const(reinterpret_cast<const uint32_t*>(p));
   uint32_t w[10];
 #pragma unroll
   for (int i = 0; i < 10; ++i) w[i] = __builtin_amdgcn_readfirstlane(src[i]);

@@ -190,25 +190,88 @@ void InsertRemaps(std::vector<DevOp>* ops, size_t begin, int L) {
     if ((*ops)[i].k > 2) return;
   std::vector<DevOp> in(ops->begin() + long(begin), ops->end());
   ops->resize(begin);
-  size_t i = 0;
-  while (i < in.size()) {
-    uint32_t S = 0;
-    size_t j = i;
-    for (; j < in.size(); ++j) {
-      const DevOp& o = in[j];
-      if (o.type == kOpDiag) continue;
-      uint32_t T = 0;
-      for (int t = 0; t < o.k; ++t) T |= 1u << o.q[t];
-      if (Popc(S | T) > R) break;
-      S |= T;
+  const size_t n = in.size();
+  // Support (targets and controls) and register needs (targets of
+  // non-diagonal ops) of every op.  Two ops commute when their supports are
+  // disjoint or both are diagonal; a segment may take an op ahead of earlier
+  // ops it commutes with.
+  std::vector<uint32_t> sup(n), need(n);
+  std::vector<char> diag(n), done(n, 0);
+  for (size_t i = 0; i < n; ++i) {
+    const DevOp& o = in[i];
+    uint32_t T = 0;
+    for (int t = 0; t < o.k; ++t) T |= 1u << o.q[t];
+    diag[i] = o.type == kOpDiag;
+    sup[i] = T | o.cmask;
+    need[i] = diag[i] ? 0u : T;
+  }
+  const uint32_t all = L >= 32 ? ~0u : (1u << L) - 1u;
+  // Ops scanned past the first pending one per segment (bounds planning time).
+  constexpr size_t kWindow = 96;
+  // Scan pending ops in order from `first`: an op joins the segment when it
+  // commutes with every op skipped before it and its targets fit the register
+  // set (grown up to R qubits when `grow`).  Returns the register set and,
+  // when `take` is given, the joined ops; `cost` counts joined non-diagonal ops.
+  auto scan = [&](size_t first, uint32_t S, bool grow, std::vector<size_t>* take, int* cost) {
+    uint32_t blocked = 0;  // supports of skipped ops (diagonal ones tracked apart)
+    uint32_t blocked_diag = 0;
+    int c = 0;
+    size_t seen = 0;
+    for (size_t j = first; j < n && seen < kWindow; ++j) {
+      if (done[j]) continue;
+      ++seen;
+      // A diagonal op commutes with skipped diagonal ops.
+      const uint32_t hit = diag[j] ? blocked : (blocked | blocked_diag);
+      bool ok = (sup[j] & hit) == 0;
+      if (ok && need[j]) ok = grow ? Popc(S | need[j]) <= R : (need[j] & ~S) == 0;
+      if (!ok) {
+        if (diag[j]) blocked_diag |= sup[j];
+        else blocked |= sup[j];
+        if ((blocked | blocked_diag) == all && blocked == all) break;
+        continue;
+      }
+      if (grow) S |= need[j];
+      if (need[j]) ++c;
+      if (take) take->push_back(j);
     }
+    if (cost) *cost = c;
+    return S;
+  };
+  size_t first = 0;
+  std::vector<size_t> take;
+  while (first < n) {
+    // Seed the register set with the targets of one of the first few pending
+    // non-diagonal ops (up to 8) and keep the seed whose segment takes the most ops.
+    uint32_t best_S = 0;
+    int best = -1;
+    int seeds = 0;
+    for (size_t j = first; j < n && seeds < 8; ++j) {
+      if (done[j] || !need[j]) continue;
+      ++seeds;
+      int c = 0;
+      const uint32_t S = scan(first, need[j], true, nullptr, &c);
+      if (c > best) {
+        best = c;
+        best_S = S;
+      }
+    }
+    uint32_t S = best_S;
+    if (best < 0) S = scan(first, 0, true, nullptr, nullptr);  // only diagonal ops left
+    // Pad with the targets of the next ops that did not fit, then low bits.
+    for (size_t j = first; j < n && Popc(S) < R; ++j)
+      if (!done[j] && (need[j] & ~S) && Popc(S | need[j]) <= R) S |= need[j];
     for (int b = 0; b < L && Popc(S) < R; ++b) S |= 1u << b;
+    take.clear();
+    scan(first, S, false, &take, nullptr);
     DevOp rm{};
     rm.type = kOpRemap;
     rm.cmask = S;
     ops->push_back(rm);
-    for (size_t t = i; t < j; ++t) ops->push_back(in[t]);
-    i = j;
+    for (size_t t : take) {
+      ops->push_back(in[t]);
+      done[t] = 1;
+    }
+    while (first < n && done[first]) ++first;
   }
 }
 

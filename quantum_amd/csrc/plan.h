@@ -73,9 +73,11 @@ void FuseGates(const Circuit& c, int begin, int end, const FuseOptions& opt,
 
 // Register-engine segments: rewrite ops[begin, end) of an op list that runs on
 // a tile of L qubits so that it starts with a kOpRemap and every non-diagonal
-// op's targets lie in the register qubits of the latest remap (greedy segments
-// of at most RegBitsForTile(L) target qubits).  No-op when the tile size has no
-// register engine or an op is wider than 2 qubits.
+// op's targets lie in the register qubits of the latest remap (segments of at
+// most RegBitsForTile(L) target qubits).  A segment may take an op ahead of
+// earlier ops that commute with it (disjoint supports, or both diagonal), so
+// the op order changes only between commuting ops.  No-op when the tile size
+// has no register engine or an op is wider than 2 qubits.
 void InsertRemaps(std::vector<DevOp>* ops, size_t begin, int L);
 
 // Planner tuning (process-wide; 0 = default): register engine on/off and the

@@ -14,7 +14,7 @@
 // the layout is the natural one.
 //
 // Op matrices are read straight from global memory with wave-uniform
-// addresses.  Parameterised ops (reverse sweep) accumulate their FP64
+// addresses, as scalar loads.  Parameterised ops (reverse sweep) accumulate their FP64
 // gradient partials per thread, reduce them across the wavefront and hand the
 // sum to the caller's accumulator.
 #pragma once

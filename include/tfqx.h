@@ -217,8 +217,9 @@ int tfqx_gemm_bf16(int32_t M, int32_t N, int32_t K, const void* A, int32_t lda, 
                    const void* B, int32_t ldb, int32_t transB, float* C, int32_t ldc, float alpha,
                    float beta, void* stream, char* err, int64_t err_len);
 
-/* Planner tuning for later plans: register engine on (1) / off (0) for the
- * batched kernel's tiles of 12-13 qubits, and the streaming path's tile
+/* Planner tuning for later plans: register engine off (0), on for the
+ * batched kernel's tiles of 12-13 qubits (1), or on for those and the
+ * streaming path's passes (2, the default); and the streaming path's tile
  * qubits for one-state and two-state passes (0 = defaults 13 and 11; at most
  * 14 and 13).  No reference counterpart (qsim's CPU path has no tiles). */
 void tfqx_set_tuning(int32_t register_engine, int32_t tile_single, int32_t tile_two);

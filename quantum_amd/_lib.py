@@ -256,9 +256,10 @@ def fusion_decisions(program, symbol_names=(), symbol_values=(), width=2):
     return out
 
 
-def set_tuning(register_engine=True, tile_single=0, tile_two=0):
-    """Planner tuning for later plans (see include/tfqx.h: tfqx_set_tuning)."""
-    lib().tfqx_set_tuning(int(bool(register_engine)), int(tile_single), int(tile_two))
+def set_tuning(register_engine=2, tile_single=0, tile_two=0):
+    """Planner tuning for later plans (see include/tfqx.h: tfqx_set_tuning).
+    register_engine: 0 off, 1 batched kernel, 2 batched and streaming (default)."""
+    lib().tfqx_set_tuning(int(register_engine), int(tile_single), int(tile_two))
 
 
 def set_fusion_width(width):

@@ -418,7 +418,7 @@ void tfqx_set_fusion_width(int32_t width) { g_fusion_width.store(width); }
 
 void tfqx_set_tuning(int32_t register_engine, int32_t tile_single, int32_t tile_two) {
   Tuning t;
-  t.reg_engine = register_engine != 0;
+  t.reg_engine = std::max(0, std::min(2, int(register_engine)));
   t.tile_single = std::max(0, int(tile_single));
   t.tile_two = std::max(0, int(tile_two));
   SetTuning(t);

@@ -155,10 +155,9 @@ void AddPasses(LargePlan* lp, const std::vector<FusedOp>& ops, int n, int L, Pas
     if (Popc(OpMask(f)) > std::min(L, n)) lp->too_wide = true;
   if (lp->too_wide) return;
   std::vector<DevPass> passes;
-  // The streaming passes keep the LDS path: with ~10 ops per pass the
-  // register engine's remaps cost more than they save (measured on the
-  // 30-qubit grid circuit).
-  SchedulePasses(ops, n, L, &passes, &lp->ops, &lp->mats, /*register_engine=*/false);
+  // The streaming passes keep the LDS path unless tuning level 2 asks for the
+  // register engine there too.
+  SchedulePasses(ops, n, L, &passes, &lp->ops, &lp->mats, GetTuning().reg_engine >= 2);
   pp->passes = passes;
 }
 

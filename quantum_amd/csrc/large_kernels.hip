@@ -43,8 +43,10 @@ struct PassArgs {
   int init;
 };
 
+// One-state passes use 64 KiB of LDS for 13-qubit tiles, so two workgroups
+// share a CU when each wave keeps to 128 VGPRs (4 waves per SIMD).
 template <bool TWO>
-__global__ __launch_bounds__(kThreads) void pass_kernel(PassArgs a) {
+__global__ __launch_bounds__(kThreads, TWO ? 2 : 4) void pass_kernel(PassArgs a) {
   extern __shared__ float2 lds[];
   __shared__ double red[kWaves];
   __shared__ double pacc[kMaxPassDerivs];

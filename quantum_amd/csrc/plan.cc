@@ -165,7 +165,7 @@ FusedOp BlockToOp(const Circuit& c, const std::vector<int>& gates, uint64_t qmas
 }  // namespace
 
 namespace {
-std::atomic<int> g_reg_engine(1), g_tile_single(0), g_tile_two(0);
+std::atomic<int> g_reg_engine(2), g_tile_single(0), g_tile_two(0);
 }  // namespace
 
 Tuning GetTuning() {

@@ -83,7 +83,7 @@ void InsertRemaps(std::vector<DevOp>* ops, size_t begin, int L);
 // Planner tuning (process-wide; 0 = default): register engine on/off and the
 // streaming path's tile qubits for one- and two-state passes.
 struct Tuning {
-  int reg_engine = 1;
+  int reg_engine = 2;
   int tile_single = 0;
   int tile_two = 0;
 };

@@ -51,6 +51,57 @@ __device__ __forceinline__ cf ldm(const float2* __restrict__ M, int i) {
 
 __device__ __forceinline__ uint32_t swz(uint32_t x) { return x ^ ((x >> 4) & 15u); }
 
+// ------------------------------------------------------------------ MFMA 4x4 products
+// v_mfma_f32_4x4x1f32 runs 16 independent 4x4x1 blocks per wavefront, block b
+// on lanes 4b..4b+3: D_b[i][j] += A_b[i] * B_b[j] with A_b[i] taken from lane
+// 4b + i, B_b[j] from lane 4b + j, and D_b[i][j] returned in register i of
+// lane 4b + j.  With A in lane l holding G[l & 3][k] and B holding each
+// lane's own amplitude of gate index k, four steps (k = 0..3) leave in
+// register r of every lane sum_k G[r][k] * v_k of that lane's amplitudes: a
+// 4x4 gate applied in place to the registers, at twice the FP32 rate of VALU
+// FMAs.  Complex products split into real ones: Re = Gr vr - Gi vi,
+// Im = Gi vr + Gr vi.  MFMA ignores the EXEC mask, so these paths run only in
+// wave-uniform control flow (ops without per-lane control predicates).
+typedef float mf4 __attribute__((ext_vector_type(4)));
+
+// A operands of a 4x4 complex matrix (or its adjoint): row (lane & 3).
+struct MfmaRows {
+  float gr[4], gi[4], ngi[4];
+};
+template <bool DAG>
+__device__ __forceinline__ void mfma_rows(const float2* __restrict__ M, MfmaRows& r) {
+  const int row = int(threadIdx.x & 3u);
+  const float* f = reinterpret_cast<const float*>(M);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    // (G^dag)[row][k] = conj(G[k][row]).
+    const int e = DAG ? k * 4 + row : row * 4 + k;
+    r.gr[k] = f[2 * e];
+    r.gi[k] = DAG ? -f[2 * e + 1] : f[2 * e + 1];
+    r.ngi[k] = -r.gi[k];
+  }
+}
+__device__ __forceinline__ mf4 mfma4(float a, float b, mf4 c) {
+  return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 0, 0, 0);
+}
+// (dr, di) = G v for the four amplitudes v_k = a[idx[k]] of every lane.
+template <int N>
+__device__ __forceinline__ void mfma_apply(const cf (&a)[N], const int (&idx)[4],
+                                           const MfmaRows& m, mf4& dr, mf4& di) {
+  dr = mf4{0.f, 0.f, 0.f, 0.f};
+  di = mf4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    dr = mfma4(m.gr[k], a[idx[k]].x, dr);
+    di = mfma4(m.gi[k], a[idx[k]].x, di);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    dr = mfma4(m.ngi[k], a[idx[k]].y, dr);
+    di = mfma4(m.gr[k], a[idx[k]].y, di);
+  }
+}
+
 __host__ __device__ constexpr int ins0(int g, int j) {
   return ((g >> j) << (j + 1)) | (g & ((1 << j) - 1));
 }
@@ -148,10 +199,36 @@ __device__ __forceinline__ void reg_dense1_c(cf (&a)[1 << R], cf (&b)[1 << R],
   }
 }
 
+template <int R, int J0, int J1>
+__device__ __forceinline__ void group_idx(int g, int (&idx)[4]) {
+  const int i0 = ins0(ins0(g, J0), J1);
+#pragma unroll
+  for (int s = 0; s < 4; ++s) idx[s] = i0 | ((s & 1) << J0) | ((s >> 1) << J1);
+}
+
 template <int R, int J0, int J1, bool DAG, bool BOTH, bool CTRL>
 __device__ __forceinline__ void reg_dense2_c(cf (&a)[1 << R], cf (&b)[1 << R],
                                              const float2* __restrict__ M, const RegMap<R>& m,
                                              uint32_t cmask, uint32_t cval) {
+  if constexpr (!CTRL) {
+    MfmaRows rows;
+    mfma_rows<DAG>(M, rows);
+#pragma unroll
+    for (int g = 0; g < (1 << (R - 2)); ++g) {
+      int idx[4];
+      group_idx<R, J0, J1>(g, idx);
+      mf4 dr, di;
+      mfma_apply(a, idx, rows, dr, di);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) a[idx[r]] = cf{dr[r], di[r]};
+      if (BOTH) {
+        mfma_apply(b, idx, rows, dr, di);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) b[idx[r]] = cf{dr[r], di[r]};
+      }
+    }
+    return;
+  }
   cf u[16];
 #pragma unroll
   for (int r = 0; r < 4; ++r)
@@ -287,6 +364,20 @@ template <int R, int K, int J0, int J1, bool CTRL>
 __device__ __forceinline__ void reg_dag_apply(cf (&a)[1 << R], const float2* __restrict__ G,
                                               const DevOp& op, const RegMap<R>& m) {
   constexpr int D = 1 << K;
+  if constexpr (K == 2 && !CTRL) {
+    MfmaRows rows;
+    mfma_rows<true>(G, rows);
+#pragma unroll
+    for (int g = 0; g < (1 << (R - 2)); ++g) {
+      int idx[4];
+      group_idx<R, J0, J1>(g, idx);
+      mf4 dr, di;
+      mfma_apply(a, idx, rows, dr, di);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) a[idx[r]] = cf{dr[r], di[r]};
+    }
+    return;
+  }
   cf gd[D * D];
 #pragma unroll
   for (int r = 0; r < D; ++r)
@@ -317,6 +408,27 @@ __device__ __forceinline__ void reg_param_kc(cf (&a)[1 << R], cf (&b)[1 << R], c
   reg_dag_apply<R, K, J0, J1, CTRL>(a, mats + op.mat, op, m);
   for (int d = 0; d < op.nderiv; ++d) {
     const float2* dG = mats + (d == 0 ? op.dmat[0] : op.dmat[1]);  // static field access
+    if constexpr (K == 2 && !CTRL) {
+      MfmaRows rows;
+      mfma_rows<false>(dG, rows);
+      float part = 0.f;
+#pragma unroll
+      for (int g = 0; g < (1 << (R - 2)); ++g) {
+        int idx[4];
+        group_idx<R, J0, J1>(g, idx);
+        mf4 dr, di;
+        mfma_apply(a, idx, rows, dr, di);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const cf l = b[idx[r]];
+          part = __builtin_fmaf(dr[r], l.x, part);
+          part = __builtin_fmaf(di[r], l.y, part);
+        }
+      }
+      if (d == 0) acc[0] += double(part);
+      else acc[1] += double(part);
+      continue;
+    }
     cf dg[D * D];
 #pragma unroll
     for (int i = 0; i < D * D; ++i) dg[i] = ldm(dG, i);

@@ -358,6 +358,48 @@ def test_streaming_path_19_qubits():
     np.testing.assert_allclose(_np(exp), ref, atol=TOL * max(1, np.abs(ref).max()))
 
 
+@pytest.fixture(scope="module")
+def streaming_case():
+    n = 19
+    circuits, names, vals, psums = _random_batch(901, n, 1, 2 * n, p_control=0.1, n_sym=4)
+    progs, ps = _ser(circuits), _ser(psums)
+    g = np.array([[1.0, 0.5]], np.float32)
+    ref_e = O.simulate_expectation(list(progs), names, vals, ps)
+    ref_g = O.adjoint_gradient(list(progs), names, vals, ps, g)
+    return progs, names, vals, ps, g, ref_e, ref_g
+
+
+@pytest.fixture(scope="module")
+def tiled_case():
+    circuits, names, vals, psums = _mixed_batch(802, [5, 12, 13, 14, 16])
+    progs, ps = _ser(circuits), _ser(psums)
+    g = np.random.default_rng(5).uniform(-1, 1, size=(len(circuits), 2)).astype(np.float32)
+    ref_e = O.simulate_expectation(list(progs), names, vals, ps)
+    ref_g = O.adjoint_gradient(list(progs), names, vals, ps, g)
+    return progs, names, vals, ps, g, ref_e, ref_g
+
+
+# Register engine off (0), batched kernel only (1), batched and streaming (2),
+# with streaming tiles of 12-13 qubits for one and two states, so that the
+# engine runs on one-state and two-state streaming passes and the LDS sweeps
+# run where it is off.
+@pytest.mark.parametrize("level,tile_single,tile_two", [(0, 13, 11), (1, 12, 12), (2, 13, 13),
+                                                        (2, 12, 12)])
+def test_register_engine_levels_match_oracle(streaming_case, tiled_case, level, tile_single,
+                                             tile_two):
+    _, adj, _ = ops()
+    _lib.set_tuning(level, tile_single, tile_two)
+    try:
+        for label, case in (("streaming n=19", streaming_case), ("tiled mixed", tiled_case)):
+            progs, names, vals, ps, g, ref_e, ref_g = case
+            _say(f"engine level {level}, tiles {tile_single}/{tile_two}: {label}")
+            exp, grad = adj.expectation_and_gradient(progs, names, vals, ps, g)
+            np.testing.assert_allclose(_np(exp), ref_e, atol=TOL * max(1, np.abs(ref_e).max()))
+            np.testing.assert_allclose(_np(grad), ref_g, atol=TOL * max(1, np.abs(ref_g).max()))
+    finally:
+        _lib.set_tuning()
+
+
 # ----------------------------------------------------------------------------- shared programs
 @pytest.mark.parametrize("derivative", ["tfq", "analytic"])
 def test_rebound_plan_matches_oracle_and_fresh_plan(derivative):

@@ -174,6 +174,19 @@ __device__ __forceinline__ void tiled_forward(float2* G, int n, const BatchArgs&
       auto gidx = [&](uint32_t j) -> uint64_t {
         return use_tab ? tile_index_tab(base, j, ps.nlow, tab) : tile_index(base, j, ps);
       };
+      // Register engine: the tile goes straight between G and the registers.
+      auto io = tile_io(
+          [&](uint32_t x, cf& va, cf&) {
+            const uint64_t g = gidx(x);
+            if (first) {
+              va = cf{g == 0 ? 1.f : 0.f, 0.f};
+            } else {
+              const float2 v = G[g];
+              va = cf{v.x, v.y};
+            }
+          },
+          [&](uint32_t x, const cf& va, const cf&) { G[gidx(x)] = make_float2(va.x, va.y); });
+      if (reg_run_any<false>(A, A, ps.L, a.ops, ps.op_begin, ps.op_end, a.mats, NoAcc(), io)) continue;
       if (first) {
         for (uint32_t j = tid; j < T; j += kThreads) A[j] = make_float2(gidx(j) == 0 ? 1.f : 0.f, 0.f);
       } else {
@@ -184,6 +197,9 @@ __device__ __forceinline__ void tiled_forward(float2* G, int n, const BatchArgs&
       tile_scatter(G, A, T, gidx);
       __syncthreads();
     }
+    // The pass's stores to G precede the next pass's loads, and the tile table
+    // is rebuilt only after every wave has finished with it.
+    __syncthreads();
     first = false;
   }
   if (first) {
@@ -210,6 +226,20 @@ __device__ __forceinline__ void tiled_reverse(float2* G, float2* Lg, int n, cons
       auto gidx = [&](uint32_t j) -> uint64_t {
         return use_tab ? tile_index_tab(base, j, ps.nlow, tab) : tile_index(base, j, ps);
       };
+      auto io = tile_io(
+          [&](uint32_t x, cf& va, cf& vb) {
+            const uint64_t g = gidx(x);
+            const float2 u = G[g], v = Lg[g];
+            va = cf{u.x, u.y};
+            vb = cf{v.x, v.y};
+          },
+          [&](uint32_t x, const cf& va, const cf& vb) {
+            const uint64_t g = gidx(x);
+            G[g] = make_float2(va.x, va.y);
+            Lg[g] = make_float2(vb.x, vb.y);
+          });
+      auto acc = [gacc](int, int col, double w) { atomicAdd(&gacc[col], 2.0 * w); };
+      if (reg_run_any<true>(A, B, ps.L, a.ops, ps.op_begin, ps.op_end, a.mats, acc, io)) continue;
       tile_gather2(A, B, G, Lg, T, gidx);
       __syncthreads();
       run_reverse<KMAX>(A, B, ps.L, a.ops, ps.op_begin, ps.op_end, a.mats, mbuf, gacc, red);
@@ -217,6 +247,7 @@ __device__ __forceinline__ void tiled_reverse(float2* G, float2* Lg, int n, cons
       tile_scatter(Lg, B, T, gidx);
       __syncthreads();
     }
+    __syncthreads();
   }
 }
 

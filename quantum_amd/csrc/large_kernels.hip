@@ -70,6 +70,33 @@ __global__ __launch_bounds__(kThreads, TWO ? 2 : 4) void pass_kernel(PassArgs a)
     auto gidx = [&](uint32_t j) -> uint64_t {
       return use_tab ? tile_index_tab(base, j, nlow, tab) : tile_index(base, j, ps);
     };
+    int pl = 0;
+    auto acc = [](int slot, int, double w) { atomicAdd(&pacc[slot], w); };
+    // Register engine: the tile goes straight between memory and registers.
+    auto io = tile_io(
+        [&](uint32_t x, cf& va, cf& vb) {
+          const uint64_t g = gidx(x);
+          if (TWO) {
+            const float2 u = a.psi[g], v = a.lam[g];
+            va = cf{u.x, u.y};
+            vb = cf{v.x, v.y};
+          } else if (a.init) {
+            va = cf{g == 0 ? 1.f : 0.f, 0.f};
+          } else {
+            const float2 u = a.src[g];
+            va = cf{u.x, u.y};
+          }
+        },
+        [&](uint32_t x, const cf& va, const cf& vb) {
+          const uint64_t g = gidx(x);
+          if (TWO) {
+            a.psi[g] = make_float2(va.x, va.y);
+            a.lam[g] = make_float2(vb.x, vb.y);
+          } else {
+            a.dst[g] = make_float2(va.x, va.y);
+          }
+        });
+    if (reg_run_any<TWO>(A, Bf, L, a.ops, ps.op_begin, ps.op_end, a.mats, acc, io)) continue;
     if (TWO) {
       tile_gather2(A, Bf, a.psi, a.lam, T, gidx);
     } else if (a.init) {
@@ -78,9 +105,6 @@ __global__ __launch_bounds__(kThreads, TWO ? 2 : 4) void pass_kernel(PassArgs a)
       tile_gather(A, a.src, T, gidx);
     }
     __syncthreads();
-    int pl = 0;
-    auto acc = [](int slot, int, double w) { atomicAdd(&pacc[slot], w); };
-    if (!reg_run_any<TWO>(A, Bf, L, a.ops, ps.op_begin, ps.op_end, a.mats, acc))
     for_ops(a.ops, ps.op_begin, ps.op_end, a.mats, mbuf,
             [&](const DevOp& op, const float2* M) {
               if (!TWO) {
@@ -109,6 +133,9 @@ __global__ __launch_bounds__(kThreads, TWO ? 2 : 4) void pass_kernel(PassArgs a)
     }
     __syncthreads();
   }
+  // Register-engine tiles end without a barrier: wait for every wave's
+  // gradient partials before reading them.
+  __syncthreads();
   if (TWO && tid == 0) {
     int pl = 0;
     for (int i = ps.op_begin; i < ps.op_end; ++i) {

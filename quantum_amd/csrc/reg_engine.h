@@ -492,18 +492,45 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // ------------------------------------------------------------------ driver
-// Apply ops [b, e) to the LDS tile A (and B, reverse sweep) of 2^L amplitudes.
+// Tile I/O of the engine.  LdsIO: the tile enters and leaves in LDS buffers A
+// (and B) in the natural layout; the caller synchronises before the run and
+// the engine synchronises before returning.  TileIO: the first remap loads the
+// registers straight from memory with load(x, a, b) and the run ends by
+// storing them with store(x, a, b), x the tile index of each register (b only
+// in the reverse sweep); LDS is then used only by the remaps in between, and
+// no barrier surrounds the run (a remap's own barriers order its LDS use).
+struct LdsIO {};
+template <typename LoadFn, typename StoreFn>
+struct TileIO {
+  LoadFn load;
+  StoreFn store;
+};
+template <typename LoadFn, typename StoreFn>
+__device__ __forceinline__ TileIO<LoadFn, StoreFn> tile_io(LoadFn l, StoreFn s) {
+  return TileIO<LoadFn, StoreFn>{l, s};
+}
+template <typename IO>
+struct IsLdsIO {
+  static constexpr bool value = false;
+};
+template <>
+struct IsLdsIO<LdsIO> {
+  static constexpr bool value = true;
+};
+
+// Apply ops [b, e) to a tile of 2^L amplitudes (A, and B in the reverse sweep).
 // REV = false: apply each op's matrix to A.  REV = true: un-apply (M^dagger)
 // non-parameterised ops on A and B; parameterised ops run the adjoint step
 // and call acc_fn(slot, column, wave_partial) from lane 0 of each wavefront
 // per derivative, slot counting the derivative matrices of earlier
 // parameterised ops and column being the derivative's symbol column.
-// Ops [b, e) must start with a kOpRemap.  The caller synchronises before
-// (tile written) and the engine synchronises before returning.
-template <int R, bool REV, typename AccFn>
+// Ops [b, e) must start with a kOpRemap.  A and B also serve as the remap
+// buffers.
+template <int R, bool REV, typename AccFn, typename IO>
 __device__ __forceinline__ void reg_run(float2* __restrict__ A, float2* __restrict__ B, int L,
                         const DevOp* __restrict__ ops, int b, int e,
-                        const float2* __restrict__ mats, AccFn&& acc_fn) {
+                        const float2* __restrict__ mats, AccFn&& acc_fn, IO& io) {
+  constexpr bool kLds = IsLdsIO<IO>::value;
   cf ra[1 << R];
   cf rb[1 << R];  // unused when !REV
   RegMap<R> m;
@@ -512,6 +539,15 @@ __device__ __forceinline__ void reg_run(float2* __restrict__ A, float2* __restri
   for (int i = b; i < e; ++i) {
     const DevOp op = load_op(ops + i);
     if (op.type == kOpRemap) {
+      if constexpr (!kLds) {
+        if (!have) {
+          make_map<R>(m, op.cmask, L);
+#pragma unroll
+          for (int r = 0; r < (1 << R); ++r) io.load(m.tbase | rdep(m, r), ra[r], rb[r]);
+          have = true;
+          continue;
+        }
+      }
       if (have) {
         __syncthreads();
         reg_store<R>(ra, A, m, true);
@@ -542,26 +578,37 @@ __device__ __forceinline__ void reg_run(float2* __restrict__ A, float2* __restri
       reg_dense<R, REV, REV>(ra, rb, op, mats + op.mat, m);
     }
   }
-  if (have) {
+  if (!have) return;
+  if constexpr (kLds) {
     __syncthreads();
     reg_store<R>(ra, A, m, false);
     if (REV) reg_store<R>(rb, B, m, false);
     __syncthreads();
+  } else {
+#pragma unroll
+    for (int r = 0; r < (1 << R); ++r) io.store(m.tbase | rdep(m, r), ra[r], rb[r]);
   }
 }
 
 // Run ops [b, e) on a tile of L qubits with the register engine when the op
-// list was planned for it (starts with a remap); returns false otherwise.
-template <bool REV, typename AccFn>
+// list was planned for it (starts with a remap); returns false otherwise,
+// before touching the tile.
+template <bool REV, typename AccFn, typename IO>
 __device__ __forceinline__ bool reg_run_any(float2* A, float2* B, int L, const DevOp* ops, int b,
-                                            int e, const float2* mats, AccFn&& acc_fn) {
+                                            int e, const float2* mats, AccFn&& acc_fn, IO& io) {
   if (b >= e) return false;
   if (uniform(int(ops[b].type)) != kOpRemap) return false;
   switch (RegBitsForTile(L)) {
-    case 3: reg_run<3, REV>(A, B, L, ops, b, e, mats, acc_fn); return true;
-    case 4: reg_run<4, REV>(A, B, L, ops, b, e, mats, acc_fn); return true;
+    case 3: reg_run<3, REV>(A, B, L, ops, b, e, mats, acc_fn, io); return true;
+    case 4: reg_run<4, REV>(A, B, L, ops, b, e, mats, acc_fn, io); return true;
     default: return false;
   }
+}
+template <bool REV, typename AccFn>
+__device__ __forceinline__ bool reg_run_any(float2* A, float2* B, int L, const DevOp* ops, int b,
+                                            int e, const float2* mats, AccFn&& acc_fn) {
+  LdsIO io;
+  return reg_run_any<REV>(A, B, L, ops, b, e, mats, acc_fn, io);
 }
 
 }  // namespace

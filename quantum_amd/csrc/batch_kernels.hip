@@ -34,8 +34,8 @@ struct NoAcc {
 // Forward ops [b, e) on an LDS tile of n qubits (register engine when planned).
 template <int KMAX>
 __device__ __forceinline__ void run_ops(float2* s, int n, const DevOp* ops, int b, int e, const float2* mats,
-                        float2* mbuf) {
-  if (reg_run_any<false>(s, s, n, ops, b, e, mats, NoAcc())) return;
+                        float2* mbuf, MatCache& mc) {
+  if (reg_run_any<false>(s, s, n, ops, b, e, mc, NoAcc())) return;
   for_ops(ops, b, e, mats, mbuf,
           [&](const DevOp& op, const float2* M) { apply_op<false, KMAX>(s, n, op, M); });
 }
@@ -43,7 +43,8 @@ __device__ __forceinline__ void run_ops(float2* s, int n, const DevOp* ops, int 
 // Adjoint sweep of ops [b, e) on psi (A) and lambda (B), gradients into gacc.
 template <int KMAX>
 __device__ __forceinline__ void run_reverse(float2* A, float2* B, int n, const DevOp* ops, int b, int e,
-                            const float2* mats, float2* mbuf, double* gacc, double* red);
+                            const float2* mats, float2* mbuf, double* gacc, double* red,
+                            MatCache& mc);
 
 __device__ __forceinline__ void set_zero_state(float2* s, uint32_t N) {
   for (uint32_t x = threadIdx.x; x < N; x += kThreads) s[x] = make_float2(x == 0 ? 1.f : 0.f, 0.f);
@@ -142,9 +143,10 @@ __device__ __forceinline__ void param_step(float2* A, float2* B, int n, const De
 
 template <int KMAX>
 __device__ __forceinline__ void run_reverse(float2* A, float2* B, int n, const DevOp* ops, int b, int e,
-                            const float2* mats, float2* mbuf, double* gacc, double* red) {
+                            const float2* mats, float2* mbuf, double* gacc, double* red,
+                            MatCache& mc) {
   auto acc = [gacc](int, int col, double w) { atomicAdd(&gacc[col], 2.0 * w); };
-  if (reg_run_any<true>(A, B, n, ops, b, e, mats, acc)) return;
+  if (reg_run_any<true>(A, B, n, ops, b, e, mc, acc)) return;
   for_ops(ops, b, e, mats, mbuf, [&](const DevOp& op, const float2* M) {
     if (op.type == kOpParam) {
       param_step(A, B, n, op, M, gacc, red);
@@ -159,7 +161,7 @@ __device__ __forceinline__ void run_reverse(float2* A, float2* B, int n, const D
 // tiles through LDS buffer A.  The first pass starts from |0>.
 template <int KMAX>
 __device__ __forceinline__ void tiled_forward(float2* G, int n, const BatchArgs& a, int pb, int pe, float2* A,
-                              float2* mbuf, uint64_t* tab) {
+                              float2* mbuf, uint64_t* tab, MatCache& mc) {
   const int tid = threadIdx.x;
   bool first = true;
   for (int p = pb; p < pe; ++p) {
@@ -186,14 +188,14 @@ __device__ __forceinline__ void tiled_forward(float2* G, int n, const BatchArgs&
             }
           },
           [&](uint32_t x, const cf& va, const cf&) { G[gidx(x)] = make_float2(va.x, va.y); });
-      if (reg_run_any<false>(A, A, ps.L, a.ops, ps.op_begin, ps.op_end, a.mats, NoAcc(), io)) continue;
+      if (reg_run_any<false>(A, A, ps.L, a.ops, ps.op_begin, ps.op_end, mc, NoAcc(), io)) continue;
       if (first) {
         for (uint32_t j = tid; j < T; j += kThreads) A[j] = make_float2(gidx(j) == 0 ? 1.f : 0.f, 0.f);
       } else {
         tile_gather(A, G, T, gidx);
       }
       __syncthreads();
-      run_ops<KMAX>(A, ps.L, a.ops, ps.op_begin, ps.op_end, a.mats, mbuf);
+      run_ops<KMAX>(A, ps.L, a.ops, ps.op_begin, ps.op_end, a.mats, mbuf, mc);
       tile_scatter(G, A, T, gidx);
       __syncthreads();
     }
@@ -212,7 +214,7 @@ __device__ __forceinline__ void tiled_forward(float2* G, int n, const BatchArgs&
 template <int KMAX>
 __device__ __forceinline__ void tiled_reverse(float2* G, float2* Lg, int n, const BatchArgs& a, int pb, int pe,
                               float2* A, float2* B, double* gacc, double* red, float2* mbuf,
-                              uint64_t* tab) {
+                              uint64_t* tab, MatCache& mc) {
   const int tid = threadIdx.x;
   for (int p = pb; p < pe; ++p) {
     const DevPass ps = a.passes[p];
@@ -239,10 +241,10 @@ __device__ __forceinline__ void tiled_reverse(float2* G, float2* Lg, int n, cons
             Lg[g] = make_float2(vb.x, vb.y);
           });
       auto acc = [gacc](int, int col, double w) { atomicAdd(&gacc[col], 2.0 * w); };
-      if (reg_run_any<true>(A, B, ps.L, a.ops, ps.op_begin, ps.op_end, a.mats, acc, io)) continue;
+      if (reg_run_any<true>(A, B, ps.L, a.ops, ps.op_begin, ps.op_end, mc, acc, io)) continue;
       tile_gather2(A, B, G, Lg, T, gidx);
       __syncthreads();
-      run_reverse<KMAX>(A, B, ps.L, a.ops, ps.op_begin, ps.op_end, a.mats, mbuf, gacc, red);
+      run_reverse<KMAX>(A, B, ps.L, a.ops, ps.op_begin, ps.op_end, a.mats, mbuf, gacc, red, mc);
       tile_scatter(G, A, T, gidx);
       tile_scatter(Lg, B, T, gidx);
       __syncthreads();
@@ -258,6 +260,8 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(BatchArgs a) {
   __shared__ int s_ci;
   __shared__ float2 mbuf[2 * kMatSlot];
   __shared__ uint64_t tab[kTileTable];
+  __shared__ float2 mcache[kMatCache];
+  MatCache mc = mat_cache(mcache, a.mats);
   __shared__ float2 rotS[8];
   if (MODE == kModeSampledExpectation && threadIdx.x == 0) {
 #pragma unroll
@@ -302,10 +306,10 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(BatchArgs a) {
 
     if (tiled) {
       tiled_forward<KMAX>(gpsi, n, a, uniform(c.fwd_pass_begin), uniform(c.fwd_pass_end), psi, mbuf,
-                    tab);
+                    tab, mc);
     } else {
       set_zero_state(psi, N);
-      run_ops<KMAX>(psi, n, a.ops, uniform(c.fwd_begin), uniform(c.fwd_end), a.mats, mbuf);
+      run_ops<KMAX>(psi, n, a.ops, uniform(c.fwd_begin), uniform(c.fwd_end), a.mats, mbuf, mc);
     }
 
     if (MODE == kModeExpectation || (MODE == kModeAdjoint && a.exp_out)) {
@@ -320,12 +324,12 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(BatchArgs a) {
       if (tiled) {
         build_lambda(gpsi, glam, N, a, lb, le);
         tiled_reverse<KMAX>(gpsi, glam, n, a, uniform(c.rev_pass_begin), uniform(c.rev_pass_end), psi,
-                      buf2, gacc, red, mbuf, tab);
+                      buf2, gacc, red, mbuf, tab, mc);
       } else {
         float2* lam = buf2;
         build_lambda(psi, lam, N, a, lb, le);
         run_reverse<KMAX>(psi, lam, n, a.ops, uniform(c.rev_begin), uniform(c.rev_end), a.mats,
-                          mbuf, gacc, red);
+                          mbuf, gacc, red, mc);
       }
       // Gradients arrive by atomics (register engine) or plain stores; read them
       // back from L2.
@@ -400,7 +404,7 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(BatchArgs a) {
       for (int o = uniform(c.other_begin); o < c.other_end; ++o) {
         const DevOther ot = a.others[o];
         set_zero_state(phi, N);
-        run_ops<KMAX>(phi, n, a.ops, uniform(ot.op_begin), uniform(ot.op_end), a.mats, mbuf);
+        run_ops<KMAX>(phi, n, a.ops, uniform(ot.op_begin), uniform(ot.op_end), a.mats, mbuf, mc);
         double re = 0.0, im = 0.0;
         for (uint32_t x = tid; x < N; x += kThreads) {
           const float2 p = cmul_conj_a(psi[x], phi[x]);

@@ -56,28 +56,72 @@ void SchedulePasses(const std::vector<FusedOp>& ops, int n, int L, std::vector<D
   const uint64_t lowmask = (uint64_t(1) << low_run) - 1;
   std::vector<int> remaining(ops.size());
   for (size_t i = 0; i < ops.size(); ++i) remaining[i] = int(i);
-  while (!remaining.empty()) {
-    uint64_t needed = 0, blocked = 0;
-    int derivs = 0;
-    std::vector<int> chosen, deferred;
+  // Ops a tile of qubits T can run now, in order: an op runs when its qubits
+  // lie in T and it commutes with every op skipped before it (disjoint
+  // qubits, or both diagonal), within the pass's derivative budget.
+  auto take_ops = [&](uint64_t T, std::vector<int>* take, uint64_t* cand, int ncand) -> int {
+    uint64_t blocked = 0, blocked_diag = 0;
+    int derivs = 0, count = 0, nc = 0;
     for (int i : remaining) {
       const FusedOp& f = ops[i];
       const uint64_t Q = OpMask(f);
       const int nd = f.param ? f.nderiv : 0;
-      if (Q & blocked) {
-        deferred.push_back(i);
-        blocked |= Q;
+      const uint64_t hit = f.diag && !f.param ? blocked : (blocked | blocked_diag);
+      bool ok = (Q & hit) == 0;
+      if (ok && (Q & ~T)) {
+        // Runnable but for missing qubits: a candidate extension of T.
+        if (cand && nc < ncand && Popc(T | Q) <= L) {
+          const uint64_t M = Q & ~T;
+          bool dup = false;
+          for (int c = 0; c < nc; ++c) dup = dup || cand[c] == M;
+          if (!dup) cand[nc++] = M;
+        }
+        ok = false;
+      }
+      if (ok && derivs + nd > kMaxPassDerivs) ok = false;
+      if (!ok) {
+        if (f.diag && !f.param) blocked_diag |= Q;
+        else blocked |= Q;
         continue;
       }
-      const uint64_t nn = needed | Q;
-      if (Popc(nn | lowmask) <= L && derivs + nd <= kMaxPassDerivs) {
-        chosen.push_back(i);
-        needed = nn;
-        derivs += nd;
-      } else {
-        deferred.push_back(i);
-        blocked |= Q;
+      derivs += nd;
+      ++count;
+      if (take) take->push_back(i);
+    }
+    if (cand) {
+      for (int c = nc; c < ncand; ++c) cand[c] = 0;
+    }
+    return count;
+  };
+  while (!remaining.empty()) {
+    // Grow the tile from the low run: each step adds the missing qubits of one
+    // of the first few runnable-but-missing ops, choosing the extension that
+    // lets the most ops run (ties: fewer qubits, then earlier op).
+    constexpr int kCand = 8;
+    uint64_t needed = Popc(lowmask) <= L ? lowmask : 0;
+    while (Popc(needed) < L) {
+      uint64_t cand[kCand];
+      take_ops(needed, nullptr, cand, kCand);
+      int best = -1;
+      uint64_t best_T = 0;
+      for (int c = 0; c < kCand && cand[c]; ++c) {
+        const uint64_t T2 = needed | cand[c];
+        const int v = take_ops(T2, nullptr, nullptr, 0);
+        if (v > best || (v == best && Popc(T2) < Popc(best_T))) {
+          best = v;
+          best_T = T2;
+        }
       }
+      if (best < 0) break;
+      needed = best_T;
+    }
+    std::vector<int> chosen, deferred;
+    take_ops(needed, &chosen, nullptr, 0);
+    {
+      std::vector<char> in(ops.size(), 0);
+      for (int i : chosen) in[size_t(i)] = 1;
+      for (int i : remaining)
+        if (!in[size_t(i)]) deferred.push_back(i);
     }
     if (chosen.empty()) {
       // An op wider than the low run allows: give it a tile of its own qubits

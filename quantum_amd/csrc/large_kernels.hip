@@ -52,6 +52,8 @@ __global__ __launch_bounds__(kThreads, TWO ? 2 : 4) void pass_kernel(PassArgs a)
   __shared__ double pacc[kMaxPassDerivs];
   __shared__ float2 mbuf[2 * kMatSlot];
   __shared__ uint64_t tab[kTileTable];
+  __shared__ float2 mcache[kMatCache];
+  MatCache mc = mat_cache(mcache, a.mats);
   const DevPass ps = a.pass;
   const int L = ps.L;
   const uint32_t T = 1u << L;
@@ -96,7 +98,7 @@ __global__ __launch_bounds__(kThreads, TWO ? 2 : 4) void pass_kernel(PassArgs a)
             a.dst[g] = make_float2(va.x, va.y);
           }
         });
-    if (reg_run_any<TWO>(A, Bf, L, a.ops, ps.op_begin, ps.op_end, a.mats, acc, io)) continue;
+    if (reg_run_any<TWO>(A, Bf, L, a.ops, ps.op_begin, ps.op_end, mc, acc, io)) continue;
     if (TWO) {
       tile_gather2(A, Bf, a.psi, a.lam, T, gidx);
     } else if (a.init) {
@@ -414,7 +416,7 @@ static void LargeLayout(LargePlan* p, size_t* off_pcols, size_t* off_ident, size
   p->grid = grid;
   size_t off = 0;
   p->off_ops = off; off = AlignUp(off + p->ops.size() * sizeof(DevOp));
-  p->off_mats = off; off = AlignUp(off + p->mats.size() * sizeof(cfloat));
+  p->off_mats = off; off = AlignUp(off + (p->mats.size() + kMatCache) * sizeof(cfloat));
   p->off_terms = off; off = AlignUp(off + p->terms.size() * sizeof(DevTermWide));
   *off_pcols = off; off = AlignUp(off + p->param_cols.size() * sizeof(int32_t));
   const size_t nslots = size_t(p->mode == kModeInnerProduct ? p->K : p->O);

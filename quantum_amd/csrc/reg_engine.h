@@ -13,8 +13,8 @@
 // and stores of a wavefront are spread over the LDS banks; on entry and exit
 // the layout is the natural one.
 //
-// Op matrices are read straight from global memory with wave-uniform
-// addresses, as scalar loads.  Parameterised ops (reverse sweep) accumulate their FP64
+// Op matrices are read from an LDS window over the plan's matrix array,
+// refilled when an op's matrices fall outside it.  Parameterised ops (reverse sweep) accumulate their FP64
 // gradient partials per thread, reduce them across the wavefront and hand the
 // sum to the caller's accumulator.
 #pragma once
@@ -492,6 +492,35 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // ------------------------------------------------------------------ driver
+// LDS window [lo, hi) of the plan's matrix array (global g).  All waves of a
+// workgroup run the same op sequence, so a refill (two barriers) happens at
+// the same op in every wave; windows stay valid for the whole launch because
+// the matrices do not change while it runs.
+struct MatCache {
+  float2* lds;  // kMatCache entries
+  const float2* g;
+  uint32_t lo, hi;
+};
+__device__ __forceinline__ MatCache mat_cache(float2* lds, const float2* g) {
+  return MatCache{lds, g, 0u, 0u};
+}
+// Make entries [off, off + len) resident in the window.
+__device__ __forceinline__ void cache_span(MatCache& c, uint32_t off, uint32_t len) {
+  if (off >= c.lo && off + len <= c.hi) return;
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < uint32_t(kMatCache); i += kThreads) c.lds[i] = c.g[off + i];
+  __syncthreads();
+  c.lo = off;
+  c.hi = off + uint32_t(kMatCache);
+}
+// Matrix entries an op reads: its unitary (or diagonal) and derivatives,
+// stored back to back.
+__device__ __forceinline__ uint32_t op_mat_len(const DevOp& op) {
+  const uint32_t D = 1u << op.k;
+  const uint32_t DD = op.type == kOpDiag ? D : D * D;
+  return DD * (1u + (op.type == kOpParam ? uint32_t(op.nderiv) : 0u));
+}
+
 // Tile I/O of the engine.  LdsIO: the tile enters and leaves in LDS buffers A
 // (and B) in the natural layout; the caller synchronises before the run and
 // the engine synchronises before returning.  TileIO: the first remap loads the
@@ -529,7 +558,8 @@ struct IsLdsIO<LdsIO> {
 template <int R, bool REV, typename AccFn, typename IO>
 __device__ __forceinline__ void reg_run(float2* __restrict__ A, float2* __restrict__ B, int L,
                         const DevOp* __restrict__ ops, int b, int e,
-                        const float2* __restrict__ mats, AccFn&& acc_fn, IO& io) {
+                        MatCache& mc, AccFn&& acc_fn, IO& io) {
+  const float2* mats = mc.lds;
   constexpr bool kLds = IsLdsIO<IO>::value;
   cf ra[1 << R];
   cf rb[1 << R];  // unused when !REV
@@ -537,7 +567,14 @@ __device__ __forceinline__ void reg_run(float2* __restrict__ A, float2* __restri
   bool have = false, swizzled = false;
   int slot = 0;
   for (int i = b; i < e; ++i) {
-    const DevOp op = load_op(ops + i);
+    DevOp op = load_op(ops + i);
+    if (op.type != kOpRemap) {
+      // Matrices from the LDS window: offsets relative to its base.
+      cache_span(mc, op.mat, op_mat_len(op));
+      op.mat -= mc.lo;
+      op.dmat[0] -= mc.lo;
+      op.dmat[1] -= mc.lo;
+    }
     if (op.type == kOpRemap) {
       if constexpr (!kLds) {
         if (!have) {
@@ -595,20 +632,20 @@ __device__ __forceinline__ void reg_run(float2* __restrict__ A, float2* __restri
 // before touching the tile.
 template <bool REV, typename AccFn, typename IO>
 __device__ __forceinline__ bool reg_run_any(float2* A, float2* B, int L, const DevOp* ops, int b,
-                                            int e, const float2* mats, AccFn&& acc_fn, IO& io) {
+                                            int e, MatCache& mc, AccFn&& acc_fn, IO& io) {
   if (b >= e) return false;
   if (uniform(int(ops[b].type)) != kOpRemap) return false;
   switch (RegBitsForTile(L)) {
-    case 3: reg_run<3, REV>(A, B, L, ops, b, e, mats, acc_fn, io); return true;
-    case 4: reg_run<4, REV>(A, B, L, ops, b, e, mats, acc_fn, io); return true;
+    case 3: reg_run<3, REV>(A, B, L, ops, b, e, mc, acc_fn, io); return true;
+    case 4: reg_run<4, REV>(A, B, L, ops, b, e, mc, acc_fn, io); return true;
     default: return false;
   }
 }
 template <bool REV, typename AccFn>
 __device__ __forceinline__ bool reg_run_any(float2* A, float2* B, int L, const DevOp* ops, int b,
-                                            int e, const float2* mats, AccFn&& acc_fn) {
+                                            int e, MatCache& mc, AccFn&& acc_fn) {
   LdsIO io;
-  return reg_run_any<REV>(A, B, L, ops, b, e, mats, acc_fn, io);
+  return reg_run_any<REV>(A, B, L, ops, b, e, mc, acc_fn, io);
 }
 
 }  // namespace

@@ -29,6 +29,11 @@ enum OpType : uint8_t {
 // qubits; other tile sizes use the LDS path.  The planner inserts kOpRemap
 // ops exactly when this returns nonzero and every op of the list has k <= 2.
 constexpr int kRegThreadBits = 9;
+
+// The register engine reads op matrices from an LDS window of this many
+// complex entries over the plan's matrix array; the device copy of that array
+// is padded by the same count so a window never reads past it.
+constexpr int kMatCache = 768;
 constexpr int RegBitsForTile(int L) { return (L >= 12 && L <= 13) ? L - kRegThreadBits : 0; }
 
 // One op on a tile.  Qubit positions are tile-local bit indices (ascending).

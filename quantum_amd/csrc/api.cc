@@ -280,6 +280,7 @@ int BuildPlan(int mode, Inputs& in, int64_t B, int64_t S, int64_t O, int64_t K,
   const bool tiled_ok = (mode == kModeExpectation || mode == kModeAdjoint || mode == kModeState) &&
                         p->max_nq <= kTiledMaxQubits;
   opt.tile_log2 = resident;
+  opt.wave = mode == kModeExpectation || mode == kModeAdjoint;
   if (p->max_nq > resident && !tiled_ok) {
     p->large = true;
     Status s = BuildLargePlan(mode, in.circuits, in.sums, downstream, in.others, B, S, O, K,
@@ -307,6 +308,9 @@ int BuildPlan(int mode, Inputs& in, int64_t B, int64_t S, int64_t O, int64_t K,
     BuildBatchProgram(jobs, slots, int(S), opt, HostThreads(), &p->prog);
   }
   p->lds_log2 = std::max(1, std::min(p->max_nq, resident));
+  // Wave mode carves the full LDS allocation into one pair of 2^kWaveMaxQubits
+  // buffers per wavefront.
+  if (p->prog.num_block < int(p->prog.circuits.size())) p->lds_log2 = resident;
   p->kmax = 2;
   for (const auto& o : p->prog.ops) p->kmax = std::max<int>(p->kmax, o.k);
   Layout(p.get());
@@ -356,6 +360,7 @@ BatchArgs MakeArgs(tfqx_plan* p, void* ws, void* output) {
   a.kmax = p->kmax;
   a.circuits = reinterpret_cast<const DevCircuit*>(base + p->off_circuits);
   a.ncirc = int(p->prog.circuits.size());
+  a.nblock = p->prog.num_block;
   a.nslots = p->mode == kModeInnerProduct ? int(p->K) : int(p->O);
   a.nsym = int(p->S);
   a.lds_log2 = p->lds_log2;
@@ -372,6 +377,7 @@ BatchArgs MakeArgs(tfqx_plan* p, void* ws, void* output) {
   a.grad_acc = reinterpret_cast<double*>(base + p->off_gacc);
   a.queue = reinterpret_cast<int*>(base + p->off_counters);
   a.done = reinterpret_cast<int*>(base + p->off_counters + 64);
+  a.wqueue = reinterpret_cast<int*>(base + p->off_counters + 128);
   a.seed = p->seed;
   a.nshots = p->nshots;
   a.max_nq = p->max_nq;

@@ -12,6 +12,7 @@ struct BatchArgs {
   int kmax;            // widest fused op in the program (selects the kernel variant)
   const DevCircuit* circuits;
   int ncirc;
+  int nblock;          // circuits [0, nblock) run per workgroup, the rest per wavefront
   int nslots;          // O: expectation columns (K for inner products)
   int nsym;            // S: gradient columns
   int lds_log2;        // LDS holds 2^lds_log2 amplitudes per state buffer
@@ -38,6 +39,7 @@ struct BatchArgs {
   float2 rot[8];       // Y^-0.5 (4 entries) then X^0.5 (4 entries): Z-basis rotations
   int* queue;          // work-queue head (zero between launches)
   int* done;           // workgroups finished (zero between launches)
+  int* wqueue;         // wave-mode queue head (zero between launches)
 };
 
 // Largest circuit the LDS kernel holds (per mode): 2^14 amplitudes (128 KiB)

@@ -157,6 +157,94 @@ __device__ __forceinline__ void run_reverse(float2* A, float2* B, int n, const D
   });
 }
 
+// ------------------------------------------------------------------ wave mode
+// One wavefront per circuit of kWaveRegBits..kWaveMaxQubits qubits: the
+// state stays in the lanes' registers (register engine, wave-synchronised
+// remaps through the wavefront's own LDS buffers), expectations and lambda
+// are wavefront reductions, and no workgroup barrier is involved.
+__device__ __forceinline__ void wave_expectations(const float2* __restrict__ S, uint32_t N,
+                                                  const BatchArgs& a, const DevCircuit& c,
+                                                  int row) {
+  const uint32_t lane = threadIdx.x & 63u;
+  int t = uniform(c.term_begin);
+  const int te = uniform(c.term_end);
+  for (int j = 0; j < a.nslots; ++j) {
+    double acc = 0.0;
+    while (t < te && uniform(a.terms[t].slot) == j) {
+      const DevTerm T = a.terms[t];
+      float part = 0.f;
+      for (uint32_t x = lane; x < N; x += 64u) {
+        const float2 pr = cmul_conj_a(S[x ^ T.fmask], S[x]);
+        const float v = T.re * pr.x - T.im * pr.y;
+        part += (__popc(x & T.zmask) & 1) ? -v : v;
+      }
+      acc += double(part);
+      ++t;
+    }
+    const double tot = wave_sum(acc);
+    if (lane == 0)
+      a.exp_out[size_t(row) * a.nslots + j] = float(tot + double(a.ident[size_t(row) * a.nslots + j]));
+  }
+}
+
+__device__ __forceinline__ void wave_lambda(const float2* __restrict__ S, float2* __restrict__ Lm,
+                                            uint32_t N, const BatchArgs& a, int lb, int le) {
+  for (uint32_t y = threadIdx.x & 63u; y < N; y += 64u) {
+    float2 l = make_float2(0.f, 0.f);
+    for (int t = lb; t < le; ++t) {
+      const DevTerm T = a.terms[t];
+      const uint32_t x = y ^ T.fmask;
+      float2 w = cmul(make_float2(T.re, T.im), S[x]);
+      if (__popc(x & T.zmask) & 1) w = make_float2(-w.x, -w.y);
+      l = cadd(l, w);
+    }
+    Lm[y] = l;
+  }
+  esync<true>();
+}
+
+static_assert(kLdsMaxQubitsOneBuffer >= kWaveMaxQubits + 4 &&
+                  kLdsMaxQubitsTwoBuffers + 1 >= kWaveMaxQubits + 4,
+              "wave mode needs two 2^kWaveMaxQubits buffers per wavefront (8 wavefronts)");
+
+// Circuit ci (wave mode) on this wavefront with LDS buffers P and Q (2^n each).
+template <int MODE>
+__device__ __forceinline__ void wave_circuit(const BatchArgs& a, int ci, float2* P, float2* Q) {
+  const DevCircuit c = a.circuits[ci];
+  const int n = uniform(c.n);
+  const int row = uniform(c.row);
+  const uint32_t N = 1u << n;
+  const uint32_t lane = threadIdx.x & 63u;
+  // Forward from |0>, final state to P in the natural layout.
+  auto fio = tile_io([](uint32_t x, cf& va, cf&) { va = cf{x == 0 ? 1.f : 0.f, 0.f}; },
+                     [P](uint32_t x, const cf& va, const cf&) { P[x] = make_float2(va.x, va.y); });
+  reg_run_wave<false>(P, P, n, a.ops, uniform(c.fwd_begin), uniform(c.fwd_end), a.mats, NoAcc(),
+                      fio);
+  esync<true>();
+  if (MODE == kModeExpectation || (MODE == kModeAdjoint && a.exp_out))
+    wave_expectations(P, N, a, c, row);
+  if (MODE == kModeAdjoint) {
+    double* gacc = a.grad_acc + size_t(row) * a.nsym;
+    for (int j = int(lane); j < a.nsym; j += 64) gacc[j] = 0.0;
+    __threadfence();
+    wave_lambda(P, Q, N, a, uniform(c.lam_begin), uniform(c.lam_end));
+    auto rio = tile_io(
+        [P, Q](uint32_t x, cf& va, cf& vb) {
+          const float2 u = P[x], v = Q[x];
+          va = cf{u.x, u.y};
+          vb = cf{v.x, v.y};
+        },
+        [](uint32_t, const cf&, const cf&) {});
+    auto acc = [gacc](int, int col, double w) { atomicAdd(&gacc[col], 2.0 * w); };
+    reg_run_wave<true>(P, Q, n, a.ops, uniform(c.rev_begin), uniform(c.rev_end), a.mats, acc, rio);
+    __threadfence();
+    for (int j = int(lane); j < a.nsym; j += 64)
+      a.grad_out[size_t(row) * a.nsym + j] =
+          float(__hip_atomic_load(&gacc[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  }
+  esync<true>();
+}
+
 // Tiled forward: the state lives in global memory G; each pass streams its
 // tiles through LDS buffer A.  The first pass starts from |0>.
 template <int KMAX>
@@ -278,7 +366,7 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(BatchArgs a) {
     __syncthreads();
     const int ci = uniform(s_ci);
     __syncthreads();
-    if (ci >= a.ncirc) break;
+    if (ci >= a.nblock) break;
     const DevCircuit c = a.circuits[ci];
     const int n = uniform(c.n);
     const int row = uniform(c.row);
@@ -420,13 +508,31 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(BatchArgs a) {
     __syncthreads();
   }
 
-  // The last workgroup out resets the queue so the launch can be replayed
+  // Wave mode: each wavefront pulls small circuits of its own.
+  if constexpr (MODE == kModeExpectation || MODE == kModeAdjoint) {
+    if (a.nblock < a.ncirc) {
+      const int nwave = a.ncirc - a.nblock;
+      float2* P = lds + size_t(tid >> 6) * (2u << kWaveMaxQubits);
+      float2* Q = P + (1u << kWaveMaxQubits);
+      while (true) {
+        int w = 0;
+        if ((tid & 63) == 0) w = atomicAdd(a.wqueue, 1);
+        w = uniform(__shfl(w, 0, 64));
+        if (w >= nwave) break;
+        wave_circuit<MODE>(a, a.nblock + w, P, Q);
+      }
+    }
+    __syncthreads();
+  }
+
+  // The last workgroup out resets the queues so the launch can be replayed
   // (e.g. from a hipGraph) without a memset node.
   if (tid == 0) {
     __threadfence();
     const int done = atomicAdd(a.done, 1);
     if (done == int(gridDim.x) - 1) {
       atomicExch(a.queue, 0);
+      atomicExch(a.wqueue, 0);
       atomicExch(a.done, 0);
     }
   }

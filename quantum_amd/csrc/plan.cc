@@ -184,7 +184,10 @@ void SetTuning(const Tuning& t) {
 
 void InsertRemaps(std::vector<DevOp>* ops, size_t begin, int L) {
   if (!g_reg_engine.load()) return;
-  const int R = RegBitsForTile(L);
+  InsertRemapsR(ops, begin, L, RegBitsForTile(L));
+}
+
+void InsertRemapsR(std::vector<DevOp>* ops, size_t begin, int L, int R) {
   if (R == 0 || begin >= ops->size()) return;
   for (size_t i = begin; i < ops->size(); ++i)
     if ((*ops)[i].k > 2) return;
@@ -517,13 +520,23 @@ void BuildPiece(const CircuitJob& job, int num_slots, const PlanOptions& opt, Pi
   if (!empty) FuseGates(c, 0, int(c.gates.size()), opt.fuse, &fused);
   const bool tiled = !empty && dc.n > opt.tile_log2;
   if (tiled) dc.flags |= kFlagTiled;
+  // Wave mode: every fused op (forward and reverse segments alike, fused at
+  // the same width) must fit the register engine's two-qubit ops.
+  bool wave = opt.wave && g_reg_engine.load() && !empty && !tiled && dc.n >= kWaveRegBits &&
+              dc.n <= kWaveMaxQubits && opt.fuse.max_fuse <= 2;
+  for (const auto& f : fused) wave = wave && f.k <= 2;
+  if (wave) dc.flags |= kFlagWave;
+  auto remaps = [&](size_t begin) {
+    if (wave) InsertRemapsR(&pc->ops, begin, dc.n, kWaveRegBits);
+    else InsertRemaps(&pc->ops, begin, dc.n);
+  };
   dc.fwd_begin = 0;
   dc.fwd_pass_begin = 0;
   if (tiled) {
     SchedulePasses(fused, dc.n, opt.tile_log2, &pc->passes, &pc->ops, &pc->mats, true);
   } else {
     for (const auto& f : fused) pc->ops.push_back(MakeDevOp(f, &pc->mats, rc, rec));
-    InsertRemaps(&pc->ops, size_t(dc.fwd_begin), dc.n);
+    remaps(size_t(dc.fwd_begin));
   }
   dc.fwd_end = int(pc->ops.size());
   dc.fwd_pass_end = int(pc->passes.size());
@@ -569,7 +582,7 @@ void BuildPiece(const CircuitJob& job, int num_slots, const PlanOptions& opt, Pi
       SchedulePasses(rev_ops, dc.n, opt.tile_log2, &pc->passes, &pc->ops, &pc->mats, true);
     } else {
       for (const auto& f : rev_ops) pc->ops.push_back(MakeDevOp(f, &pc->mats, rc, rec));
-      InsertRemaps(&pc->ops, size_t(dc.rev_begin), dc.n);
+      remaps(size_t(dc.rev_begin));
     }
     dc.rev_end = int(pc->ops.size());
     dc.rev_pass_end = int(pc->passes.size());
@@ -662,6 +675,11 @@ void Concatenate(std::vector<Piece>& pieces, int num_slots, int num_symbols, con
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int b) { return pieces[a].cost > pieces[b].cost; });
+  // Wave-mode circuits run after every workgroup-sized one.
+  const auto wave_begin = std::stable_partition(order.begin(), order.end(), [&](int i) {
+    return (pieces[i].dc.flags & kFlagWave) == 0;
+  });
+  out->num_block = int(wave_begin - order.begin());
   out->circuits.clear();
   out->others.clear();
   out->passes.clear();

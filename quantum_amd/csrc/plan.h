@@ -79,6 +79,8 @@ void FuseGates(const Circuit& c, int begin, int end, const FuseOptions& opt,
 // the op order changes only between commuting ops.  No-op when the tile size
 // has no register engine or an op is wider than 2 qubits.
 void InsertRemaps(std::vector<DevOp>* ops, size_t begin, int L);
+// The same with an explicit register-qubit count R (wave mode).
+void InsertRemapsR(std::vector<DevOp>* ops, size_t begin, int L, int R);
 
 // Planner tuning (process-wide; 0 = default): register engine on/off and the
 // streaming path's tile qubits for one- and two-state passes.
@@ -106,6 +108,7 @@ struct BatchProgram {
   std::vector<float> ident;           // [B * O] identity-term constants
   std::vector<uint32_t> row_mat_base; // shared programs: row -> first matrix of the row
   int max_n = 0;
+  int num_block = 0;                  // circuits run by whole workgroups (the rest: wave mode)
   int num_rows = 0, num_slots = 0, num_symbols = 0;
   bool adjoint = false;
   bool any_tiled = false;
@@ -125,6 +128,7 @@ struct PlanOptions {
   bool adjoint = false;
   bool tfq_fd = true;     // adjoint derivative: TFQ's float32 FD (true) or exact
   int tile_log2 = 30;     // circuits wider than this run tiled (state in global scratch)
+  bool wave = false;      // small circuits (kWaveRegBits..kWaveMaxQubits qubits) in wave mode
 };
 
 // Build the per-circuit program pieces and concatenate them.

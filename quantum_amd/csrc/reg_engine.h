@@ -123,7 +123,7 @@ __device__ __forceinline__ uint32_t rdep(const RegMap<R>& m, int i) {
   return o;
 }
 
-template <int R>
+template <int R, bool WAVE = false>
 __device__ __forceinline__ void make_map(RegMap<R>& m, uint32_t mask, int L) {
   mask = uint32_t(uniform(int(mask)));
   m.mask = mask;
@@ -133,7 +133,9 @@ __device__ __forceinline__ void make_map(RegMap<R>& m, uint32_t mask, int L) {
     m.rpos[j] = mm ? uint32_t(__builtin_ctz(mm)) : 0u;  // the planner always sets R bits
     mm &= mm - 1;
   }
-  const uint32_t tid = threadIdx.x;
+  // Wave mode: the lane id supplies the L - R <= 6 thread bits; lanes whose
+  // higher bits are set replicate a lane below them.
+  const uint32_t tid = WAVE ? (threadIdx.x & 63u) : threadIdx.x;
   uint32_t tb = 0;
   int k = 0;
   for (int p = 0; p < L; ++p) {
@@ -521,6 +523,36 @@ __device__ __forceinline__ uint32_t op_mat_len(const DevOp& op) {
   return DD * (1u + (op.type == kOpParam ? uint32_t(op.nderiv) : 0u));
 }
 
+// Matrices read straight from the plan's array (wave mode: waves run
+// independent circuits, so no workgroup-wide refill is possible).
+struct GlobalMats {
+  const float2* g;
+};
+// Make an op's matrices readable and return the array its offsets index
+// (offsets rebased to the LDS window when cached).
+__device__ __forceinline__ const float2* mats_for(MatCache& c, DevOp& op) {
+  cache_span(c, op.mat, op_mat_len(op));
+  op.mat -= c.lo;
+  op.dmat[0] -= c.lo;
+  op.dmat[1] -= c.lo;
+  return c.lds;
+}
+__device__ __forceinline__ const float2* mats_for(GlobalMats& c, DevOp&) { return c.g; }
+
+// Engine synchronisation: the workgroup (one tile per workgroup) or, in wave
+// mode, the wavefront (one circuit per wavefront; its LDS traffic is ordered
+// by the wavefront's own in-order LDS queue).
+template <bool WAVE>
+__device__ __forceinline__ void esync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 // Tile I/O of the engine.  LdsIO: the tile enters and leaves in LDS buffers A
 // (and B) in the natural layout; the caller synchronises before the run and
 // the engine synchronises before returning.  TileIO: the first remap loads the
@@ -555,11 +587,12 @@ struct IsLdsIO<LdsIO> {
 // parameterised ops and column being the derivative's symbol column.
 // Ops [b, e) must start with a kOpRemap.  A and B also serve as the remap
 // buffers.
-template <int R, bool REV, typename AccFn, typename IO>
+template <int R, bool REV, bool WAVE, typename MS, typename AccFn, typename IO>
 __device__ __forceinline__ void reg_run(float2* __restrict__ A, float2* __restrict__ B, int L,
                         const DevOp* __restrict__ ops, int b, int e,
-                        MatCache& mc, AccFn&& acc_fn, IO& io) {
-  const float2* mats = mc.lds;
+                        MS& mc, AccFn&& acc_fn, IO& io) {
+  // Wave mode: gradient partials count each amplitude once (replica lanes 0).
+  const bool canon = !WAVE || ((threadIdx.x & 63u) >> (L - R)) == 0u;
   constexpr bool kLds = IsLdsIO<IO>::value;
   cf ra[1 << R];
   cf rb[1 << R];  // unused when !REV
@@ -568,17 +601,11 @@ __device__ __forceinline__ void reg_run(float2* __restrict__ A, float2* __restri
   int slot = 0;
   for (int i = b; i < e; ++i) {
     DevOp op = load_op(ops + i);
-    if (op.type != kOpRemap) {
-      // Matrices from the LDS window: offsets relative to its base.
-      cache_span(mc, op.mat, op_mat_len(op));
-      op.mat -= mc.lo;
-      op.dmat[0] -= mc.lo;
-      op.dmat[1] -= mc.lo;
-    }
+    const float2* mats = op.type != kOpRemap ? mats_for(mc, op) : nullptr;
     if (op.type == kOpRemap) {
       if constexpr (!kLds) {
         if (!have) {
-          make_map<R>(m, op.cmask, L);
+          make_map<R, WAVE>(m, op.cmask, L);
 #pragma unroll
           for (int r = 0; r < (1 << R); ++r) io.load(m.tbase | rdep(m, r), ra[r], rb[r]);
           have = true;
@@ -586,13 +613,13 @@ __device__ __forceinline__ void reg_run(float2* __restrict__ A, float2* __restri
         }
       }
       if (have) {
-        __syncthreads();
+        esync<WAVE>();
         reg_store<R>(ra, A, m, true);
         if (REV) reg_store<R>(rb, B, m, true);
         swizzled = true;
       }
-      __syncthreads();
-      make_map<R>(m, op.cmask, L);
+      esync<WAVE>();
+      make_map<R, WAVE>(m, op.cmask, L);
       reg_load<R>(ra, A, m, swizzled);
       if (REV) reg_load<R>(rb, B, m, swizzled);
       have = true;
@@ -604,6 +631,7 @@ __device__ __forceinline__ void reg_run(float2* __restrict__ A, float2* __restri
       double acc[kMaxDeriv] = {0.0, 0.0};
       if (op.k == 1) param_dispatch1<R>(reg_slot(m, op.q[0]), ra, rb, op, mats, m, acc);
       else param_dispatch2<R>(reg_slot(m, op.q[0]), reg_slot(m, op.q[1]), ra, rb, op, mats, m, acc);
+      if (!canon) acc[0] = acc[1] = 0.0;
       const double w0 = wave_sum(acc[0]);
       if ((threadIdx.x & 63) == 0) acc_fn(slot, op.col[0], w0);
       if (op.nderiv > 1) {
@@ -617,10 +645,10 @@ __device__ __forceinline__ void reg_run(float2* __restrict__ A, float2* __restri
   }
   if (!have) return;
   if constexpr (kLds) {
-    __syncthreads();
+    esync<WAVE>();
     reg_store<R>(ra, A, m, false);
     if (REV) reg_store<R>(rb, B, m, false);
-    __syncthreads();
+    esync<WAVE>();
   } else {
 #pragma unroll
     for (int r = 0; r < (1 << R); ++r) io.store(m.tbase | rdep(m, r), ra[r], rb[r]);
@@ -636,8 +664,8 @@ __device__ __forceinline__ bool reg_run_any(float2* A, float2* B, int L, const D
   if (b >= e) return false;
   if (uniform(int(ops[b].type)) != kOpRemap) return false;
   switch (RegBitsForTile(L)) {
-    case 3: reg_run<3, REV>(A, B, L, ops, b, e, mc, acc_fn, io); return true;
-    case 4: reg_run<4, REV>(A, B, L, ops, b, e, mc, acc_fn, io); return true;
+    case 3: reg_run<3, REV, false>(A, B, L, ops, b, e, mc, acc_fn, io); return true;
+    case 4: reg_run<4, REV, false>(A, B, L, ops, b, e, mc, acc_fn, io); return true;
     default: return false;
   }
 }
@@ -646,6 +674,17 @@ __device__ __forceinline__ bool reg_run_any(float2* A, float2* B, int L, const D
                                             int e, MatCache& mc, AccFn&& acc_fn) {
   LdsIO io;
   return reg_run_any<REV>(A, B, L, ops, b, e, mc, acc_fn, io);
+}
+
+// Wave mode: one wavefront runs a whole circuit of L = kWaveRegBits..
+// kWaveMaxQubits qubits (ops planned with remaps for kWaveRegBits register
+// qubits), A and B being the wavefront's own LDS buffers of 2^L amplitudes.
+template <bool REV, typename AccFn, typename IO>
+__device__ __forceinline__ void reg_run_wave(float2* A, float2* B, int L, const DevOp* ops, int b,
+                                             int e, const float2* mats, AccFn&& acc_fn, IO& io) {
+  if (b >= e) return;
+  GlobalMats gm{mats};
+  reg_run<kWaveRegBits, REV, true>(A, B, L, ops, b, e, gm, acc_fn, io);
 }
 
 }  // namespace

@@ -36,6 +36,12 @@ constexpr int kRegThreadBits = 9;
 constexpr int kMatCache = 768;
 constexpr int RegBitsForTile(int L) { return (L >= 12 && L <= 13) ? L - kRegThreadBits : 0; }
 
+// Wave mode of the batched kernel: after the workgroup-sized circuits, each
+// wavefront runs one small circuit (kWaveRegBits..kWaveMaxQubits qubits) in
+// registers, kWaveRegBits register qubits per lane and the rest on lane bits.
+constexpr int kWaveRegBits = 4;
+constexpr int kWaveMaxQubits = 10;
+
 // One op on a tile.  Qubit positions are tile-local bit indices (ascending).
 // Matrix layout: row-major 2^k x 2^k complex64, index bit j <-> q[j].
 struct alignas(8) DevOp {
@@ -120,5 +126,6 @@ enum BatchMode : int {
 
 constexpr int kFlagEmpty = 1;
 constexpr int kFlagTiled = 2;
+constexpr int kFlagWave = 4;   // runs in wave mode (ops planned for kWaveRegBits)
 
 }  // namespace tfqx

@@ -400,6 +400,25 @@ def test_register_engine_levels_match_oracle(streaming_case, tiled_case, level, 
         _lib.set_tuning()
 
 
+# Wave mode: circuits of 4..10 qubits run one per wavefront after the
+# workgroup-sized ones; 4 qubits leaves every lane a replica of lane 0, 10 uses
+# all 64 lanes.  Controls exercise the predicated register paths.
+@pytest.mark.parametrize("widths", [[4, 5, 10, 14, 7, 13], [4, 6, 8, 10]])
+def test_wave_mode_batches(widths):
+    sim, adj, _ = ops()
+    circuits, names, vals, psums = _mixed_batch(803 + len(widths), widths)
+    progs, ps = _ser(circuits), _ser(psums)
+    g = np.random.default_rng(7).uniform(-1, 1, size=(len(circuits), 2)).astype(np.float32)
+    _say(f"wave mode widths {widths}")
+    exp, grad = adj.expectation_and_gradient(progs, names, vals, ps, g)
+    ref_e = O.simulate_expectation(list(progs), names, vals, ps)
+    ref_g = O.adjoint_gradient(list(progs), names, vals, ps, g)
+    np.testing.assert_allclose(_np(exp), ref_e, atol=TOL * max(1, np.abs(ref_e).max()))
+    np.testing.assert_allclose(_np(grad), ref_g, atol=TOL * max(1, np.abs(ref_g).max()))
+    out = _np(sim.tfq_simulate_expectation(progs, names, vals, ps))
+    np.testing.assert_allclose(out, ref_e, atol=TOL * max(1, np.abs(ref_e).max()))
+
+
 # ----------------------------------------------------------------------------- shared programs
 @pytest.mark.parametrize("derivative", ["tfq", "analytic"])
 def test_rebound_plan_matches_oracle_and_fresh_plan(derivative):

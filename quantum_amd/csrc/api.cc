@@ -672,6 +672,8 @@ int tfqx_plan_launch_plain(tfqx_plan* p, void* ws, void* output, void* stream, c
   for (size_t c = 0; c < p->prog.circuits.size(); ++c) {
     a.circuits = all + c;
     a.ncirc = 1;
+    // One circuit: run by the workgroup, or by a wavefront in wave mode.
+    a.nblock = int(c) < p->prog.num_block ? 1 : 0;
     hipError_t e = LaunchBatch(a, 1, AsStream(stream));
     if (e != hipSuccess) return CheckHip(e, err, len, "plain launch");
   }

@@ -366,7 +366,7 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(BatchArgs a) {
     __syncthreads();
     const int ci = uniform(s_ci);
     __syncthreads();
-    if (ci >= a.nblock) break;
+    if (ci >= a.nblock || ci >= a.ncirc) break;
     const DevCircuit c = a.circuits[ci];
     const int n = uniform(c.n);
     const int row = uniform(c.row);

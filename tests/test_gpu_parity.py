@@ -419,6 +419,33 @@ def test_wave_mode_batches(widths):
     np.testing.assert_allclose(out, ref_e, atol=TOL * max(1, np.abs(ref_e).max()))
 
 
+def test_plain_launch_matches_batched_launch():
+    """One launch per circuit (the bench's plain baseline) over a batch with
+    wave-mode, LDS-resident and tiled circuits gives the batched results."""
+    import torch
+    from quantum_amd import tfq_adj_grad_op as A
+    circuits, names, vals, psums = _mixed_batch(811, [4, 9, 12, 14, 6, 10])
+    progs, ps = _ser(circuits), _ser(psums)
+    g = np.random.default_rng(9).uniform(-1, 1, size=(len(circuits), 2)).astype(np.float32)
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream().cuda_stream
+    plan, shape = A.plan_adj_grad(progs, names, vals, ps, g, with_expectation=True)
+    out = torch.empty(shape, dtype=torch.float32, device=dev)
+    out_plain = torch.empty_like(out)
+    plan.upload(dev, stream)
+    _say("batched launch, mixed batch")
+    plan.launch(out, stream)
+    torch.cuda.synchronize()
+    _say("plain launches, mixed batch")
+    plan.launch(out_plain, stream, plain=True)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(_np(out_plain), _np(out), atol=1e-6)
+    ref_g = O.adjoint_gradient(list(progs), names, vals, ps, g)
+    B, S = ref_g.shape
+    got_g = _np(out).reshape(-1)[-B * S:].reshape(B, S)
+    np.testing.assert_allclose(got_g, ref_g, atol=TOL * max(1, np.abs(ref_g).max()))
+
+
 # ----------------------------------------------------------------------------- shared programs
 @pytest.mark.parametrize("derivative", ["tfq", "analytic"])
 def test_rebound_plan_matches_oracle_and_fresh_plan(derivative):

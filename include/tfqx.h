@@ -112,6 +112,23 @@ int tfqx_plan_inner_product(
     const uint8_t* other_programs, const int64_t* other_offsets, int64_t num_other,
     tfqx_plan** plan, char* err, int64_t err_len);
 
+/* TfqInnerProductGrad(programs[B], symbol_names[S], symbol_values[B,S],
+ *                     other_programs[B,K], prev_grad[B,K]) -> complex64 [B,S]
+ * Replaces core/ops/math_ops/tfq_inner_product_grad.cc:46-501 (Compute
+ * :51-208, adjoint sweep :259-314 and :380-444): out[i,s] = sum over gates using s of
+ * <dG psi_k | lambda_k>, lambda = sum_j prev_grad[i,j] |phi_ij>
+ * (AccumulateFusedCircuits, util_qsim.h:423-442); no conjugation (the Python
+ * wrapper conjugates, inner_product_op.py:65-68).  S must be positive;
+ * circuits of at most 13 qubits (TFQX_UNIMPLEMENTED beyond).  derivative as
+ * for tfqx_plan_adjoint_gradient. */
+int tfqx_plan_inner_product_grad(
+    const uint8_t* programs, const int64_t* program_offsets, int64_t batch,
+    const uint8_t* symbol_names, const int64_t* symbol_offsets, int64_t num_symbols,
+    const float* symbol_values,
+    const uint8_t* other_programs, const int64_t* other_offsets, int64_t num_others,
+    const float* prev_grad, int32_t derivative,
+    tfqx_plan** plan, char* err, int64_t err_len);
+
 /* TfqAdjointGradient(programs[B], symbol_names[S], symbol_values[B,S],
  * pauli_sums[B,O], downstream_grads[B,O]) -> float[B,S]. */
 int tfqx_plan_adjoint_gradient(

@@ -6,7 +6,8 @@ in this directory, and only as the checker.  The product (``quantum_amd``)
 never does, and fails loudly when its HIP library is missing.
 
 Contents:
-* ``tfq_oracle.py``  complex128 numpy restatement of TFQ's six ops;
+* ``tfq_oracle.py``  complex128 numpy restatement of TFQ's six ops and the
+                     inner product's gradient op;
 * ``wire.py``        an independent proto3 decoder for Program / PauliSum;
 * ``philox.py``      host replica of the device sampler's Philox4x32-10 stream;
 * ``mlp.py``         float64 restatement of the VQE energy surrogate network;

@@ -14,7 +14,9 @@ It restates, in numpy complex128, what TFQ 0.7.7 computes for:
   ``core/src/util_qsim.h:199-270``);
 * ``tfq_inner_product``         (``core/ops/math_ops/tfq_inner_product.cc:45-323``);
 * ``tfq_adj_grad``              (``core/ops/tfq_adj_grad_op.cc:46-427``,
-  ``core/src/adj_util.cc:37-302``, ``core/src/util_qsim.h:363-414``).
+  ``core/src/adj_util.cc:37-302``, ``core/src/util_qsim.h:363-414``);
+* ``tfq_inner_product_grad``    (``core/ops/math_ops/tfq_inner_product_grad.cc:46-501``,
+  ``core/src/util_qsim.h:423-442``).
 
 Circuit decoding restates ``core/src/program_resolution.cc:49-186`` (qubit
 renumbering) and ``core/src/circuit_parser_qsim.cc:53-596`` (gate ids, argument
@@ -563,6 +565,47 @@ def adjoint_gradient(programs, symbol_names, symbol_values, pauli_sums, downstre
                     s2 = np.where(ok, s2, 0)
                 s2 = apply_matrix(s2, dG, g.bits, n)
                 out[i, col[sym]] += 2.0 * np.vdot(s2, lam).real
+            lam = apply_gate(lam, g, n, dagger=True, U=U)
+    return out
+
+
+def inner_product_gradient(programs, symbol_names, symbol_values, other_programs,
+                           downstream_grads, mode="tfq"):
+    """``TfqInnerProductGrad`` (tfq_inner_product_grad.cc:210-315): complex
+    out[i, s] = sum over gates using s of <dG psi_k | lambda_k> with
+    lambda = sum_j g_ij |phi_ij> (AccumulateFusedCircuits, util_qsim.h:423-442,
+    float32 weights).  Empty programs give zeros.  mode as adjoint_gradient."""
+    names = [_name(s) for s in symbol_names]
+    col = {s: j for j, s in enumerate(names)}
+    maps = _symbol_maps(names, symbol_values)
+    B = len(programs)
+    K = np.asarray(other_programs, dtype=object).shape[1] if B else 0
+    grads = np.asarray(downstream_grads, dtype=np.float32).reshape(B, K)
+    out = np.zeros((B, len(names)), dtype=np.complex128)
+    for i in range(B):
+        prog = Program(programs[i], maps[i])
+        if not prog.gates:
+            continue
+        n = prog.n
+        lam = np.zeros(2 ** n, dtype=np.complex128)
+        for j in range(K):
+            other = Program(other_programs[i][j], {}, prog.qmap, prog.n)
+            lam += float(grads[i, j]) * simulate(other)
+        psi = simulate(prog)
+        for g in reversed(prog.gates):
+            U = g.unitary()
+            psi = apply_gate(psi, g, n, dagger=True, U=U)
+            for sym, pidx in g.symbols:
+                dG = _fd_matrix(g, pidx) if mode == "tfq" else _analytic_matrix(g, pidx)
+                s2 = psi.copy()
+                if g.cbits:
+                    idx = np.arange(2 ** n, dtype=np.int64)
+                    ok = np.ones(2 ** n, dtype=bool)
+                    for b, v in zip(g.cbits, g.cvals):
+                        ok &= ((idx >> b) & 1) == v
+                    s2 = np.where(ok, s2, 0)
+                s2 = apply_matrix(s2, dG, g.bits, n)
+                out[i, col[sym]] += np.vdot(s2, lam)
             lam = apply_gate(lam, g, n, dagger=True, U=U)
     return out
 

@@ -3,7 +3,7 @@
 Public surface (TFQ names):
   convert_to_tensor, from circuits / Pauli sums to serialized string tensors
   get_expectation_op, get_sampled_expectation_op, get_sampling_op, get_state_op
-  math.inner_product, math.fidelity
+  math.inner_product, math.fidelity (differentiable), math.inner_product_grad
   layers.Expectation, layers.SampledExpectation, layers.Sample, layers.State
   differentiators.Adjoint, differentiators.CentralDifference, ...
   tfq_simulate_ops.*, tfq_adj_grad_op.tfq_adj_grad (raw ops)

@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(_HERE, "libtfqx.so")
 
 TFQX_OK = 0
 TFQX_INVALID_ARGUMENT = 3
+TFQX_UNIMPLEMENTED = 12
 TFQX_GRAD_TFQ_FD = 0
 TFQX_GRAD_ANALYTIC = 1
 
@@ -46,6 +47,7 @@ def _declare(lib):
     lib.tfqx_plan_simulate_samples.argtypes = s + [i32, u64, P, c.c_char_p, i64]
     lib.tfqx_plan_simulate_state.argtypes = s + [P, c.c_char_p, i64]
     lib.tfqx_plan_inner_product.argtypes = s + [P, P, i64, P, c.c_char_p, i64]
+    lib.tfqx_plan_inner_product_grad.argtypes = s + [P, P, i64, P, i32, P, c.c_char_p, i64]
     lib.tfqx_plan_adjoint_gradient.argtypes = s + [P, P, i64, P, i32, P, c.c_char_p, i64]
     lib.tfqx_plan_expectation_and_gradient.argtypes = s + [P, P, i64, P, i32, P, c.c_char_p, i64]
     lib.tfqx_debug_fuse.argtypes = [P, i64, P, P, i64, P, i32, P, i64, P, i64, P, P,
@@ -57,8 +59,8 @@ def _declare(lib):
     lib.tfqx_plan_stats.restype = c.c_int
     for name in ("tfqx_plan_simulate_expectation", "tfqx_plan_simulate_sampled_expectation",
                  "tfqx_plan_simulate_samples", "tfqx_plan_simulate_state",
-                 "tfqx_plan_inner_product", "tfqx_plan_adjoint_gradient",
-                 "tfqx_plan_expectation_and_gradient", "tfqx_plan_upload", "tfqx_plan_launch", "tfqx_plan_launch_plain",
+                 "tfqx_plan_inner_product", "tfqx_plan_inner_product_grad",
+                 "tfqx_plan_adjoint_gradient", "tfqx_plan_expectation_and_gradient", "tfqx_plan_upload", "tfqx_plan_launch", "tfqx_plan_launch_plain",
                  "tfqx_graph_capture", "tfqx_graph_launch"):
         getattr(lib, name).restype = c.c_int
     for name in ("tfqx_plan_max_num_qubits", "tfqx_plan_workspace_bytes", "tfqx_plan_output_bytes"):
@@ -107,7 +109,7 @@ def lib():
 EXPORTED_SYMBOLS = (
     "tfqx_plan_simulate_expectation", "tfqx_plan_simulate_sampled_expectation",
     "tfqx_plan_simulate_samples", "tfqx_plan_simulate_state", "tfqx_plan_inner_product",
-    "tfqx_plan_adjoint_gradient", "tfqx_plan_expectation_and_gradient", "tfqx_plan_rebind",
+    "tfqx_plan_inner_product_grad", "tfqx_plan_adjoint_gradient", "tfqx_plan_expectation_and_gradient", "tfqx_plan_rebind",
     "tfqx_debug_fuse",
     "tfqx_plan_stats",
     "tfqx_plan_max_num_qubits", "tfqx_plan_workspace_bytes",
@@ -123,6 +125,8 @@ def check(rc, err):
     msg = err.value.decode("utf-8", "replace")
     if rc == TFQX_INVALID_ARGUMENT:
         raise InvalidArgumentError(msg)
+    if rc == TFQX_UNIMPLEMENTED:
+        raise NotImplementedError(msg)
     raise DeviceError(f"tfqx error {rc}: {msg}")
 
 

@@ -242,6 +242,8 @@ void Layout(tfqx_plan* p) {
   p->off_gacc = off;
   const size_t image = off;
   if (p->mode == kModeAdjoint) off = Align(off + size_t(p->B) * size_t(p->S) * sizeof(double));
+  if (p->mode == kModeInnerProductGrad)
+    off = Align(off + 2 * size_t(p->B) * size_t(p->S) * sizeof(double));
   p->off_scratch = off;
   if (g.any_tiled) off = Align(off + size_t(kTiledGridCap) * 2 * (size_t(1) << p->max_nq) * sizeof(float2));
   p->total = off;
@@ -273,7 +275,7 @@ int BuildPlan(int mode, Inputs& in, int64_t B, int64_t S, int64_t O, int64_t K,
   for (const auto& c : in.circuits) p->max_nq = std::max(p->max_nq, c.num_qubits);
   PlanOptions opt;
   opt.mode = mode;
-  opt.adjoint = mode == kModeAdjoint;
+  opt.adjoint = mode == kModeAdjoint || mode == kModeInnerProductGrad;
   opt.tfq_fd = tfq_fd;
   opt.fuse.max_fuse = AutoFusionWidth(p->max_nq);
   const int resident = LdsLimit(mode);
@@ -281,6 +283,10 @@ int BuildPlan(int mode, Inputs& in, int64_t B, int64_t S, int64_t O, int64_t K,
                         p->max_nq <= kTiledMaxQubits;
   opt.tile_log2 = resident;
   opt.wave = mode == kModeExpectation || mode == kModeAdjoint;
+  if (mode == kModeInnerProductGrad && p->max_nq > resident)
+    return Fail(err, len, TFQX_UNIMPLEMENTED,
+                "tfq_inner_product_grad: circuits of at most " + std::to_string(resident) +
+                    " qubits are supported, got " + std::to_string(p->max_nq) + ".");
   if (p->max_nq > resident && !tiled_ok) {
     p->large = true;
     Status s = BuildLargePlan(mode, in.circuits, in.sums, downstream, in.others, B, S, O, K,
@@ -294,11 +300,12 @@ int BuildPlan(int mode, Inputs& in, int64_t B, int64_t S, int64_t O, int64_t K,
   for (int64_t i = 0; i < B; ++i) {
     jobs[i].circuit = &in.circuits[in.shared ? 0 : i];
     jobs[i].sums = in.sums[i].empty() ? nullptr : &in.sums[i];
-    jobs[i].downstream = downstream ? downstream + size_t(i) * O : nullptr;
+    const int64_t dstride = mode == kModeInnerProductGrad ? K : O;
+    jobs[i].downstream = downstream ? downstream + size_t(i) * dstride : nullptr;
     jobs[i].others = K > 0 ? &in.others[i] : nullptr;
     jobs[i].row = int(i);
   }
-  const int slots = mode == kModeInnerProduct ? int(K) : int(O);
+  const int slots = (mode == kModeInnerProduct || mode == kModeInnerProductGrad) ? int(K) : int(O);
   if (in.shared) {
     p->shared = true;
     p->tmpl = in.circuits[0];
@@ -511,6 +518,23 @@ int tfqx_plan_inner_product(const uint8_t* programs, const int64_t* poff, int64_
   return BuildPlan(kModeInnerProduct, in, B, S, 0, K, nullptr, true, plan, err, len);
 }
 
+int tfqx_plan_inner_product_grad(const uint8_t* programs, const int64_t* poff, int64_t B,
+                                 const uint8_t* names, const int64_t* noff, int64_t S,
+                                 const float* values, const uint8_t* other, const int64_t* ooff,
+                                 int64_t K, const float* downstream, int32_t derivative,
+                                 tfqx_plan** plan, char* err, int64_t len) {
+  if (S <= 0)
+    return Fail(err, len, TFQX_INVALID_ARGUMENT,
+                "The number of symbols must be a positive integer, got " + std::to_string(S) +
+                    " symbols.");
+  Inputs in;
+  Status s = Prepare(programs, poff, B, names, noff, S, values, nullptr, nullptr, 0, other, ooff,
+                     K, &in);
+  if (!s.ok()) return Fail(err, len, s.code, s.msg);
+  return BuildPlan(kModeInnerProductGrad, in, B, S, 0, K, downstream,
+                   derivative == TFQX_GRAD_TFQ_FD, plan, err, len);
+}
+
 int tfqx_plan_adjoint_gradient(const uint8_t* programs, const int64_t* poff, int64_t B,
                                const uint8_t* names, const int64_t* noff, int64_t S,
                                const float* values, const uint8_t* psums, const int64_t* psoff,
@@ -637,6 +661,7 @@ int64_t tfqx_plan_output_bytes(const tfqx_plan* p) {
     case kModeAdjoint: return (p->B * p->S + (p->with_expectation ? p->B * p->O : 0)) * 4;
     case kModeState: return p->B * (int64_t(1) << p->max_nq) * 8;
     case kModeSamples: return p->B * int64_t(p->nshots) * p->max_nq;
+    case kModeInnerProductGrad: return p->B * p->S * 8;
     default: return p->B * p->K * 8;
   }
 }

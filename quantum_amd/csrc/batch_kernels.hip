@@ -28,7 +28,7 @@ namespace tfqx {
 namespace {
 
 struct NoAcc {
-  __device__ void operator()(int, int, double) const {}
+  __device__ void operator()(int, int, double, double) const {}
 };
 
 // Forward ops [b, e) on an LDS tile of n qubits (register engine when planned).
@@ -41,7 +41,7 @@ __device__ __forceinline__ void run_ops(float2* s, int n, const DevOp* ops, int 
 }
 
 // Adjoint sweep of ops [b, e) on psi (A) and lambda (B), gradients into gacc.
-template <int KMAX>
+template <int KMAX, bool CPLX = false>
 __device__ __forceinline__ void run_reverse(float2* A, float2* B, int n, const DevOp* ops, int b, int e,
                             const float2* mats, float2* mbuf, double* gacc, double* red,
                             MatCache& mc);
@@ -126,9 +126,13 @@ __device__ __forceinline__ void build_lambda(const float2* __restrict__ S, float
   __syncthreads();
 }
 
+// CPLX = false: gacc[col] += 2 Re<dG psi | lam> (adjoint gradient).
+// CPLX = true: (gacc[2 col], gacc[2 col + 1]) += <dG psi | lam> (inner-product
+// gradient).
+template <bool CPLX>
 __device__ __forceinline__ void param_step(float2* A, float2* B, int n, const DevOp& op,
                                            const float2* M, double* gacc, double* red) {
-  double acc[kMaxDeriv] = {0.0, 0.0};
+  double acc[2 * kMaxDeriv] = {0.0, 0.0, 0.0, 0.0};
   switch (op.k) {
     case 1: apply_param<1>(A, B, n, op, M, acc); break;
     default: apply_param<2>(A, B, n, op, M, acc); break;
@@ -137,19 +141,34 @@ __device__ __forceinline__ void param_step(float2* A, float2* B, int n, const De
   for (int d = 0; d < kMaxDeriv; ++d) {
     if (d >= op.nderiv) break;
     const double tot = block_sum(acc[d], red);
-    if (threadIdx.x == 0) gacc[op.col[d]] += 2.0 * tot;
+    if (CPLX) {
+      const double tim = block_sum(acc[kMaxDeriv + d], red);
+      if (threadIdx.x == 0) {
+        gacc[2 * op.col[d]] += tot;
+        gacc[2 * op.col[d] + 1] += tim;
+      }
+    } else if (threadIdx.x == 0) {
+      gacc[op.col[d]] += 2.0 * tot;
+    }
   }
 }
 
-template <int KMAX>
+template <int KMAX, bool CPLX>
 __device__ __forceinline__ void run_reverse(float2* A, float2* B, int n, const DevOp* ops, int b, int e,
                             const float2* mats, float2* mbuf, double* gacc, double* red,
                             MatCache& mc) {
-  auto acc = [gacc](int, int col, double w) { atomicAdd(&gacc[col], 2.0 * w); };
+  auto acc = [gacc](int, int col, double w, double v) {
+    if (CPLX) {
+      atomicAdd(&gacc[2 * col], w);
+      atomicAdd(&gacc[2 * col + 1], v);
+    } else {
+      atomicAdd(&gacc[col], 2.0 * w);
+    }
+  };
   if (reg_run_any<true>(A, B, n, ops, b, e, mc, acc)) return;
   for_ops(ops, b, e, mats, mbuf, [&](const DevOp& op, const float2* M) {
     if (op.type == kOpParam) {
-      param_step(A, B, n, op, M, gacc, red);
+      param_step<CPLX>(A, B, n, op, M, gacc, red);
     } else {
       apply_op<true, KMAX>(A, n, op, M);
       apply_op<true, KMAX>(B, n, op, M);
@@ -235,7 +254,7 @@ __device__ __forceinline__ void wave_circuit(const BatchArgs& a, int ci, float2*
           vb = cf{v.x, v.y};
         },
         [](uint32_t, const cf&, const cf&) {});
-    auto acc = [gacc](int, int col, double w) { atomicAdd(&gacc[col], 2.0 * w); };
+    auto acc = [gacc](int, int col, double w, double) { atomicAdd(&gacc[col], 2.0 * w); };
     reg_run_wave<true>(P, Q, n, a.ops, uniform(c.rev_begin), uniform(c.rev_end), a.mats, acc, rio);
     __threadfence();
     for (int j = int(lane); j < a.nsym; j += 64)
@@ -328,7 +347,7 @@ __device__ __forceinline__ void tiled_reverse(float2* G, float2* Lg, int n, cons
             G[g] = make_float2(va.x, va.y);
             Lg[g] = make_float2(vb.x, vb.y);
           });
-      auto acc = [gacc](int, int col, double w) { atomicAdd(&gacc[col], 2.0 * w); };
+      auto acc = [gacc](int, int col, double w, double) { atomicAdd(&gacc[col], 2.0 * w); };
       if (reg_run_any<true>(A, B, ps.L, a.ops, ps.op_begin, ps.op_end, mc, acc, io)) continue;
       tile_gather2(A, B, G, Lg, T, gidx);
       __syncthreads();
@@ -383,6 +402,9 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(BatchArgs a) {
       if (MODE == kModeInnerProduct)
         for (int j = tid; j < a.nslots; j += kThreads)
           a.ip_out[size_t(row) * a.nslots + j] = make_float2(1.f, 0.f);
+      if (MODE == kModeInnerProductGrad)
+        for (int j = tid; j < a.nsym; j += kThreads)
+          a.ip_out[size_t(row) * a.nsym + j] = make_float2(0.f, 0.f);
       if (MODE == kModeState)
         for (long long x = tid; x < a.state_stride; x += kThreads)
           a.state_out[size_t(row) * a.state_stride + x] = make_float2(x == 0 ? 1.f : -2.f, 0.f);
@@ -392,6 +414,23 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(BatchArgs a) {
       continue;
     }
 
+    if (MODE == kModeInnerProductGrad) {
+      // lambda = sum_j g_j |phi_j> in buf2 (AccumulateFusedCircuits,
+      // util_qsim.h:423-442), each phi_j simulated in the psi buffer first.
+      float2* lam = buf2;
+      for (uint32_t x = tid; x < N; x += kThreads) lam[x] = make_float2(0.f, 0.f);
+      for (int o = uniform(c.other_begin); o < c.other_end; ++o) {
+        const DevOther ot = a.others[o];
+        set_zero_state(psi, N);
+        run_ops<KMAX>(psi, n, a.ops, uniform(ot.op_begin), uniform(ot.op_end), a.mats, mbuf, mc);
+        const float w = ot.weight;
+        for (uint32_t x = tid; x < N; x += kThreads) {
+          const float2 p = psi[x];
+          lam[x] = make_float2(lam[x].x + w * p.x, lam[x].y + w * p.y);
+        }
+        __syncthreads();
+      }
+    }
     if (tiled) {
       tiled_forward<KMAX>(gpsi, n, a, uniform(c.fwd_pass_begin), uniform(c.fwd_pass_end), psi, mbuf,
                     tab, mc);
@@ -426,6 +465,22 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(BatchArgs a) {
       for (int j = tid; j < a.nsym; j += kThreads)
         a.grad_out[size_t(row) * a.nsym + j] =
             float(__hip_atomic_load(&gacc[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+
+    if (MODE == kModeInnerProductGrad) {
+      // Adjoint sweep with complex accumulation (tfq_inner_product_grad.cc:259-314).
+      double* gacc = a.grad_acc + size_t(row) * a.nsym * 2;
+      for (int j = tid; j < 2 * a.nsym; j += kThreads) gacc[j] = 0.0;
+      __syncthreads();
+      run_reverse<KMAX, true>(psi, buf2, n, a.ops, uniform(c.rev_begin), uniform(c.rev_end), a.mats,
+                              mbuf, gacc, red, mc);
+      __threadfence();
+      __syncthreads();
+      for (int j = tid; j < a.nsym; j += kThreads)
+        a.ip_out[size_t(row) * a.nsym + j] =
+            make_float2(float(__hip_atomic_load(&gacc[2 * j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)),
+                        float(__hip_atomic_load(&gacc[2 * j + 1], __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT)));
     }
 
     if (MODE == kModeState) {
@@ -551,6 +606,7 @@ const void* KernelFor(int mode, int kmax) {
     case kModeState: return KernelPtr<kModeState>(kmax);
     case kModeSamples: return KernelPtr<kModeSamples>(kmax);
     case kModeSampledExpectation: return KernelPtr<kModeSampledExpectation>(kmax);
+    case kModeInnerProductGrad: return KernelPtr<kModeInnerProductGrad>(kmax);
     default: return KernelPtr<kModeInnerProduct>(kmax);
   }
 }
@@ -558,7 +614,10 @@ const void* KernelFor(int mode, int kmax) {
 }  // namespace
 
 int BatchBuffers(int mode) {
-  return (mode == kModeAdjoint || mode == kModeSampledExpectation || mode == kModeInnerProduct) ? 2 : 1;
+  return (mode == kModeAdjoint || mode == kModeSampledExpectation || mode == kModeInnerProduct ||
+          mode == kModeInnerProductGrad)
+             ? 2
+             : 1;
 }
 
 size_t BatchLdsBytes(int lds_log2, int mode) {

@@ -73,7 +73,7 @@ __global__ __launch_bounds__(kThreads, TWO ? 2 : 4) void pass_kernel(PassArgs a)
       return use_tab ? tile_index_tab(base, j, nlow, tab) : tile_index(base, j, ps);
     };
     int pl = 0;
-    auto acc = [](int slot, int, double w) { atomicAdd(&pacc[slot], w); };
+    auto acc = [](int slot, int, double w, double) { atomicAdd(&pacc[slot], w); };
     // Register engine: the tile goes straight between memory and registers.
     auto io = tile_io(
         [&](uint32_t x, cf& va, cf& vb) {
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(kThreads, TWO ? 2 : 4) void pass_kernel(PassArgs a)
               if (!TWO) {
                 apply_op<false>(A, L, op, M);
               } else if (op.type == kOpParam) {
-                double acc[kMaxDeriv] = {0.0, 0.0};
+                double acc[2 * kMaxDeriv] = {0.0, 0.0, 0.0, 0.0};
                 if (op.k == 1) apply_param<1>(A, Bf, L, op, M, acc);
                 else apply_param<2>(A, Bf, L, op, M, acc);
 #pragma unroll

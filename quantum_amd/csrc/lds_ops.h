@@ -253,7 +253,8 @@ __device__ __forceinline__ void apply_1q(float2* __restrict__ s, int n, int q, c
 }
 
 // Adjoint step for one parameterised gate G (targets q, controls cmask):
-//   psi <- G^dag psi;  acc_d += Re<lam| dG_d psi>;  lam <- G^dag lam.
+//   psi <- G^dag psi;  acc_d += <dG_d psi | lam> (real part in acc[d], imaginary
+//   part in acc[kMaxDeriv + d]);  lam <- G^dag lam.
 // Control-mismatched amplitudes are untouched and contribute nothing
 // (the BulkSetAmpl projection of tfq_adj_grad_op.cc:241-260).
 template <int K>
@@ -299,15 +300,17 @@ __device__ __forceinline__ void apply_param(float2* __restrict__ psi, float2* __
     for (int d = 0; d < kMaxDeriv; ++d) {
       if (d >= nd) break;
       const float2* dG = M + D * D * (1 + d);
-      float part = 0.f;
+      float part = 0.f, part_im = 0.f;
 #pragma unroll
       for (int r = 0; r < D; ++r) {
         float2 s = make_float2(0.f, 0.f);
 #pragma unroll
         for (int c = 0; c < D; ++c) s = c == 0 ? cmul(dG[r * D], t[0]) : cmacf(s, dG[r * D + c], t[c]);
-        part += s.x * b[r].x + s.y * b[r].y;  // Re(conj(s) * lam)
+        part += s.x * b[r].x + s.y * b[r].y;     // Re(conj(s) * lam)
+        part_im += s.x * b[r].y - s.y * b[r].x;  // Im(conj(s) * lam)
       }
       acc[d] += double(part);
+      acc[kMaxDeriv + d] += double(part_im);
     }
 #pragma unroll
     for (int r = 0; r < D; ++r) {

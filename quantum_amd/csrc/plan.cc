@@ -489,7 +489,9 @@ double AddTerms(const CircuitJob& job, int num_slots, const PlanOptions& opt, Pi
   }
   dc.term_end = int(pc->terms.size());
   dc.lam_begin = dc.lam_end = dc.term_end;
-  if (opt.adjoint && !empty) {
+  // Adjoint lambda terms (the inner-product gradient builds lambda from its
+  // paired programs instead and has no Pauli sums).
+  if (opt.adjoint && !empty && job.sums) {
     for (int j = 0; j < num_slots; ++j) {
       const float gj = job.downstream[j];
       for (const auto& p : (*job.sums)[j]) {
@@ -588,7 +590,8 @@ void BuildPiece(const CircuitJob& job, int num_slots, const PlanOptions& opt, Pi
     dc.rev_pass_end = int(pc->passes.size());
   }
   if (tiled) cost *= 3.0;  // streams through L2 instead of staying in LDS
-  if (opt.mode == kModeInnerProduct && !empty && job.others) {
+  if ((opt.mode == kModeInnerProduct || opt.mode == kModeInnerProductGrad) && !empty &&
+      job.others) {
     for (size_t k = 0; k < job.others->size(); ++k) {
       const Circuit& oc = (*job.others)[k];
       std::vector<FusedOp> of;
@@ -602,6 +605,7 @@ void BuildPiece(const CircuitJob& job, int num_slots, const PlanOptions& opt, Pi
       InsertRemaps(&pc->ops, size_t(o.op_begin), dc.n);
       o.op_end = int(pc->ops.size());
       o.col = int(k);
+      o.weight = opt.mode == kModeInnerProductGrad && job.downstream ? job.downstream[k] : 0.0f;
       pc->others.push_back(o);
     }
   }

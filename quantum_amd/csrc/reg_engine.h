@@ -354,7 +354,8 @@ __device__ __forceinline__ void reg_diag(cf (&a)[1 << R], cf (&b)[1 << R], const
 }
 
 // ------------------------------------------------------------------ parameterised ops
-// psi <- G^dag psi;  acc_d += Re<lam| dG_d psi>;  lam <- G^dag lam  (see apply_param).
+// psi <- G^dag psi;  acc_d += <dG_d psi | lam> (real part in acc[d], imaginary
+// part in acc[kMaxDeriv + d]);  lam <- G^dag lam  (see apply_param).
 // Three sweeps over the registers so that only one matrix is live at a time.
 template <int R, int K, int J0, int J1>
 __device__ __forceinline__ int param_idx(int g, int s) {
@@ -413,7 +414,7 @@ __device__ __forceinline__ void reg_param_kc(cf (&a)[1 << R], cf (&b)[1 << R], c
     if constexpr (K == 2 && !CTRL) {
       MfmaRows rows;
       mfma_rows<false>(dG, rows);
-      float part = 0.f;
+      float part = 0.f, part_im = 0.f;
 #pragma unroll
       for (int g = 0; g < (1 << (R - 2)); ++g) {
         int idx[4];
@@ -425,16 +426,23 @@ __device__ __forceinline__ void reg_param_kc(cf (&a)[1 << R], cf (&b)[1 << R], c
           const cf l = b[idx[r]];
           part = __builtin_fmaf(dr[r], l.x, part);
           part = __builtin_fmaf(di[r], l.y, part);
+          part_im = __builtin_fmaf(dr[r], l.y, part_im);
+          part_im = __builtin_fmaf(-di[r], l.x, part_im);
         }
       }
-      if (d == 0) acc[0] += double(part);
-      else acc[1] += double(part);
+      if (d == 0) {
+        acc[0] += double(part);
+        acc[kMaxDeriv] += double(part_im);
+      } else {
+        acc[1] += double(part);
+        acc[kMaxDeriv + 1] += double(part_im);
+      }
       continue;
     }
     cf dg[D * D];
 #pragma unroll
     for (int i = 0; i < D * D; ++i) dg[i] = ldm(dG, i);
-    float part = 0.f;
+    float part = 0.f, part_im = 0.f;
 #pragma unroll
     for (int g = 0; g < (1 << (R - K)); ++g) {
       const int i0 = param_idx<R, K, J0, J1>(g, 0);
@@ -446,10 +454,16 @@ __device__ __forceinline__ void reg_param_kc(cf (&a)[1 << R], cf (&b)[1 << R], c
         for (int c = 1; c < D; ++c) s = cmac(s, dg[r * D + c], a[param_idx<R, K, J0, J1>(g, c)]);
         const cf l = b[param_idx<R, K, J0, J1>(g, r)];
         part += s.x * l.x + s.y * l.y;
+        part_im += s.x * l.y - s.y * l.x;
       }
     }
-    if (d == 0) acc[0] += double(part);
-    else acc[1] += double(part);
+    if (d == 0) {
+      acc[0] += double(part);
+      acc[kMaxDeriv] += double(part_im);
+    } else {
+      acc[1] += double(part);
+      acc[kMaxDeriv + 1] += double(part_im);
+    }
   }
   reg_dag_apply<R, K, J0, J1, CTRL>(b, mats + op.mat, op, m);
 }
@@ -582,7 +596,8 @@ struct IsLdsIO<LdsIO> {
 // Apply ops [b, e) to a tile of 2^L amplitudes (A, and B in the reverse sweep).
 // REV = false: apply each op's matrix to A.  REV = true: un-apply (M^dagger)
 // non-parameterised ops on A and B; parameterised ops run the adjoint step
-// and call acc_fn(slot, column, wave_partial) from lane 0 of each wavefront
+// and call acc_fn(slot, column, re, im) from lane 0 of each wavefront with the
+// wavefront's partial <dG psi | lambda>
 // per derivative, slot counting the derivative matrices of earlier
 // parameterised ops and column being the derivative's symbol column.
 // Ops [b, e) must start with a kOpRemap.  A and B also serve as the remap
@@ -628,15 +643,17 @@ __device__ __forceinline__ void reg_run(float2* __restrict__ A, float2* __restri
     if (op.type == kOpDiag) {
       reg_diag<R, REV, REV>(ra, rb, op, mats + op.mat, m);
     } else if (REV && op.type == kOpParam) {
-      double acc[kMaxDeriv] = {0.0, 0.0};
+      double acc[2 * kMaxDeriv] = {0.0, 0.0, 0.0, 0.0};
       if (op.k == 1) param_dispatch1<R>(reg_slot(m, op.q[0]), ra, rb, op, mats, m, acc);
       else param_dispatch2<R>(reg_slot(m, op.q[0]), reg_slot(m, op.q[1]), ra, rb, op, mats, m, acc);
-      if (!canon) acc[0] = acc[1] = 0.0;
-      const double w0 = wave_sum(acc[0]);
-      if ((threadIdx.x & 63) == 0) acc_fn(slot, op.col[0], w0);
+      if (!canon) acc[0] = acc[1] = acc[2] = acc[3] = 0.0;
+      // The imaginary parts reach only the callers that use them (the
+      // compiler drops them elsewhere).
+      const double w0 = wave_sum(acc[0]), v0 = wave_sum(acc[kMaxDeriv]);
+      if ((threadIdx.x & 63) == 0) acc_fn(slot, op.col[0], w0, v0);
       if (op.nderiv > 1) {
-        const double w1 = wave_sum(acc[1]);
-        if ((threadIdx.x & 63) == 0) acc_fn(slot + 1, op.col[1], w1);
+        const double w1 = wave_sum(acc[1]), v1 = wave_sum(acc[kMaxDeriv + 1]);
+        if ((threadIdx.x & 63) == 0) acc_fn(slot + 1, op.col[1], w1, v1);
       }
       slot += op.nderiv;
     } else {

@@ -111,7 +111,7 @@ struct alignas(8) DevCircuit {
 struct alignas(8) DevOther {
   int32_t op_begin, op_end;
   int32_t col;            // output column
-  int32_t pad;
+  float weight;           // inner-product gradient: downstream weight g[row, col]
 };
 
 // Kernel modes of the persistent batch kernel.
@@ -122,6 +122,7 @@ enum BatchMode : int {
   kModeSamples = 3,
   kModeSampledExpectation = 4,
   kModeInnerProduct = 5,
+  kModeInnerProductGrad = 6,
 };
 
 constexpr int kFlagEmpty = 1;

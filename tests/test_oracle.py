@@ -13,6 +13,8 @@ How each oracle function is pinned:
   ``gradient_test.py`` closed form, and agreement with parameter shift /
   central differences of the oracle's own expectation on random circuits;
 * inner product: ``inner_product_op.py:85-109`` docstring matrix;
+* inner-product gradient: a one-qubit closed form, and agreement with
+  central differences of the oracle's own inner product on random circuits;
 * Philox stream: Random123 known-answer vectors;
 * sampler: chi-square against |psi|^2;
 * wire decoder: the official protobuf runtime parsing the same bytes.
@@ -174,6 +176,47 @@ def test_inner_product_docstring(goldens):
                           [[_ser(c) for c in others]] * 2)
     exp = np.array([[complex(*v) for v in row] for row in g["expected"]])
     np.testing.assert_allclose(out, exp, atol=g["tol"])
+
+
+def test_inner_product_gradient_closed_form():
+    """psi = X**a |0>, phi = |0>: <psi|phi> = (1 + e^{-i pi a}) / 2, so the
+    gradient with weight g is g * (-i pi / 2) e^{-i pi a}."""
+    q = C.LineQubit(0)
+    a = C.Symbol("a")
+    val, g = 0.37, 0.8
+    out = O.inner_product_gradient([_ser(C.Circuit((C.X ** a).on(q)))], ["a"],
+                                   np.array([[val]], np.float32),
+                                   [[_ser(C.Circuit(C.I(q)))]], np.array([[g]]),
+                                   mode="analytic")
+    a32 = float(np.float32(val))
+    want = g * (-1j * math.pi / 2) * np.exp(-1j * math.pi * a32)
+    np.testing.assert_allclose(out[0, 0], want, atol=1e-9)
+    tfq = O.inner_product_gradient([_ser(C.Circuit((C.X ** a).on(q)))], ["a"],
+                                   np.array([[val]], np.float32),
+                                   [[_ser(C.Circuit(C.I(q)))]], np.array([[g]]))
+    np.testing.assert_allclose(tfq[0, 0], want, atol=1e-4)
+
+
+def test_inner_product_gradient_vs_central_difference():
+    rng = np.random.default_rng(12)
+    q = C.LineQubit.range(4)
+    names = ["a", "b", "c"]
+    circ = W.random_circuit(rng, q, 25, symbols=names, p_symbol=0.6, p_control=0.2)
+    others = [[_ser(W.random_circuit(rng, q, 12) + C.Circuit([C.I(x) for x in q]))
+               for _ in range(2)]]
+    vals = rng.uniform(-1, 1, size=(1, 3)).astype(np.float32)
+    g = np.array([[0.7, -1.3]])
+    an = O.inner_product_gradient([_ser(circ)], names, vals, others, g, mode="analytic")[0]
+    h = 1e-3
+    for k in range(3):
+        vp = vals.astype(np.float64).copy()
+        vm = vp.copy()
+        vp[0, k] += h
+        vm[0, k] -= h
+        ip = (O.inner_product([_ser(circ)], names, vp, others)
+              - O.inner_product([_ser(circ)], names, vm, others))[0] / (2 * h)
+        # float32 symbol values limit CD accuracy to ~ eps32/h
+        np.testing.assert_allclose(an[k], np.dot(g[0], ip), atol=3e-3)
 
 
 # ----------------------------------------------------------------------------- brute force

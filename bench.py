@@ -18,7 +18,9 @@ measured with HIP events on the launch stream, against the FP32 peak.
 WRITE_SIZE counters, read from profiles/batch_kernel_uneven4096_traffic.json
 when that file matches this workload, else null.  `hbm_pass` is the 30-qubit
 streaming gate pass (5x6 grid, 14 cycles): 16 B x 2^30 per pass / average
-pass time, against 8 TB/s.
+pass time, against 8 TB/s.  `inner_product_grad` times the tfq.math
+inner-product gradient op on the 64 x 10-qubit rotation batch with two paired
+programs per row (graph replays), spot-checked against the oracle.
 """
 
 import argparse
@@ -116,6 +118,53 @@ def hbm_roofline(stream):
             "ms_per_pass": round(per_pass * 1e3, 4), "bytes_per_pass": bytes_per_pass}
 
 
+def inner_product_grad_measure(stream, steps=20, warmup=3, oracle_rows=2):
+    """tfq.math inner-product gradient on the 64 x 10-qubit rotation batch
+    (600 symbols), two symbol-free paired programs per row: one launch per
+    step from a hipGraph."""
+    import torch
+    from quantum_amd import _lib
+    from quantum_amd import circuits as C
+    from quantum_amd import serialize as S
+    from quantum_amd import workloads as W
+    from quantum_amd.tfq_simulate_ops import _common, _plan_args
+    circuits, names, values, _ = W.config1_line_rotations()
+    rng = np.random.default_rng(2024)
+    q = C.LineQubit.range(10)
+    others = np.array([[S.serialize_circuit(W.random_circuit(rng, q, 40) +
+                                            C.Circuit([C.I(x) for x in q]))
+                        for _ in range(2)] for _ in circuits], dtype=object)
+    g = rng.normal(size=(len(circuits), 2)).astype(np.float32)
+    progs, pnames, pvalues = _common(S.convert_to_tensor(circuits), names, values)
+    P, N, args = _plan_args(progs, pnames, pvalues)
+    Ot = _lib.StringTensor(others)
+    plan = _lib.Plan(_lib.lib().tfqx_plan_inner_product_grad, *args, Ot.ptr, Ot.off, 2,
+                     g.ctypes.data, 0)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    out = torch.empty((len(circuits), len(names)), dtype=torch.complex64, device=dev)
+    plan.upload(dev, stream)
+    elapsed, _, graph = time_graph(plan, out, stream, steps, warmup)
+    per = elapsed / steps
+    flops = plan.stats()["flops"]
+    res = out.cpu().numpy()
+    err = None
+    if oracle_rows:
+        from oracle import tfq_oracle as O
+        ref = O.inner_product_gradient(list(progs[:oracle_rows]), names, values[:oracle_rows],
+                                       others[:oracle_rows], g[:oracle_rows])
+        err = float(np.abs(res[:oracle_rows] - ref).max())
+    log(f"inner-product gradient: {per * 1e3:.3f} ms per 64-circuit launch, "
+        f"{len(circuits) / per:.0f} circuits/s, {flops / per / 1e12:.2f} TFLOP/s, "
+        f"max |gpu - oracle| = {err}")
+    del graph, plan
+    return {"workload": "64 circuits x 10 qubits (20 layers rx ry rz + CZ ladder, 600 symbols), "
+                        "2 symbol-free paired programs per row",
+            "circuits": len(circuits), "ms_per_launch": round(per * 1e3, 4),
+            "circuits_per_s": round(len(circuits) / per, 1),
+            "flops_per_launch": flops, "tflops": round(flops / per / 1e12, 3),
+            "max_abs_err_vs_oracle": err, "oracle_rows_checked": oracle_rows}
+
+
 def measured_traffic(circuits, flops):
     """HBM bytes per launch from the committed rocprofv3 counter summary, if it
     was taken on this same workload (same circuit count and flop count)."""
@@ -165,10 +214,11 @@ def main():
     ap.add_argument("--oracle-check", type=int, default=32,
                     help="circuits checked against the complex128 oracle")
     ap.add_argument("--no-roofline", action="store_true")
-    ap.add_argument("--only", choices=["all", "batch", "hbm", "copy"], default="all",
+    ap.add_argument("--only", choices=["all", "batch", "hbm", "copy", "ipgrad"], default="all",
                     help="profiling runs: 'batch' launches only the batched kernel (graph "
                          "replays, no plain launches, no CPU baseline); 'hbm' only the 30-qubit "
-                         "streaming passes; 'copy' a 1 GiB device copy (counter calibration)")
+                         "streaming passes; 'copy' a 1 GiB device copy (counter calibration); "
+                         "'ipgrad' the inner-product gradient batch")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py measures one GPU: this build has no multi-GPU path (each box has one "
@@ -194,6 +244,9 @@ def main():
         return
     if args.only == "hbm":
         print(json.dumps({"hbm_pass": hbm_roofline(stream)}), flush=True)
+        return
+    if args.only == "ipgrad":
+        print(json.dumps({"inner_product_grad": inner_product_grad_measure(stream)}), flush=True)
         return
     progs, names, values, ps, weights = build_uneven(args.circuits)
     t = time.time()
@@ -236,6 +289,7 @@ def main():
     if not args.no_roofline:
         del graph
         hbm = hbm_roofline(stream)
+    ipg = inner_product_grad_measure(stream)
 
     done, cpu_dt, cores, err, checked, err_cpu = cpu_baseline(
         progs, names, values, ps, weights, gpu_exp, gpu_grad, args.cpu_sample, args.oracle_check)
@@ -267,6 +321,7 @@ def main():
         "max_abs_err_vs_cpu_baseline": err_cpu,
         "roofline": roof,
         "hbm_pass": hbm,
+        "inner_product_grad": ipg,
         "cpu_baseline": {"value": round(done / cpu_dt, 3), "unit": "circuits/s", "cores": cores,
                          "kind": "port",
                          "sample": f"first {done} circuits of the same batch, expectation (basic "

@@ -178,8 +178,8 @@ int tfqx_debug_fuse(const uint8_t* program, int64_t program_len, const uint8_t* 
 /* Plan statistics: out[0] circuits, [1] forward ops, [2] forward HBM passes,
  * [3] max qubits, [4] adjoint ops, [5] adjoint HBM passes, [6] 1 if the
  * HBM-streaming path is used, [7] workspace bytes, [8] algorithmic real flops
- * of one launch (batch path; 0 on the streaming path).  No reference
- * counterpart. */
+ * of one launch (batch path; 0 on the streaming path), [9] the part of those
+ * flops that runs on FP32 MFMA (both paths).  No reference counterpart. */
 int tfqx_plan_stats(const tfqx_plan* plan, int64_t* out, int64_t n);
 
 /* Largest qubit count in the batch (sizes the samples / state outputs). */

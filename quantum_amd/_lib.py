@@ -167,10 +167,10 @@ class Plan:
         return int(self._lib.tfqx_plan_max_num_qubits(self.handle))
 
     def stats(self):
-        out = (ctypes.c_int64 * 9)()
-        self._lib.tfqx_plan_stats(self.handle, out, 9)
+        out = (ctypes.c_int64 * 10)()
+        self._lib.tfqx_plan_stats(self.handle, out, 10)
         keys = ("circuits", "forward_ops", "forward_passes", "max_qubits", "adjoint_ops",
-                "adjoint_passes", "streaming", "workspace_bytes", "flops")
+                "adjoint_passes", "streaming", "workspace_bytes", "flops", "mfma_flops")
         return dict(zip(keys, [int(x) for x in out]))
 
     def rebind(self, values):

@@ -360,6 +360,42 @@ double AlgorithmicFlops(const BatchProgram& g) {
   return f;
 }
 
+// Algorithmic flops of one launch that run on MFMA: uncontrolled two-qubit
+// dense and parameterised ops of op ranges the register engine runs (ranges
+// that start with a remap).  Per amplitude: 8*4 - 2 per product; the adjoint
+// applies G^dag to psi and lambda and one dG product per derivative.
+double MfmaFlopsOfRange(const std::vector<DevOp>& ops, int b, int e, double A, bool rev) {
+  if (b >= e || ops[size_t(b)].type != kOpRemap) return 0.0;
+  double f = 0;
+  for (int i = b; i < e; ++i) {
+    const DevOp& o = ops[size_t(i)];
+    if (o.k != 2 || o.cmask != 0) continue;
+    if (o.type == kOpUnitary) f += (rev ? 2.0 : 1.0) * A * 30.0;
+    if (o.type == kOpParam) f += (2.0 + o.nderiv) * A * 30.0;
+  }
+  return f;
+}
+
+double MfmaFlops(const BatchProgram& g) {
+  double f = 0;
+  for (const auto& c : g.circuits) {
+    if (c.flags & kFlagEmpty) continue;
+    const double A = std::ldexp(1.0, c.n);
+    if (c.flags & kFlagTiled) {
+      for (int k = c.fwd_pass_begin; k < c.fwd_pass_end; ++k)
+        f += MfmaFlopsOfRange(g.ops, g.passes[size_t(k)].op_begin, g.passes[size_t(k)].op_end, A, false);
+      for (int k = c.rev_pass_begin; k < c.rev_pass_end; ++k)
+        f += MfmaFlopsOfRange(g.ops, g.passes[size_t(k)].op_begin, g.passes[size_t(k)].op_end, A, true);
+    } else {
+      f += MfmaFlopsOfRange(g.ops, c.fwd_begin, c.fwd_end, A, false);
+      f += MfmaFlopsOfRange(g.ops, c.rev_begin, c.rev_end, A, true);
+    }
+    for (int k = c.other_begin; k < c.other_end; ++k)
+      f += MfmaFlopsOfRange(g.ops, g.others[size_t(k)].op_begin, g.others[size_t(k)].op_end, A, false);
+  }
+  return f;
+}
+
 BatchArgs MakeArgs(tfqx_plan* p, void* ws, void* output) {
   uint8_t* base = static_cast<uint8_t*>(ws);
   BatchArgs a{};
@@ -619,7 +655,7 @@ int tfqx_plan_rebind(tfqx_plan* p, const float* values, char* err, int64_t len) 
 
 int tfqx_plan_stats(const tfqx_plan* p, int64_t* out, int64_t n) {
   if (!p || !out) return TFQX_INVALID_ARGUMENT;
-  int64_t v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  int64_t v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   v[3] = p->max_nq;
   if (p->large) {
     const LargePlan& lp = *p->large_plan;
@@ -629,12 +665,17 @@ int tfqx_plan_stats(const tfqx_plan* p, int64_t* out, int64_t n) {
       for (int i = ps.op_begin; i < ps.op_end; ++i) k += lp.ops[size_t(i)].type != kOpRemap;
       return k;
     };
+    double mf = 0;
     for (const auto& c : lp.circuits) {
       v[2] += int64_t(c.fwd.passes.size());
       v[5] += int64_t(c.rev.passes.size());
       for (const auto& ps : c.fwd.passes) v[1] += real_ops(ps);
       for (const auto& ps : c.rev.passes) v[4] += real_ops(ps);
+      const double A = std::ldexp(1.0, c.n);
+      for (const auto& ps : c.fwd.passes) mf += MfmaFlopsOfRange(lp.ops, ps.op_begin, ps.op_end, A, false);
+      for (const auto& ps : c.rev.passes) mf += MfmaFlopsOfRange(lp.ops, ps.op_begin, ps.op_end, A, true);
     }
+    v[9] = int64_t(mf);
     v[6] = 1;
   } else {
     v[0] = int64_t(p->prog.circuits.size());
@@ -644,9 +685,10 @@ int tfqx_plan_stats(const tfqx_plan* p, int64_t* out, int64_t n) {
       for (int i = c.rev_begin; i < c.rev_end; ++i) v[4] += ops[size_t(i)].type != kOpRemap;
     }
     v[8] = int64_t(AlgorithmicFlops(p->prog));
+    v[9] = int64_t(MfmaFlops(p->prog));
   }
   v[7] = int64_t(p->total);
-  for (int64_t i = 0; i < n && i < 9; ++i) out[i] = v[i];
+  for (int64_t i = 0; i < n && i < 10; ++i) out[i] = v[i];
   return TFQX_OK;
 }
 

@@ -17,7 +17,11 @@ LIB = os.path.join(HERE, "libtfqx.so")
 ARCH = os.environ.get("TFQX_OFFLOAD_ARCH", "gfx950")
 
 HOST_SOURCES = ["wire.cc", "circuit.cc", "plan.cc", "large.cc", "api.cc", "surrogate_api.cc"]
-HIP_SOURCES = ["batch_kernels.hip", "large_kernels.hip", "surrogate_kernels.hip"]
+# The batch kernel's modes are separate translation units so they compile in
+# parallel (each instantiates the register engine many times).
+HIP_SOURCES = ["batch_mode_adjoint.hip", "batch_mode_expectation.hip", "batch_mode_other.hip",
+               "batch_mode_ipgrad.hip", "batch_kernels.hip", "large_kernels.hip",
+               "surrogate_kernels.hip"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
           # Packed FP32 (v_pk_*_f32) stays off for every device code object.
@@ -53,7 +57,8 @@ def _compile(src):
 
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
-    sources = HOST_SOURCES + HIP_SOURCES
+    sources = HIP_SOURCES + HOST_SOURCES
+    # Longest compiles first (the batch modes), so they start together.
     with cf.ThreadPoolExecutor(max_workers=min(8, len(sources))) as ex:
         results = list(ex.map(_compile, sources))
     if verbose:

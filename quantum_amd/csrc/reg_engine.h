@@ -353,6 +353,98 @@ __device__ __forceinline__ void reg_diag(cf (&a)[1 << R], cf (&b)[1 << R], const
   }
 }
 
+// Uncontrolled diagonal ops with the position of each target resolved at
+// compile time: P0 / P1 is the register slot of q[0] / q[1], or R when that
+// target is a thread bit (fixed for the whole thread).  The phase of every
+// register is then a compile-time choice among at most two per-thread values,
+// so each amplitude costs one complex multiply.
+template <int R, bool DAG, bool BOTH, bool TWO, int P0, int P1>
+__device__ __forceinline__ void reg_diag_p(cf (&a)[1 << R], cf (&b)[1 << R], const DevOp& op,
+                                           const float2* __restrict__ Dg, const RegMap<R>& m) {
+  cf d[4];
+  d[0] = ldm(Dg, 0);
+  d[1] = ldm(Dg, 1);
+  d[2] = TWO ? ldm(Dg, 2) : d[0];
+  d[3] = TWO ? ldm(Dg, 3) : d[1];
+  if (DAG) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) d[k] = cconj(d[k]);
+  }
+  // Fold the thread-bit targets in: e[c] is the phase for register-bit
+  // pattern c (bit 0: q0, bit 1: q1) of this thread.
+  const bool t0 = P0 == R && ((m.tbase >> op.q[0]) & 1u);
+  const bool t1 = TWO && P1 == R && ((m.tbase >> op.q[1]) & 1u);
+  cf e[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int b0 = P0 == R ? -1 : (c & 1);
+    const int b1 = (!TWO) ? 0 : (P1 == R ? -1 : (c >> 1));
+    // Candidates for the thread bits, chosen per thread.
+    const cf x0 = d[(b1 < 0 ? 0 : b1) * 2 + (b0 < 0 ? 0 : b0)];
+    const cf x1 = d[(b1 < 0 ? 0 : b1) * 2 + (b0 < 0 ? 1 : b0)];
+    const cf y0 = d[(b1 < 0 ? 1 : b1) * 2 + (b0 < 0 ? 0 : b0)];
+    const cf y1 = d[(b1 < 0 ? 1 : b1) * 2 + (b0 < 0 ? 1 : b0)];
+    const cf lo = (b0 < 0 && t0) ? x1 : x0;
+    const cf hi = (b0 < 0 && t0) ? y1 : y0;
+    e[c] = (b1 < 0 && t1) ? hi : lo;
+  }
+#pragma unroll
+  for (int i = 0; i < (1 << R); ++i) {
+    const int c = (P0 == R ? 0 : ((i >> P0) & 1)) | (TWO && P1 != R ? (((i >> P1) & 1) << 1) : 0);
+    a[i] = cmulr(e[c], a[i]);
+    if (BOTH) b[i] = cmulr(e[c], b[i]);
+  }
+}
+
+template <int R, bool DAG, bool BOTH, int P0 = 0, int P1 = 0>
+__device__ __forceinline__ void diag_dispatch2(int p0, int p1, cf (&a)[1 << R], cf (&b)[1 << R],
+                                               const DevOp& op, const float2* Dg,
+                                               const RegMap<R>& m) {
+  if constexpr (P0 <= R) {
+    if constexpr (P1 <= R) {
+      // Two register targets sit in ascending slots (q[0] < q[1]); other
+      // pairs cannot occur and get no code.
+      if constexpr (P0 == R || P1 == R || P0 < P1) {
+        if (p0 == P0 && p1 == P1) {
+          reg_diag_p<R, DAG, BOTH, true, P0, P1>(a, b, op, Dg, m);
+          return;
+        }
+      }
+      diag_dispatch2<R, DAG, BOTH, P0, P1 + 1>(p0, p1, a, b, op, Dg, m);
+    } else {
+      diag_dispatch2<R, DAG, BOTH, P0 + 1, 0>(p0, p1, a, b, op, Dg, m);
+    }
+  }
+}
+
+template <int R, bool DAG, bool BOTH, int P0 = 0>
+__device__ __forceinline__ void diag_dispatch1(int p0, cf (&a)[1 << R], cf (&b)[1 << R],
+                                               const DevOp& op, const float2* Dg,
+                                               const RegMap<R>& m) {
+  if constexpr (P0 <= R) {
+    if (p0 == P0) reg_diag_p<R, DAG, BOTH, false, P0, R>(a, b, op, Dg, m);
+    else diag_dispatch1<R, DAG, BOTH, P0 + 1>(p0, a, b, op, Dg, m);
+  }
+}
+
+// Diagonal op: position-resolved variants without controls, the generic
+// per-amplitude selection with them.
+template <int R, bool DAG, bool BOTH>
+__device__ __forceinline__ void reg_diag_any(cf (&a)[1 << R], cf (&b)[1 << R], const DevOp& op,
+                                             const float2* __restrict__ Dg, const RegMap<R>& m) {
+  if (op.cmask) {
+    reg_diag<R, DAG, BOTH>(a, b, op, Dg, m);
+    return;
+  }
+  const int p0 = ((m.mask >> op.q[0]) & 1u) ? reg_slot(m, op.q[0]) : R;
+  if (op.k == 2) {
+    const int p1 = ((m.mask >> op.q[1]) & 1u) ? reg_slot(m, op.q[1]) : R;
+    diag_dispatch2<R, DAG, BOTH>(p0, p1, a, b, op, Dg, m);
+  } else {
+    diag_dispatch1<R, DAG, BOTH>(p0, a, b, op, Dg, m);
+  }
+}
+
 // ------------------------------------------------------------------ parameterised ops
 // psi <- G^dag psi;  acc_d += <dG_d psi | lam> (real part in acc[d], imaginary
 // part in acc[kMaxDeriv + d]);  lam <- G^dag lam  (see apply_param).
@@ -641,7 +733,7 @@ __device__ __forceinline__ void reg_run(float2* __restrict__ A, float2* __restri
       continue;
     }
     if (op.type == kOpDiag) {
-      reg_diag<R, REV, REV>(ra, rb, op, mats + op.mat, m);
+      reg_diag_any<R, REV, REV>(ra, rb, op, mats + op.mat, m);
     } else if (REV && op.type == kOpParam) {
       double acc[2 * kMaxDeriv] = {0.0, 0.0, 0.0, 0.0};
       if (op.k == 1) param_dispatch1<R>(reg_slot(m, op.q[0]), ra, rb, op, mats, m, acc);

@@ -118,9 +118,10 @@ int tfqx_plan_inner_product(
  * :51-208, adjoint sweep :259-314 and :380-444): out[i,s] = sum over gates using s of
  * <dG psi_k | lambda_k>, lambda = sum_j prev_grad[i,j] |phi_ij>
  * (AccumulateFusedCircuits, util_qsim.h:423-442); no conjugation (the Python
- * wrapper conjugates, inner_product_op.py:65-68).  S must be positive;
- * circuits of at most 13 qubits (TFQX_UNIMPLEMENTED beyond).  derivative as
- * for tfqx_plan_adjoint_gradient. */
+ * wrapper conjugates, inner_product_op.py:65-68).  S must be positive.
+ * Circuits of up to 13 qubits run in the batched kernel; wider ones run one
+ * after another through streaming passes.  derivative as for
+ * tfqx_plan_adjoint_gradient. */
 int tfqx_plan_inner_product_grad(
     const uint8_t* programs, const int64_t* program_offsets, int64_t batch,
     const uint8_t* symbol_names, const int64_t* symbol_offsets, int64_t num_symbols,

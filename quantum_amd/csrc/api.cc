@@ -283,10 +283,6 @@ int BuildPlan(int mode, Inputs& in, int64_t B, int64_t S, int64_t O, int64_t K,
                         p->max_nq <= kTiledMaxQubits;
   opt.tile_log2 = resident;
   opt.wave = mode == kModeExpectation || mode == kModeAdjoint;
-  if (mode == kModeInnerProductGrad && p->max_nq > resident)
-    return Fail(err, len, TFQX_UNIMPLEMENTED,
-                "tfq_inner_product_grad: circuits of at most " + std::to_string(resident) +
-                    " qubits are supported, got " + std::to_string(p->max_nq) + ".");
   if (p->max_nq > resident && !tiled_ok) {
     p->large = true;
     Status s = BuildLargePlan(mode, in.circuits, in.sums, downstream, in.others, B, S, O, K,

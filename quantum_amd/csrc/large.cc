@@ -231,8 +231,8 @@ Status BuildLargePlan(int mode, const std::vector<Circuit>& circuits,
     lc.empty = c.gates.empty();
     lc.ident.assign(size_t(mode == kModeInnerProduct ? K : O), 0.0f);
     if (lc.empty) continue;
-    const bool two = mode == kModeAdjoint;
-    lc.L = std::min(lc.n, TileQubits(mode, two));
+    const bool grad = mode == kModeAdjoint || mode == kModeInnerProductGrad;
+    lc.L = std::min(lc.n, TileQubits(mode, grad));
     std::vector<FusedOp> fused;
     FuseGates(c, 0, int(c.gates.size()), fo, &fused);
     AddPasses(lp.get(), fused, lc.n, std::min(lc.n, TileQubits(mode, false)), &lc.fwd);
@@ -274,6 +274,8 @@ Status BuildLargePlan(int mode, const std::vector<Circuit>& circuits,
         }
       }
       lc.lam_end = int(lp->terms.size());
+    }
+    if (grad) {
       // Reverse program (as the LDS planner), param ops numbered by partial slot.
       std::vector<int> pg;
       for (int g = 0; g < int(c.gates.size()); ++g)
@@ -305,18 +307,26 @@ Status BuildLargePlan(int mode, const std::vector<Circuit>& circuits,
       const size_t op0 = lp->ops.size();
       AddPasses(lp.get(), rev, lc.n, lc.L, &lc.rev);
       // Number gradient partial slots in pass order so each pass owns a
-      // contiguous slot range; param_cols maps slot -> symbol column.
+      // contiguous slot range; param_cols maps slot -> accumulator index.
+      // Inner-product gradients take two slots per derivative (real part,
+      // then imaginary part) accumulating into gacc[2 col] and gacc[2 col + 1].
+      const bool cplx = mode == kModeInnerProductGrad;
       for (size_t i = op0; i < lp->ops.size(); ++i) {
         DevOp& d = lp->ops[i];
         if (d.type != kOpParam) continue;
         for (int k = 0; k < d.nderiv; ++k) {
           const int col = d.col[k];
           d.col[k] = int(lp->param_cols.size());
-          lp->param_cols.push_back(col);
+          if (cplx) {
+            lp->param_cols.push_back(2 * col);
+            lp->param_cols.push_back(2 * col + 1);
+          } else {
+            lp->param_cols.push_back(col);
+          }
         }
       }
     }
-    if (mode == kModeInnerProduct) {
+    if (mode == kModeInnerProduct || mode == kModeInnerProductGrad) {
       for (int64_t k = 0; k < K; ++k) {
         const Circuit& oc = others[i][k];
         std::vector<FusedOp> of;
@@ -324,6 +334,7 @@ Status BuildLargePlan(int mode, const std::vector<Circuit>& circuits,
         PassPlan pp;
         AddPasses(lp.get(), of, lc.n, std::min(lc.n, TileQubits(mode, false)), &pp);
         lc.others.push_back(pp);
+        lc.other_weights.push_back(mode == kModeInnerProductGrad ? downstream[i * K + k] : 0.0f);
       }
     }
   }

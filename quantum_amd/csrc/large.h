@@ -33,6 +33,7 @@ struct LargeCircuit {
   PassPlan fwd;                       // forward passes
   PassPlan rev;                       // adjoint reverse passes (two buffers)
   std::vector<PassPlan> others;       // inner product paired programs
+  std::vector<float> other_weights;   // inner-product gradient: g[row, k] of each paired program
   int term_begin = 0, term_end = 0;   // DevTermWide ranges in the plan's term table
   int lam_begin = 0, lam_end = 0;
   std::vector<int> rot_groups;        // sampled expectation: term ranges per rotation pattern

@@ -45,11 +45,15 @@ struct PassArgs {
 
 // One-state passes use 64 KiB of LDS for 13-qubit tiles, so two workgroups
 // share a CU when each wave keeps to 128 VGPRs (4 waves per SIMD).
-template <bool TWO>
+// CPLX (two-buffer passes only): each derivative also accumulates the
+// imaginary part of <dG psi|lambda>, written to the partial slot after the
+// real part's.
+template <bool TWO, bool CPLX = false>
 __global__ __launch_bounds__(kThreads, TWO ? 2 : 4) void pass_kernel(PassArgs a) {
+  constexpr int kPacc = TWO ? (CPLX ? 2 : 1) * kMaxPassDerivs : 1;
   extern __shared__ float2 lds[];
   __shared__ double red[kWaves];
-  __shared__ double pacc[kMaxPassDerivs];
+  __shared__ double pacc[kPacc];
   __shared__ float2 mbuf[2 * kMatSlot];
   __shared__ uint64_t tab[kTileTable];
   __shared__ float2 mcache[kMatCache];
@@ -65,7 +69,7 @@ __global__ __launch_bounds__(kThreads, TWO ? 2 : 4) void pass_kernel(PassArgs a)
   const int nlow = ps.nlow;
   build_tile_table(ps, tab);
   if (TWO)
-    for (int k = tid; k < kMaxPassDerivs; k += kThreads) pacc[k] = 0.0;
+    for (int k = tid; k < kPacc; k += kThreads) pacc[k] = 0.0;
   __syncthreads();
   for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const uint64_t base = tile_base(t, ps.tilemask, a.n);
@@ -73,7 +77,10 @@ __global__ __launch_bounds__(kThreads, TWO ? 2 : 4) void pass_kernel(PassArgs a)
       return use_tab ? tile_index_tab(base, j, nlow, tab) : tile_index(base, j, ps);
     };
     int pl = 0;
-    auto acc = [](int slot, int, double w, double) { atomicAdd(&pacc[slot], w); };
+    auto acc = [](int slot, int, double w, double v) {
+      atomicAdd(&pacc[slot], w);
+      if (CPLX) atomicAdd(&pacc[kMaxPassDerivs + slot], v);
+    };
     // Register engine: the tile goes straight between memory and registers.
     auto io = tile_io(
         [&](uint32_t x, cf& va, cf& vb) {
@@ -120,6 +127,10 @@ __global__ __launch_bounds__(kThreads, TWO ? 2 : 4) void pass_kernel(PassArgs a)
                   if (d >= op.nderiv) break;
                   const double tot = block_sum(acc[d], red);
                   if (tid == 0) pacc[pl + d] += tot;
+                  if (CPLX) {
+                    const double im = block_sum(acc[kMaxDeriv + d], red);
+                    if (tid == 0) pacc[kMaxPassDerivs + pl + d] += im;
+                  }
                 }
                 pl += op.nderiv;
               } else {
@@ -143,8 +154,11 @@ __global__ __launch_bounds__(kThreads, TWO ? 2 : 4) void pass_kernel(PassArgs a)
     for (int i = ps.op_begin; i < ps.op_end; ++i) {
       const DevOp op = load_op(a.ops + i);
       if (op.type != kOpParam) continue;
-      a.partials[size_t(op.col[0]) * gridDim.x + blockIdx.x] = pacc[pl];
-      if (op.nderiv > 1) a.partials[size_t(op.col[1]) * gridDim.x + blockIdx.x] = pacc[pl + 1];
+      for (int d = 0; d < op.nderiv; ++d) {
+        a.partials[size_t(op.col[d]) * gridDim.x + blockIdx.x] = pacc[pl + d];
+        if (CPLX)
+          a.partials[size_t(op.col[d] + 1) * gridDim.x + blockIdx.x] = pacc[kMaxPassDerivs + pl + d];
+      }
       pl += op.nderiv;
     }
   }
@@ -240,9 +254,21 @@ __global__ void finish_ip(const double* sums, float2* out) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *out = make_float2(float(sums[0]), float(sums[1]));
 }
 
-__global__ void accumulate_grads(const double* sums, const int* cols, int count, double* gacc) {
+__global__ void accumulate_grads(const double* sums, const int* cols, int count, double scale,
+                                 double* gacc) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  for (int i = 0; i < count; ++i) gacc[cols[i]] += 2.0 * sums[i];
+  for (int i = 0; i < count; ++i) gacc[cols[i]] += scale * sums[i];
+}
+
+// dst = w src (first) or dst += w src: one term of lambda = sum_j g_j |phi_j>.
+__global__ void axpy_c64(const float2* __restrict__ src, float2* __restrict__ dst, int64_t count,
+                         float w, int first) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const float2 v = src[i];
+    const float2 d = first ? make_float2(0.f, 0.f) : dst[i];
+    dst[i] = make_float2(d.x + w * v.x, d.y + w * v.y);
+  }
 }
 
 __global__ void fill_f64(double* p, int64_t n, double v) {
@@ -428,11 +454,12 @@ static void LargeLayout(LargePlan* p, size_t* off_pcols, size_t* off_ident, size
   p->partial_slots = slots;
   p->off_partials = off; off = AlignUp(off + slots * size_t(grid) * sizeof(double));
   *off_red = off; off = AlignUp(off + std::max<size_t>(slots, p->terms.size() + 8) * sizeof(double));
-  p->off_gacc = off; off = AlignUp(off + size_t(std::max<int64_t>(p->S, 1)) * sizeof(double));
+  const size_t gacc_n = size_t(std::max<int64_t>(p->S, 1)) * (p->mode == kModeInnerProductGrad ? 2 : 1);
+  p->off_gacc = off; off = AlignUp(off + gacc_n * sizeof(double));
   const size_t amps = size_t(1) << p->max_nq;
   p->off_psi = off; off = AlignUp(off + amps * sizeof(float2));
   const bool two = p->mode == kModeAdjoint || p->mode == kModeInnerProduct ||
-                   p->mode == kModeSampledExpectation;
+                   p->mode == kModeSampledExpectation || p->mode == kModeInnerProductGrad;
   p->off_buf2 = off;
   if (two) off = AlignUp(off + amps * sizeof(float2));
   p->off_cdf = off;
@@ -562,6 +589,11 @@ int LargeLaunch(LargePlan* p, void* ws, void* output, hipStream_t stream, char* 
                              static_cast<float2*>(output) + size_t(row) * p->K, int64_t(p->K),
                              make_float2(1.f, 0.f), make_float2(1.f, 0.f));
           break;
+        case kModeInnerProductGrad:
+          hipLaunchKernelGGL(fill_row_c64, dim3(1), dim3(256), 0, stream,
+                             static_cast<float2*>(output) + size_t(row) * p->S, int64_t(p->S),
+                             make_float2(0.f, 0.f), make_float2(0.f, 0.f));
+          break;
         case kModeState:
           hipLaunchKernelGGL(fill_row_c64, dim3(grid), dim3(256), 0, stream,
                              static_cast<float2*>(output) + size_t(row) * amps_max, int64_t(amps_max),
@@ -576,6 +608,20 @@ int LargeLaunch(LargePlan* p, void* ws, void* output, hipStream_t stream, char* 
       continue;
     }
     const int sgrid = int(std::min<uint64_t>(uint64_t(grid), ((uint64_t(1) << n) + 255) / 256));
+    if (p->mode == kModeInnerProductGrad) {
+      // lambda = sum_j g_j |phi_j> in buf2 (AccumulateFusedCircuits,
+      // util_qsim.h:421-442), each phi_j simulated in psi's buffer first.
+      const int64_t amps = int64_t(1) << n;
+      const int agrid = int(std::min<int64_t>(grid, (amps + 255) / 256));
+      for (size_t k = 0; k < lc.others.size(); ++k) {
+        run_passes(lc.others[k], n, psi, true);
+        hipLaunchKernelGGL(axpy_c64, dim3(agrid), dim3(256), 0, stream, psi, buf2, amps,
+                           lc.other_weights[k], k == 0 ? 1 : 0);
+      }
+      if (lc.others.empty())
+        hipLaunchKernelGGL(fill_row_c64, dim3(agrid), dim3(256), 0, stream, buf2, amps,
+                           make_float2(0.f, 0.f), make_float2(0.f, 0.f));
+    }
     run_passes(lc.fwd, n, psi, true);
     const float* ident = reinterpret_cast<const float*>(base + o.ident) + size_t(row) * nslots;
     if (p->mode == kModeExpectation || (p->mode == kModeAdjoint && p->with_expectation)) {
@@ -595,12 +641,18 @@ int LargeLaunch(LargePlan* p, void* ws, void* output, hipStream_t stream, char* 
         hipLaunchKernelGGL(finish_expect, dim3(1), dim3(64), 0, stream, redbuf, lc.ident[j], out);
       }
     }
-    if (p->mode == kModeAdjoint) {
+    if (p->mode == kModeAdjoint || p->mode == kModeInnerProductGrad) {
+      // Adjoint sweep.  Inner-product gradients accumulate the complex
+      // <dG psi|lambda> (tfq_inner_product_grad.cc:259-314) into gacc[2 S];
+      // expectation gradients 2 Re<dG psi|lambda> into gacc[S].
+      const bool cplx = p->mode == kModeInnerProductGrad;
+      const int width = cplx ? 2 : 1;
       float* grad_base = static_cast<float*>(output) +
                          (p->with_expectation ? size_t(p->B) * size_t(p->O) : 0);
-      hipLaunchKernelGGL(lambda_kernel, dim3(sgrid), dim3(kThreads), 0, stream, psi, buf2, n, terms,
-                         lc.lam_begin, lc.lam_end);
-      hipLaunchKernelGGL(fill_f64, dim3(1), dim3(256), 0, stream, gacc, int64_t(p->S), 0.0);
+      if (!cplx)
+        hipLaunchKernelGGL(lambda_kernel, dim3(sgrid), dim3(kThreads), 0, stream, psi, buf2, n, terms,
+                           lc.lam_begin, lc.lam_end);
+      hipLaunchKernelGGL(fill_f64, dim3(1), dim3(256), 0, stream, gacc, int64_t(p->S) * width, 0.0);
       for (const DevPass& ps : lc.rev.passes) {
         PassArgs a{};
         a.psi = psi;
@@ -613,7 +665,8 @@ int LargeLaunch(LargePlan* p, void* ws, void* output, hipStream_t stream, char* 
         const uint64_t ntiles = uint64_t(1) << (n - ps.L);
         const int g = int(std::min<uint64_t>(ntiles, uint64_t(grid)));
         const size_t lds = 2 * (size_t(1) << ps.L) * sizeof(float2);
-        hipLaunchKernelGGL(pass_kernel<true>, dim3(g), dim3(kThreads), lds, stream, a);
+        if (cplx) hipLaunchKernelGGL((pass_kernel<true, true>), dim3(g), dim3(kThreads), lds, stream, a);
+        else hipLaunchKernelGGL((pass_kernel<true, false>), dim3(g), dim3(kThreads), lds, stream, a);
         // Partial slots of a pass are contiguous [s0, s0 + nd) (numbered at planning).
         int s0 = -1, nd = 0;
         for (int i = ps.op_begin; i < ps.op_end; ++i) {
@@ -624,13 +677,15 @@ int LargeLaunch(LargePlan* p, void* ws, void* output, hipStream_t stream, char* 
         }
         if (nd == 0) continue;
         // partials[slot * g + wg]: row r = slot - s0 -> base + r * g.
-        hipLaunchKernelGGL(reduce_kernel, dim3(nd), dim3(kThreads), 0, stream,
+        const int rows = nd * width;
+        hipLaunchKernelGGL(reduce_kernel, dim3(rows), dim3(kThreads), 0, stream,
                            partials + size_t(s0) * g, g, 1, g, redbuf);
-        hipLaunchKernelGGL(accumulate_grads, dim3(1), dim3(64), 0, stream, redbuf, pcols + s0, nd,
-                           gacc);
+        hipLaunchKernelGGL(accumulate_grads, dim3(1), dim3(64), 0, stream, redbuf, pcols + s0, rows,
+                           cplx ? 1.0 : 2.0, gacc);
       }
-      hipLaunchKernelGGL(grads_to_float, dim3(std::max<int>(1, int((p->S + 255) / 256))), dim3(256), 0,
-                         stream, gacc, int(p->S), grad_base + size_t(row) * p->S);
+      const int64_t cnt = p->S * width;
+      hipLaunchKernelGGL(grads_to_float, dim3(std::max<int>(1, int((cnt + 255) / 256))), dim3(256), 0,
+                         stream, gacc, int(cnt), grad_base + size_t(row) * cnt);
     } else if (p->mode == kModeState) {
       hipLaunchKernelGGL(copy_state, dim3(grid), dim3(256), 0, stream, psi, n,
                          static_cast<float2*>(output) + size_t(row) * amps_max, int64_t(amps_max));

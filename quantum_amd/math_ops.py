@@ -115,7 +115,6 @@ def inner_product_grad(programs, symbol_names, symbol_values, other_programs, pr
 
     derivative "tfq": TFQ's float32 central-difference gate derivative
     (epsilon 5e-3, ``adj_util.cc:175-302``); "analytic": the exact one.
-    Circuits of at most 13 qubits.
     """
     programs, names, values = _common(programs, symbol_names, symbol_values)
     other = _checked_others(programs, other_programs)

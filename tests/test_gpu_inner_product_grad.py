@@ -5,6 +5,8 @@
   TFQ's finite-difference gate derivative and the exact one, 1-3 paired
   programs per row, controls included; empty programs and symbols absent from
   a circuit give zeros.
+* Circuits wider than 13 qubits (the streaming passes): 14 and 16 qubits, a
+  batch mixing widths, and an empty program in such a batch.
 * Input checks with the reference's messages (``inner_product_grad_test.py``).
 * Autograd: ``fidelity`` and ``inner_product`` losses back-propagate to the
   symbol values as the oracle's central differences say.
@@ -87,6 +89,43 @@ def test_inner_product_grad_empty_program_and_unused_symbol():
     assert abs(out[1, 0]) > 1e-3
 
 
+@pytest.mark.parametrize("n,K", [(14, 2), (16, 1)])
+@pytest.mark.parametrize("mode", ["tfq", "analytic"])
+def test_inner_product_grad_wide_matches_oracle(n, K, mode):
+    from quantum_amd import math_ops as M
+    progs, names, vals, oth, g = _case(600 + n, n, 2, K, depth=2 * n)
+    _say(f"inner product grad (streaming) n={n} K={K} {mode}")
+    out = M.inner_product_grad(progs, names, vals, oth, g, derivative=mode).cpu().numpy()
+    ref = np.conj(O.inner_product_gradient(list(progs), names, vals, oth, g, mode=mode))
+    assert np.abs(ref).max() > 1e-2
+    np.testing.assert_allclose(out, ref, atol=TOL * max(1, np.abs(ref).max()))
+
+
+def test_inner_product_grad_wide_mixed_batch_with_empty_program():
+    from quantum_amd import math_ops as M
+    rng = np.random.default_rng(88)
+    names = ["a", "b"]
+    circuits, others = [], []
+    for n in (4, 15, 0, 9):
+        if n == 0:
+            circuits.append(C.Circuit())
+            others.append([C.Circuit(), C.Circuit()])
+            continue
+        q = C.LineQubit.range(n)
+        ident = C.Circuit([C.I(x) for x in q])
+        circuits.append(W.random_circuit(rng, q, 2 * n, symbols=names, p_control=0.2) + ident)
+        others.append([W.random_circuit(rng, q, n) + ident for _ in range(2)])
+    progs = S.convert_to_tensor(circuits)
+    oth = np.array([[S.serialize_circuit(c) for c in row] for row in others], dtype=object)
+    vals = rng.uniform(-2, 2, size=(4, 2)).astype(np.float32)
+    g = rng.normal(size=(4, 2)).astype(np.float32)
+    _say("inner product grad (streaming): mixed widths, empty program")
+    out = M.inner_product_grad(progs, names, vals, oth, g).cpu().numpy()
+    ref = np.conj(O.inner_product_gradient(list(progs), names, vals, oth, g))
+    np.testing.assert_allclose(out[2], 0.0)
+    np.testing.assert_allclose(out, ref, atol=TOL * max(1, np.abs(ref).max()))
+
+
 def test_inner_product_grad_input_checks():
     from quantum_amd import math_ops as M
     from quantum_amd._lib import InvalidArgumentError
@@ -100,9 +139,6 @@ def test_inner_product_grad_input_checks():
     sym_others = np.array([[p, p] for p in progs], dtype=object)
     with pytest.raises(InvalidArgumentError, match="Found symbols in other_programs"):
         M.inner_product_grad(progs, names, vals, sym_others, g)
-    big, names14, vals14, oth14, g14 = _case(78, 14, 1, 1, depth=20)
-    with pytest.raises(NotImplementedError, match="at most 13 qubits"):
-        M.inner_product_grad(big, names14, vals14, oth14, g14)
 
 
 def test_fidelity_and_inner_product_autograd():

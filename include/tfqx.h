@@ -95,6 +95,49 @@ int tfqx_plan_simulate_samples(
     const float* symbol_values, int32_t num_samples, uint64_t seed,
     tfqx_plan** plan, char* err, int64_t err_len);
 
+/* TfqNoisyExpectation(programs[B], symbol_names[S], symbol_values[B,S],
+ * pauli_sums[B,O], num_samples int32[B,O]) -> float[B,O].
+ * Replaces core/ops/noise/tfq_noisy_expectation.cc:57-283: term j of row i
+ * averages its exact expectation over num_samples[i][j] Monte-Carlo
+ * trajectories (channels DP ADP GAD AD RST PD PF BF,
+ * circuit_parser_qsim.cc:598-870); -2 for empty programs.  Channel c of
+ * trajectory t of row i draws Philox4x32-10 uniform (seed, shot t, stream
+ * (i, c)); the reference seeds from tensorflow::random::New64().  Circuits of
+ * at most 13 qubits (TFQX_UNIMPLEMENTED beyond). */
+int tfqx_plan_noisy_expectation(
+    const uint8_t* programs, const int64_t* program_offsets, int64_t batch,
+    const uint8_t* symbol_names, const int64_t* symbol_offsets, int64_t num_symbols,
+    const float* symbol_values,
+    const uint8_t* pauli_sums, const int64_t* pauli_offsets, int64_t num_ops,
+    const int32_t* num_samples, uint64_t seed,
+    tfqx_plan** plan, char* err, int64_t err_len);
+
+/* TfqNoisySampledExpectation(..., num_samples int32[B,O]) -> float[B,O].
+ * Replaces core/ops/noise/tfq_noisy_sampled_expectation.cc: per trajectory,
+ * each Pauli term contributes one measured parity (+-1) times its
+ * coefficient (ComputeSampledExpectationQsim with one sample,
+ * util_qsim.h:199-270).  The parity is drawn as a Bernoulli variate with
+ * P(+1) = (1 + <P>/<psi|psi>) / 2, the distribution of one measurement of the
+ * rotated state, from stream (i, 0x80000000 | term index). */
+int tfqx_plan_noisy_sampled_expectation(
+    const uint8_t* programs, const int64_t* program_offsets, int64_t batch,
+    const uint8_t* symbol_names, const int64_t* symbol_offsets, int64_t num_symbols,
+    const float* symbol_values,
+    const uint8_t* pauli_sums, const int64_t* pauli_offsets, int64_t num_ops,
+    const int32_t* num_samples, uint64_t seed,
+    tfqx_plan** plan, char* err, int64_t err_len);
+
+/* TfqNoisySamples(programs[B], symbol_names[S], symbol_values[B,S],
+ * num_samples int32[1]) -> int8[B, num_samples, max_nq].
+ * Replaces core/ops/noise/tfq_noisy_samples.cc:48-205: sample t of row i is
+ * the terminal measurement of trajectory t (stream (i, 0xFFFFFFFF)); bit q
+ * goes to column max_nq-1-q, leading -2 padding; empty programs give -2. */
+int tfqx_plan_noisy_samples(
+    const uint8_t* programs, const int64_t* program_offsets, int64_t batch,
+    const uint8_t* symbol_names, const int64_t* symbol_offsets, int64_t num_symbols,
+    const float* symbol_values, int32_t num_samples, uint64_t seed,
+    tfqx_plan** plan, char* err, int64_t err_len);
+
 /* TfqSimulateState(programs[B], symbol_names[S], symbol_values[B,S])
  * -> complex64[B, 2^max_nq]. */
 int tfqx_plan_simulate_state(
@@ -180,7 +223,9 @@ int tfqx_debug_fuse(const uint8_t* program, int64_t program_len, const uint8_t* 
  * [3] max qubits, [4] adjoint ops, [5] adjoint HBM passes, [6] 1 if the
  * HBM-streaming path is used, [7] workspace bytes, [8] algorithmic real flops
  * of one launch (batch path; 0 on the streaming path), [9] the part of those
- * flops that runs on FP32 MFMA (both paths).  No reference counterpart. */
+ * flops that runs on FP32 MFMA (both paths).  Noisy plans: [0] non-empty
+ * circuits, [1] ops (gates and channels), [2] trajectories per launch.  No
+ * reference counterpart. */
 int tfqx_plan_stats(const tfqx_plan* plan, int64_t* out, int64_t n);
 
 /* Largest qubit count in the batch (sizes the samples / state outputs). */

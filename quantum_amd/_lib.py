@@ -46,6 +46,9 @@ def _declare(lib):
     lib.tfqx_plan_simulate_sampled_expectation.argtypes = s + [P, P, i64, P, u64, P, c.c_char_p, i64]
     lib.tfqx_plan_simulate_samples.argtypes = s + [i32, u64, P, c.c_char_p, i64]
     lib.tfqx_plan_simulate_state.argtypes = s + [P, c.c_char_p, i64]
+    lib.tfqx_plan_noisy_expectation.argtypes = s + [P, P, i64, P, u64, P, c.c_char_p, i64]
+    lib.tfqx_plan_noisy_sampled_expectation.argtypes = s + [P, P, i64, P, u64, P, c.c_char_p, i64]
+    lib.tfqx_plan_noisy_samples.argtypes = s + [i32, u64, P, c.c_char_p, i64]
     lib.tfqx_plan_inner_product.argtypes = s + [P, P, i64, P, c.c_char_p, i64]
     lib.tfqx_plan_inner_product_grad.argtypes = s + [P, P, i64, P, i32, P, c.c_char_p, i64]
     lib.tfqx_plan_adjoint_gradient.argtypes = s + [P, P, i64, P, i32, P, c.c_char_p, i64]
@@ -59,6 +62,8 @@ def _declare(lib):
     lib.tfqx_plan_stats.restype = c.c_int
     for name in ("tfqx_plan_simulate_expectation", "tfqx_plan_simulate_sampled_expectation",
                  "tfqx_plan_simulate_samples", "tfqx_plan_simulate_state",
+                 "tfqx_plan_noisy_expectation", "tfqx_plan_noisy_sampled_expectation",
+                 "tfqx_plan_noisy_samples",
                  "tfqx_plan_inner_product", "tfqx_plan_inner_product_grad",
                  "tfqx_plan_adjoint_gradient", "tfqx_plan_expectation_and_gradient", "tfqx_plan_upload", "tfqx_plan_launch", "tfqx_plan_launch_plain",
                  "tfqx_graph_capture", "tfqx_graph_launch"):
@@ -108,7 +113,8 @@ def lib():
 
 EXPORTED_SYMBOLS = (
     "tfqx_plan_simulate_expectation", "tfqx_plan_simulate_sampled_expectation",
-    "tfqx_plan_simulate_samples", "tfqx_plan_simulate_state", "tfqx_plan_inner_product",
+    "tfqx_plan_simulate_samples", "tfqx_plan_simulate_state", "tfqx_plan_noisy_expectation",
+    "tfqx_plan_noisy_sampled_expectation", "tfqx_plan_noisy_samples", "tfqx_plan_inner_product",
     "tfqx_plan_inner_product_grad", "tfqx_plan_adjoint_gradient", "tfqx_plan_expectation_and_gradient", "tfqx_plan_rebind",
     "tfqx_debug_fuse",
     "tfqx_plan_stats",

@@ -16,12 +16,13 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtfqx.so")
 ARCH = os.environ.get("TFQX_OFFLOAD_ARCH", "gfx950")
 
-HOST_SOURCES = ["wire.cc", "circuit.cc", "plan.cc", "large.cc", "api.cc", "surrogate_api.cc"]
+HOST_SOURCES = ["wire.cc", "circuit.cc", "plan.cc", "large.cc", "noisy.cc", "api.cc",
+                "surrogate_api.cc"]
 # The batch kernel's modes are separate translation units so they compile in
 # parallel (each instantiates the register engine many times).
 HIP_SOURCES = ["batch_mode_adjoint.hip", "batch_mode_expectation.hip", "batch_mode_other.hip",
                "batch_mode_ipgrad.hip", "batch_kernels.hip", "large_kernels.hip",
-               "surrogate_kernels.hip"]
+               "surrogate_kernels.hip", "noisy_kernels.hip"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
           # Packed FP32 (v_pk_*_f32) stays off for every device code object.

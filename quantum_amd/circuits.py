@@ -181,6 +181,12 @@ class Gate:
     def __call__(self, *qubits) -> "Operation":
         return self.on(*qubits)
 
+    def on_each(self, *qubits) -> List["Operation"]:
+        """One operation of this single-qubit gate on each qubit."""
+        if self.num_qubits != 1:
+            raise ValueError("on_each needs a single-qubit gate")
+        return [self.on(q) for q in _flatten_qubits(qubits)]
+
     def params(self) -> Dict[str, Param]:
         return {}
 
@@ -311,6 +317,159 @@ class IdentityGate(Gate):
     gate_id = "I"
 
 
+# --------------------------------------------------------------------------- channels
+class Channel(Gate):
+    """A single-qubit noise channel.  Channels carry plain float arguments (no
+    symbols) and serialize under the ids of ``serializer.py:174-455``; only the
+    noisy ops (``tfq.noise``) accept them."""
+
+    def _check(self, **probs):
+        for name, v in probs.items():
+            if not (0.0 <= v <= 1.0):
+                raise ValueError(f"{name} was {v} but must be between 0 and 1.")
+
+
+class DepolarizingChannel(Channel):
+    """rho -> (1 - p) rho + p/3 (X rho X + Y rho Y + Z rho Z) (id ``DP``)."""
+
+    gate_id = "DP"
+
+    def __init__(self, p: float):
+        self.p = float(p)
+        self._check(p=self.p)
+
+    def params(self):
+        return {"p": self.p}
+
+
+class AsymmetricDepolarizingChannel(Channel):
+    """rho -> (1 - px - py - pz) rho + px X rho X + py Y rho Y + pz Z rho Z (id ``ADP``)."""
+
+    gate_id = "ADP"
+
+    def __init__(self, p_x: float = 0.0, p_y: float = 0.0, p_z: float = 0.0):
+        self.p_x, self.p_y, self.p_z = float(p_x), float(p_y), float(p_z)
+        self._check(p_x=self.p_x, p_y=self.p_y, p_z=self.p_z)
+        if self.p_x + self.p_y + self.p_z > 1.0:
+            raise ValueError("Probabilities must sum to at most 1.")
+
+    def params(self):
+        return {"p_x": self.p_x, "p_y": self.p_y, "p_z": self.p_z}
+
+
+class GeneralizedAmplitudeDampingChannel(Channel):
+    """Kraus ops sqrt(p)[[1,0],[0,sqrt(1-g)]], sqrt(p)[[0,sqrt(g)],[0,0]],
+    sqrt(1-p)[[sqrt(1-g),0],[0,1]], sqrt(1-p)[[0,0],[sqrt(g),0]] (id ``GAD``)."""
+
+    gate_id = "GAD"
+
+    def __init__(self, p: float, gamma: float):
+        self.p, self.gamma = float(p), float(gamma)
+        self._check(p=self.p, gamma=self.gamma)
+
+    def params(self):
+        return {"p": self.p, "gamma": self.gamma}
+
+
+class AmplitudeDampingChannel(Channel):
+    """Kraus ops [[1,0],[0,sqrt(1-g)]] and [[0,sqrt(g)],[0,0]] (id ``AD``)."""
+
+    gate_id = "AD"
+
+    def __init__(self, gamma: float):
+        self.gamma = float(gamma)
+        self._check(gamma=self.gamma)
+
+    def params(self):
+        return {"gamma": self.gamma}
+
+
+class ResetChannel(Channel):
+    """Kraus ops |0><0| and |0><1| (id ``RST``)."""
+
+    gate_id = "RST"
+
+
+class PhaseDampingChannel(Channel):
+    """Kraus ops [[1,0],[0,sqrt(1-g)]] and [[0,0],[0,sqrt(g)]] (id ``PD``)."""
+
+    gate_id = "PD"
+
+    def __init__(self, gamma: float):
+        self.gamma = float(gamma)
+        self._check(gamma=self.gamma)
+
+    def params(self):
+        return {"gamma": self.gamma}
+
+
+class PhaseFlipChannel(Channel):
+    """rho -> (1 - p) rho + p Z rho Z (id ``PF``)."""
+
+    gate_id = "PF"
+
+    def __init__(self, p: float):
+        self.p = float(p)
+        self._check(p=self.p)
+
+    def params(self):
+        return {"p": self.p}
+
+
+class BitFlipChannel(Channel):
+    """rho -> (1 - p) rho + p X rho X (id ``BF``)."""
+
+    gate_id = "BF"
+
+    def __init__(self, p: float):
+        self.p = float(p)
+        self._check(p=self.p)
+
+    def params(self):
+        return {"p": self.p}
+
+
+def depolarize(p: float) -> DepolarizingChannel:
+    return DepolarizingChannel(p)
+
+
+def asymmetric_depolarize(p_x: float = 0.0, p_y: float = 0.0,
+                          p_z: float = 0.0) -> AsymmetricDepolarizingChannel:
+    return AsymmetricDepolarizingChannel(p_x, p_y, p_z)
+
+
+def generalized_amplitude_damp(p: float, gamma: float) -> GeneralizedAmplitudeDampingChannel:
+    return GeneralizedAmplitudeDampingChannel(p, gamma)
+
+
+def amplitude_damp(gamma: float) -> AmplitudeDampingChannel:
+    return AmplitudeDampingChannel(gamma)
+
+
+def phase_damp(gamma: float) -> PhaseDampingChannel:
+    return PhaseDampingChannel(gamma)
+
+
+def phase_flip(p: float) -> PhaseFlipChannel:
+    return PhaseFlipChannel(p)
+
+
+def bit_flip(p: float) -> BitFlipChannel:
+    return BitFlipChannel(p)
+
+
+def reset(qubit) -> "Operation":
+    return ResetChannel().on(qubit)
+
+
+def supported_channels() -> List[Channel]:
+    """One instance of every channel the noisy ops accept (the channel list of
+    ``tensorflow_quantum/python/util.py`` ``get_supported_channels``)."""
+    return [depolarize(0.01), asymmetric_depolarize(0.01, 0.02, 0.03),
+            generalized_amplitude_damp(0.01, 0.02), amplitude_damp(0.01), ResetChannel(),
+            phase_damp(0.01), phase_flip(0.01), bit_flip(0.01)]
+
+
 # Cirq-style constants and constructors.
 X = XPowGate()
 Y = YPowGate()
@@ -438,6 +597,14 @@ class Circuit:
 
     def __add__(self, other: "Circuit") -> "Circuit":
         return Circuit.from_moments(list(self.moments) + list(other.moments))
+
+
+def _flatten_qubits(items):
+    for it in items:
+        if isinstance(it, _Qubit):
+            yield it
+        else:
+            yield from _flatten_qubits(it)
 
 
 def _flatten(items):

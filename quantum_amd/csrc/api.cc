@@ -16,6 +16,7 @@
 #include "batch_kernels.h"
 #include "circuit.h"
 #include "large.h"
+#include "noisy.h"
 #include "plan.h"
 #include "wire.h"
 
@@ -106,6 +107,9 @@ struct tfqx_plan {
   // States wider than one LDS tile run on the HBM-streaming path.
   bool large = false;
   std::unique_ptr<LargePlan> large_plan;
+  // Noisy programs run on the trajectory kernel.
+  bool noisy = false;
+  std::unique_ptr<NoisyPlan> noisy_plan;
 };
 
 struct tfqx_graph {
@@ -149,12 +153,12 @@ Status ParsePrograms(const uint8_t* b, const int64_t* off, int64_t n, std::vecto
 Status Prepare(const uint8_t* programs, const int64_t* poff, int64_t B, const uint8_t* names,
                const int64_t* noff, int64_t S, const float* values, const uint8_t* psums,
                const int64_t* psoff, int64_t O, const uint8_t* other, const int64_t* ooff,
-               int64_t K, Inputs* in, int shared_max_n = -1) {
+               int64_t K, Inputs* in, int shared_max_n = -1, bool noisy = false) {
   if (B < 0 || S < 0 || O < 0 || K < 0) return Status::Invalid("negative dimension");
   in->values = values;
   // Byte-identical programs (a circuit tiled over a batch of parameter sets)
   // are parsed and resolved once.
-  bool same = B >= 1 && !other;
+  bool same = B >= 1 && !other && !noisy;
   for (int64_t i = 1; same && i < B; ++i) same = SameBytes(programs, poff, i, 0);
   Status s = ParsePrograms(programs, poff, same ? 1 : B, &in->programs);
   if (!s.ok()) return s;
@@ -197,7 +201,8 @@ Status Prepare(const uint8_t* programs, const int64_t* poff, int64_t B, const ui
         SymbolMap sm;
         sm.columns = &in->columns;
         sm.values = values + size_t(i) * S;
-        st = BuildCircuit(in->programs[i], sm, &in->circuits[i]);
+        st = noisy ? BuildNoisyCircuit(in->programs[i], sm, &in->circuits[i])
+                   : BuildCircuit(in->programs[i], sm, &in->circuits[i]);
         if (!st.ok()) return fe.record(st);
       }
     }
@@ -529,6 +534,74 @@ int tfqx_plan_simulate_samples(const uint8_t* programs, const int64_t* poff, int
   return TFQX_OK;
 }
 
+namespace {
+
+int BuildNoisy(int mode, Inputs& in, int64_t B, int64_t O, const int32_t* num_samples, int nshots,
+               uint64_t seed, tfqx_plan** plan, char* err, int64_t len) {
+  auto p = std::make_unique<tfqx_plan>();
+  p->mode = mode;
+  p->B = B;
+  p->O = O;
+  p->nshots = nshots;
+  p->seed = seed;
+  p->noisy = true;
+  Status s = BuildNoisyPlan(mode, in.circuits, in.sums, num_samples, nshots, B, O, &p->noisy_plan);
+  if (!s.ok()) return Fail(err, len, s.code, s.msg);
+  p->noisy_plan->seed = seed;
+  p->max_nq = p->noisy_plan->max_nq;
+  p->total = p->noisy_plan->total;
+  *plan = p.release();
+  return TFQX_OK;
+}
+
+}  // namespace
+
+int tfqx_plan_noisy_expectation(const uint8_t* programs, const int64_t* poff, int64_t B,
+                                const uint8_t* names, const int64_t* noff, int64_t S,
+                                const float* values, const uint8_t* psums, const int64_t* psoff,
+                                int64_t O, const int32_t* num_samples, uint64_t seed,
+                                tfqx_plan** plan, char* err, int64_t len) {
+  for (int64_t i = 0; i < B * O; ++i)
+    if (num_samples[i] < 1)
+      return Fail(err, len, TFQX_INVALID_ARGUMENT,
+                  "Each element of num_samples must be greater than 0.");
+  Inputs in;
+  Status s = Prepare(programs, poff, B, names, noff, S, values, psums, psoff, O, nullptr, nullptr,
+                     0, &in, -1, true);
+  if (!s.ok()) return Fail(err, len, s.code, s.msg);
+  return BuildNoisy(kModeNoisyExpectation, in, B, O, num_samples, 0, seed, plan, err, len);
+}
+
+int tfqx_plan_noisy_sampled_expectation(const uint8_t* programs, const int64_t* poff, int64_t B,
+                                        const uint8_t* names, const int64_t* noff, int64_t S,
+                                        const float* values, const uint8_t* psums,
+                                        const int64_t* psoff, int64_t O,
+                                        const int32_t* num_samples, uint64_t seed,
+                                        tfqx_plan** plan, char* err, int64_t len) {
+  for (int64_t i = 0; i < B * O; ++i)
+    if (num_samples[i] < 1)
+      return Fail(err, len, TFQX_INVALID_ARGUMENT,
+                  "Each element of num_samples must be greater than 0.");
+  Inputs in;
+  Status s = Prepare(programs, poff, B, names, noff, S, values, psums, psoff, O, nullptr, nullptr,
+                     0, &in, -1, true);
+  if (!s.ok()) return Fail(err, len, s.code, s.msg);
+  return BuildNoisy(kModeNoisySampledExpectation, in, B, O, num_samples, 0, seed, plan, err, len);
+}
+
+int tfqx_plan_noisy_samples(const uint8_t* programs, const int64_t* poff, int64_t B,
+                            const uint8_t* names, const int64_t* noff, int64_t S,
+                            const float* values, int32_t num_samples, uint64_t seed,
+                            tfqx_plan** plan, char* err, int64_t len) {
+  if (num_samples < 0)
+    return Fail(err, len, TFQX_INVALID_ARGUMENT, "num_samples must be non-negative.");
+  Inputs in;
+  Status s = Prepare(programs, poff, B, names, noff, S, values, nullptr, nullptr, 0, nullptr,
+                     nullptr, 0, &in, -1, true);
+  if (!s.ok()) return Fail(err, len, s.code, s.msg);
+  return BuildNoisy(kModeNoisySamples, in, B, 0, nullptr, num_samples, seed, plan, err, len);
+}
+
 int tfqx_plan_simulate_state(const uint8_t* programs, const int64_t* poff, int64_t B,
                              const uint8_t* names, const int64_t* noff, int64_t S,
                              const float* values, tfqx_plan** plan, char* err, int64_t len) {
@@ -683,6 +756,13 @@ int tfqx_plan_stats(const tfqx_plan* p, int64_t* out, int64_t n) {
     v[8] = int64_t(AlgorithmicFlops(p->prog));
     v[9] = int64_t(MfmaFlops(p->prog));
   }
+  if (p->noisy) {
+    const NoisyPlan& np = *p->noisy_plan;
+    v[0] = int64_t(np.circuits.size());
+    v[1] = 0;
+    for (const auto& op : np.ops) v[1] += op.type != kOpRemap;
+    v[2] = np.trajectories;
+  }
   v[7] = int64_t(p->total);
   for (int64_t i = 0; i < n && i < 10; ++i) out[i] = v[i];
   return TFQX_OK;
@@ -700,6 +780,9 @@ int64_t tfqx_plan_output_bytes(const tfqx_plan* p) {
     case kModeState: return p->B * (int64_t(1) << p->max_nq) * 8;
     case kModeSamples: return p->B * int64_t(p->nshots) * p->max_nq;
     case kModeInnerProductGrad: return p->B * p->S * 8;
+    case kModeNoisyExpectation:
+    case kModeNoisySampledExpectation: return p->B * p->O * 4;
+    case kModeNoisySamples: return p->B * int64_t(p->nshots) * p->max_nq;
     default: return p->B * p->K * 8;
   }
 }
@@ -707,6 +790,7 @@ int64_t tfqx_plan_output_bytes(const tfqx_plan* p) {
 int tfqx_plan_upload(tfqx_plan* p, void* ws, void* stream, char* err, int64_t len) {
   if (!p || (!ws && p->total)) return Fail(err, len, TFQX_INVALID_ARGUMENT, "null plan or workspace");
   if (p->large) return LargeUpload(p->large_plan.get(), ws, AsStream(stream), err, len);
+  if (p->noisy) return NoisyUpload(p->noisy_plan.get(), ws, AsStream(stream), err, len);
   if (p->staging.empty()) return TFQX_OK;
   return CheckHip(hipMemcpyAsync(ws, p->staging.data(), p->staging.size(), hipMemcpyHostToDevice,
                                  AsStream(stream)),
@@ -716,6 +800,7 @@ int tfqx_plan_upload(tfqx_plan* p, void* ws, void* stream, char* err, int64_t le
 int tfqx_plan_launch(tfqx_plan* p, void* ws, void* output, void* stream, char* err, int64_t len) {
   if (!p) return Fail(err, len, TFQX_INVALID_ARGUMENT, "null plan");
   if (p->large) return LargeLaunch(p->large_plan.get(), ws, output, AsStream(stream), err, len);
+  if (p->noisy) return NoisyLaunch(p->noisy_plan.get(), ws, output, AsStream(stream), err, len);
   if (p->prog.circuits.empty()) return TFQX_OK;
   if (p->grid == 0) {
     int maxg = BatchMaxGrid(p->lds_log2, p->mode, p->kmax);
@@ -730,6 +815,7 @@ int tfqx_plan_launch_plain(tfqx_plan* p, void* ws, void* output, void* stream, c
                            int64_t len) {
   if (!p) return Fail(err, len, TFQX_INVALID_ARGUMENT, "null plan");
   if (p->large) return LargeLaunch(p->large_plan.get(), ws, output, AsStream(stream), err, len);
+  if (p->noisy) return NoisyLaunch(p->noisy_plan.get(), ws, output, AsStream(stream), err, len);
   BatchArgs a = MakeArgs(p, ws, output);
   const DevCircuit* all = a.circuits;
   for (size_t c = 0; c < p->prog.circuits.size(); ++c) {

@@ -465,6 +465,147 @@ Status BuildCircuit(const Program& prog, const SymbolMap& sym, Circuit* out) {
   return Status();
 }
 
+namespace {
+
+void SetKraus(cdouble (&K)[4], cdouble a, cdouble b, cdouble c, cdouble d) {
+  K[0] = a; K[1] = b; K[2] = c; K[3] = d;
+}
+
+// Channel builders (circuit_parser_qsim.cc:598-790): the target qubit and the
+// float arguments; channel arguments cannot be symbols.
+Status ParseChannel(const Operation& op, const Circuit& circ, Channel* ch) {
+  static const char* kIds[] = {"DP", "ADP", "GAD", "AD", "RST", "PD", "PF", "BF"};
+  int kind = -1;
+  for (int i = 0; i < 8; ++i)
+    if (op.gate_id == kIds[i]) kind = i;
+  if (kind < 0) return Status::Invalid("Could not parse channel id: " + op.gate_id);
+  ch->kind = ChannelKind(kind);
+  if (op.qubits.size() != 1)
+    return Status::Invalid("Channel " + op.gate_id + " expects 1 qubit, got " +
+                           std::to_string(op.qubits.size()) + ".");
+  auto it = circ.qubit_index.find(op.qubits[0]);
+  if (it == circ.qubit_index.end()) return Status::Invalid("Unable to parse qubit: " + op.qubits[0]);
+  ch->bit = circ.num_qubits - 1 - it->second;
+  SymbolMap none;
+  auto arg = [&](const char* name, double* v) {
+    float f = 0.0f;
+    Status s = ParseArg(op, name, none, &f, nullptr);
+    *v = double(f);
+    return s;
+  };
+  const cdouble o(0.0), l(1.0), im(0.0, 1.0);
+  double p = 0, g = 0, px = 0, py = 0, pz = 0;
+  Status s;
+  switch (ch->kind) {
+    case kChanDP:
+      if (!(s = arg("p", &p)).ok()) return s;
+      px = py = pz = p / 3.0;
+      ch->mixture = true;
+      break;
+    case kChanADP:
+      if (!(s = arg("p_x", &px)).ok()) return s;
+      if (!(s = arg("p_y", &py)).ok()) return s;
+      if (!(s = arg("p_z", &pz)).ok()) return s;
+      ch->mixture = true;
+      break;
+    case kChanPF:
+      if (!(s = arg("p", &p)).ok()) return s;
+      ch->mixture = true;
+      ch->nk = 2;
+      ch->probs[0] = 1.0 - p;
+      ch->probs[1] = p;
+      SetKraus(ch->K[0], l, o, o, l);
+      SetKraus(ch->K[1], l, o, o, -l);
+      return Status();
+    case kChanBF:
+      if (!(s = arg("p", &p)).ok()) return s;
+      ch->mixture = true;
+      ch->nk = 2;
+      ch->probs[0] = 1.0 - p;
+      ch->probs[1] = p;
+      SetKraus(ch->K[0], l, o, o, l);
+      SetKraus(ch->K[1], o, l, l, o);
+      return Status();
+    case kChanAD:
+      if (!(s = arg("gamma", &g)).ok()) return s;
+      ch->nk = 2;
+      SetKraus(ch->K[0], l, o, o, std::sqrt(1.0 - g));
+      SetKraus(ch->K[1], o, std::sqrt(g), o, o);
+      return Status();
+    case kChanPD:
+      if (!(s = arg("gamma", &g)).ok()) return s;
+      ch->nk = 2;
+      SetKraus(ch->K[0], l, o, o, std::sqrt(1.0 - g));
+      SetKraus(ch->K[1], o, o, o, std::sqrt(g));
+      return Status();
+    case kChanGAD: {
+      if (!(s = arg("p", &p)).ok()) return s;
+      if (!(s = arg("gamma", &g)).ok()) return s;
+      ch->nk = 4;
+      const double a = std::sqrt(p), b = std::sqrt(1.0 - p);
+      const double sg = std::sqrt(g), cg = std::sqrt(1.0 - g);
+      SetKraus(ch->K[0], a, o, o, a * cg);
+      SetKraus(ch->K[1], o, a * sg, o, o);
+      SetKraus(ch->K[2], b * cg, o, o, b);
+      SetKraus(ch->K[3], o, o, b * sg, o);
+      return Status();
+    }
+    case kChanRST:
+      ch->nk = 2;
+      SetKraus(ch->K[0], l, o, o, o);
+      SetKraus(ch->K[1], o, l, o, o);
+      return Status();
+  }
+  // Depolarising mixtures: I, X, Y, Z.
+  ch->nk = 4;
+  ch->probs[0] = ch->kind == kChanDP ? 1.0 - p : 1.0 - px - py - pz;
+  ch->probs[1] = px;
+  ch->probs[2] = py;
+  ch->probs[3] = pz;
+  SetKraus(ch->K[0], l, o, o, l);
+  SetKraus(ch->K[1], o, l, l, o);
+  SetKraus(ch->K[2], o, -im, im, o);
+  SetKraus(ch->K[3], l, o, o, -l);
+  return Status();
+}
+
+}  // namespace
+
+Status BuildNoisyCircuit(const Program& prog, const SymbolMap& sym, Circuit* out) {
+  out->gates.clear();
+  out->channels.clear();
+  out->qubit_index.clear();
+  out->num_qubits = 0;
+  out->no_moments = prog.moments.empty();
+  if (out->no_moments) return Status();
+  std::vector<std::string> ids;
+  Status s = CollectQubits(prog, &ids);
+  if (!s.ok()) return s;
+  out->num_qubits = int(ids.size());
+  for (size_t i = 0; i < ids.size(); ++i) out->qubit_index[ids[i]] = int(i);
+  if (out->num_qubits == 0) return Status();
+  if (out->num_qubits > 40) return Status::Invalid("Circuits of more than 40 qubits are not supported.");
+  for (const auto& m : prog.moments) {
+    for (const auto& op : m) {
+      Gate g;
+      s = ParseGate(op, sym, *out, &g);
+      if (s.ok()) {
+        out->gates.push_back(g);
+        continue;
+      }
+      // A gate id that parsed and failed keeps its error; an unknown id is
+      // tried as a channel.
+      if (s.msg.rfind("Could not parse gate id", 0) != 0) return s;
+      Channel ch;
+      s = ParseChannel(op, *out, &ch);
+      if (!s.ok()) return s;
+      ch.gate_pos = int(out->gates.size());
+      out->channels.push_back(ch);
+    }
+  }
+  return Status();
+}
+
 void RebindCircuit(const Circuit& tmpl, const float* values, Circuit* out) {
   *out = tmpl;
   for (auto& g : out->gates) {

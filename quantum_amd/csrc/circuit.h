@@ -57,15 +57,36 @@ struct Gate {
   cdouble U[16];     // unitary, row-major (2^nt x 2^nt), bits[0] = MSB of the index
 };
 
+// A single-qubit noise channel of a noisy program (ParseAppendChannel,
+// core/src/circuit_parser_qsim.cc:598-826; ids DP ADP GAD AD RST PD PF BF).
+// Kraus operators follow Cirq's channel definitions (as qsim's
+// lib/channels_cirq.h implements them), evaluated in double from the float32
+// arguments.  Unitary mixtures (DP, ADP, PF, BF) keep each unitary with its
+// probability; the other channels keep their Kraus matrices.
+enum ChannelKind : int { kChanDP = 0, kChanADP, kChanGAD, kChanAD, kChanRST, kChanPD, kChanPF, kChanBF };
+struct Channel {
+  ChannelKind kind;
+  int bit = 0;           // target bit position
+  int gate_pos = 0;      // number of gates that precede it in program order
+  bool mixture = false;
+  int nk = 0;            // Kraus operators (2 or 4)
+  double probs[4] = {0, 0, 0, 0};  // mixtures: probability of each unitary
+  cdouble K[4][4];       // 2x2 row-major Kraus matrices (mixtures: the unitaries)
+};
+
 struct Circuit {
   int num_qubits = 0;
   bool no_moments = true;  // program had no moments: Pauli ids are not checked
   std::vector<Gate> gates;
+  std::vector<Channel> channels;  // noisy programs only, in program order
   std::unordered_map<std::string, int> qubit_index;  // proto id -> sorted index
 };
 
 // Parse + resolve one program.  `sym` may have no columns (empty map).
 Status BuildCircuit(const Program& prog, const SymbolMap& sym, Circuit* out);
+// Noisy programs (NoisyQsimCircuitFromProgram, circuit_parser_qsim.cc:828-870):
+// an operation whose id is not a gate id is parsed as a channel.
+Status BuildNoisyCircuit(const Program& prog, const SymbolMap& sym, Circuit* out);
 // `tmpl` with its symbolic arguments re-read from another row of values
 // (same program bytes, so the same gates, qubits and symbol columns).
 void RebindCircuit(const Circuit& tmpl, const float* values, Circuit* out);

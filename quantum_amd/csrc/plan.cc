@@ -200,15 +200,19 @@ void InsertRemapsR(std::vector<DevOp>* ops, size_t begin, int L, int R) {
   // ops it commutes with.
   std::vector<uint32_t> sup(n), need(n);
   std::vector<char> diag(n), done(n, 0);
+  // Noise channels keep their order among themselves (a channel's Kraus
+  // choice depends on the state the earlier channels left): they share a
+  // pseudo-qubit (bit 31, above any tile bit) in their support.
+  constexpr uint32_t kChannelBit = 1u << 31;
   for (size_t i = 0; i < n; ++i) {
     const DevOp& o = in[i];
     uint32_t T = 0;
     for (int t = 0; t < o.k; ++t) T |= 1u << o.q[t];
     diag[i] = o.type == kOpDiag;
-    sup[i] = T | o.cmask;
+    sup[i] = T | o.cmask | (o.type == kOpChannel ? kChannelBit : 0u);
     need[i] = diag[i] ? 0u : T;
   }
-  const uint32_t all = L >= 32 ? ~0u : (1u << L) - 1u;
+  const uint32_t all = L >= 31 ? ~kChannelBit : (1u << L) - 1u;
   // Ops scanned past the first pending one per segment (bounds planning time).
   constexpr size_t kWindow = 96;
   // Scan pending ops in order from `first`: an op joins the segment when it
@@ -230,7 +234,7 @@ void InsertRemapsR(std::vector<DevOp>* ops, size_t begin, int L, int R) {
       if (!ok) {
         if (diag[j]) blocked_diag |= sup[j];
         else blocked |= sup[j];
-        if ((blocked | blocked_diag) == all && blocked == all) break;
+        if (((blocked | blocked_diag) & all) == all && (blocked & all) == all) break;
         continue;
       }
       if (grow) S |= need[j];
@@ -748,6 +752,10 @@ void Concatenate(std::vector<Piece>& pieces, int num_slots, int num_symbols, con
 }
 
 }  // namespace
+
+DevOp FusedToDevOp(const FusedOp& f, std::vector<cfloat>* mats) { return MakeDevOp(f, mats); }
+
+DevTerm PauliToTerm(const Pauli& p, float w, int slot) { return MakeTerm(p, w, slot, true, 0); }
 
 double CircuitCost(const DevCircuit& c, const BatchProgram& p) {
   (void)p;

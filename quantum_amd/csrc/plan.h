@@ -147,6 +147,11 @@ void BuildSharedBatchProgram(const Circuit& tmpl, const std::vector<CircuitJob>&
 void RebindSharedBatchProgram(const Circuit& tmpl, const SharedRecipe& recipe, const float* values,
                               int num_symbols, bool tfq_fd, int threads, BatchProgram* prog);
 
+// A fused block as a device op (matrices appended to `mats`), and a Pauli term
+// in XOR-mask form with weight w * i^{#Y}.
+DevOp FusedToDevOp(const FusedOp& f, std::vector<cfloat>* mats);
+DevTerm PauliToTerm(const Pauli& p, float w, int slot);
+
 // Cost estimate used for longest-first ordering and the autotuner.
 double CircuitCost(const DevCircuit& c, const BatchProgram& p);
 

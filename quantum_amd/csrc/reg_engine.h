@@ -19,6 +19,8 @@
 // sum to the caller's accumulator.
 #pragma once
 
+#include <type_traits>
+
 #include "lds_ops.h"
 
 namespace tfqx {
@@ -599,6 +601,110 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// ------------------------------------------------------------------ noise channels
+// Trajectory context of a noisy forward run, passed as the run's acc_fn:
+// channel c of trajectory `traj` of output row `row` draws
+// uniform(seed, shot = traj, stream = (row, c)).  In wave mode a wavefront
+// holds 2^(6 - gbits) trajectories side by side, each on 2^gbits consecutive
+// lanes, and traj is this lane's.
+struct NoiseDraw {
+  unsigned long long seed;
+  unsigned long long traj;
+  uint32_t row;
+  int gbits;
+  __device__ __forceinline__ double operator()(int ch) const {
+    return philox_uniform(seed, traj, row, uint32_t(ch));
+  }
+  // Gradient partials (reverse runs only; never called on a forward run).
+  __device__ __forceinline__ void operator()(int, int, double, double) const {}
+};
+template <typename T>
+struct IsNoise : std::false_type {};
+template <>
+struct IsNoise<NoiseDraw> : std::true_type {};
+
+// Sum over the 2^gbits lanes of this lane's trajectory group (all of them
+// end with the same bits: each butterfly step adds the same two values).
+__device__ __forceinline__ double group_sum(double v, int gbits) {
+  for (int off = 1; off < (1 << gbits); off <<= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// One channel on register slot J of this lane's trajectory.  Unitary
+// mixtures pick unitary k with the first u < cum[k]; other channels take
+// p_k = <psi|K_k^dag K_k|psi> from the qubit's reduced density matrix
+// (group sums), pick the first k with u < p_0 + ... + p_k (the last k with
+// p_k > 0 when none) and apply K_k / sqrt(p_k).  Lanes of different
+// trajectories may pick different operators: the matrix is read per lane.
+template <int R, int J>
+__device__ __forceinline__ void reg_channel_j(cf (&a)[1 << R], const DevOp& op,
+                                              const float2* __restrict__ mats,
+                                              const NoiseDraw& nd) {
+  constexpr int G = 1 << (R - 1);
+  const int nk = ChanKraus(op.cval);
+  const double u = nd(op.col[0]);
+  const float2* X = mats + op.dmat[0];
+  int pick = nk - 1;
+  double pp = 1.0;
+  if (op.cval & kChanMixture) {
+    for (int k = nk - 2; k >= 0; --k)
+      if (u < double(X[k].x)) pick = k;
+    if ((op.cval & kChanFirstIdentity) && pick == 0) return;
+  } else {
+    float s00 = 0.f, s11 = 0.f, sr = 0.f, si = 0.f;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int i0 = ((g >> J) << (J + 1)) | (g & ((1 << J) - 1));
+      const cf x0 = a[i0], x1 = a[i0 | (1 << J)];
+      s00 += x0.x * x0.x + x0.y * x0.y;
+      s11 += x1.x * x1.x + x1.y * x1.y;
+      sr += x0.x * x1.x + x0.y * x1.y;  // conj(x0) * x1
+      si += x0.x * x1.y - x0.y * x1.x;
+    }
+    const double d00 = group_sum(double(s00), nd.gbits), d11 = group_sum(double(s11), nd.gbits);
+    const double dr = group_sum(double(sr), nd.gbits), di = group_sum(double(si), nd.gbits);
+    double cum = 0.0;
+    int last = 0;
+    pick = -1;
+    for (int k = 0; k < nk; ++k) {
+      const float2 e00 = X[4 * k], e01 = X[4 * k + 1], e11 = X[4 * k + 3];
+      const double pk = double(e00.x) * d00 + double(e11.x) * d11 +
+                        2.0 * (double(e01.x) * dr - double(e01.y) * di);
+      if (pk > 0.0) last = k;
+      cum += pk;
+      if (pick < 0 && u < cum) {
+        pick = k;
+        pp = pk;
+      }
+    }
+    if (pick < 0) {
+      pick = last;
+      const float2 e00 = X[4 * last], e01 = X[4 * last + 1], e11 = X[4 * last + 3];
+      pp = double(e00.x) * d00 + double(e11.x) * d11 + 2.0 * (double(e01.x) * dr - double(e01.y) * di);
+    }
+  }
+  const float2* M = mats + op.mat + 4 * pick;
+  const float sc = (op.cval & kChanMixture) ? 1.0f : float(1.0 / sqrt(pp));
+  const cf m00{M[0].x * sc, M[0].y * sc}, m01{M[1].x * sc, M[1].y * sc};
+  const cf m10{M[2].x * sc, M[2].y * sc}, m11{M[3].x * sc, M[3].y * sc};
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const int i0 = ((g >> J) << (J + 1)) | (g & ((1 << J) - 1));
+    const cf x0 = a[i0], x1 = a[i0 | (1 << J)];
+    a[i0] = cmac(cmulr(m00, x0), m01, x1);
+    a[i0 | (1 << J)] = cmac(cmulr(m10, x0), m11, x1);
+  }
+}
+
+template <int R, int J = 0>
+__device__ __forceinline__ void reg_channel(int j, cf (&a)[1 << R], const DevOp& op,
+                                            const float2* mats, const NoiseDraw& nd) {
+  if constexpr (J < R) {
+    if (j == J) reg_channel_j<R, J>(a, op, mats, nd);
+    else reg_channel<R, J + 1>(j, a, op, mats, nd);
+  }
+}
+
 // ------------------------------------------------------------------ driver
 // LDS window [lo, hi) of the plan's matrix array (global g).  All waves of a
 // workgroup run the same op sequence, so a refill (two barriers) happens at
@@ -748,6 +854,9 @@ __device__ __forceinline__ void reg_run(float2* __restrict__ A, float2* __restri
         if ((threadIdx.x & 63) == 0) acc_fn(slot + 1, op.col[1], w1, v1);
       }
       slot += op.nderiv;
+    } else if (op.type == kOpChannel) {
+      if constexpr (!REV && IsNoise<std::decay_t<AccFn>>::value)
+        reg_channel<R>(reg_slot(m, op.q[0]), ra, op, mats, acc_fn);
     } else {
       reg_dense<R, REV, REV>(ra, rb, op, mats + op.mat, m);
     }

@@ -22,7 +22,18 @@ enum OpType : uint8_t {
   kOpParam = 1,     // adjoint: psi <- G^dag psi; grad += 2Re<lam|dG psi>; lam <- G^dag lam
   kOpDiag = 2,      // diagonal M: only the 2^k diagonal entries are stored
   kOpRemap = 3,     // register engine: reload the tile with cmask = register-qubit mask
+  kOpChannel = 4,   // noise channel on q[0] (noisy trajectories; layout below)
 };
+
+// kOpChannel layout.  cval = nk | flags << 8 (cmask 0); mat = nk 2x2 Kraus
+// matrices (unitary mixtures: the unitaries); dmat[0] = unitary mixtures: nk
+// entries whose .x is the float32 cumulative probability p_0 + ... + p_k;
+// other channels: nk 2x2 matrices K_k^dag K_k.  nderiv = 1 (dmat[0] is
+// rebased with the matrices).  col[0] = the channel's ordinal in its program
+// (its random stream).
+constexpr uint32_t kChanMixture = 1u << 8;    // unitary mixture
+constexpr uint32_t kChanFirstIdentity = 1u << 9;  // unitary 0 is the identity
+constexpr int ChanKraus(uint32_t cval) { return int(cval & 15u); }
 
 // Register engine (reg_engine.h): 2^9 threads per workgroup, each holding
 // 2^R amplitudes of a 2^L tile in registers, R = L - 9 for tiles of 12..13
@@ -123,6 +134,19 @@ enum BatchMode : int {
   kModeSampledExpectation = 4,
   kModeInnerProduct = 5,
   kModeInnerProductGrad = 6,
+  // Noisy trajectory kernel (noisy_kernels.hip).
+  kModeNoisyExpectation = 7,
+  kModeNoisySampledExpectation = 8,
+  kModeNoisySamples = 9,
+};
+
+// Noisy trajectories: one work item of the trajectory kernel, trajectories
+// [t0, t1) of circuit `circuit` (an index into the plan's DevCircuit table),
+// whose program has `nq` qubits.
+struct alignas(8) DevChunk {
+  int32_t circuit;
+  int32_t t0, t1;
+  int32_t nq;
 };
 
 constexpr int kFlagEmpty = 1;

@@ -132,7 +132,10 @@ def _op_args(op: C.Operation) -> List:
     g = op.gate
     ctrl_q = ",".join(q.proto_id() for q in op.controls)
     ctrl_v = ",".join(str(v) for v in op.control_values)
-    if isinstance(g, C.IdentityGate):
+    if isinstance(g, C.Channel):
+        # Channel arguments are plain floats (serializer.py:174-455).
+        args = [(k, float(v)) for k, v in g.params().items()]
+    elif isinstance(g, C.IdentityGate):
         args = [("unused", 1.0)]
     elif isinstance(g, C.FSimGate):
         args = [("theta", symbol_part(g.theta)), ("phi", symbol_part(g.phi)),

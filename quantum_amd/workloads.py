@@ -91,6 +91,47 @@ def random_circuit(rng, qubits, n_ops, symbols=(), p_symbol=0.5, p_control=0.0, 
     return C.Circuit(ops)
 
 
+def random_noisy_circuit(rng, qubits, n_ops, symbols=(), p_symbol=0.5, p_control=0.0,
+                         p_channel=0.4, channels=None):
+    """random_circuit with a random channel (from ``channels``, default every
+    supported kind with random strengths) after each gate with probability
+    p_channel, on one of that gate's qubits."""
+    fams = ALL_GATE_FAMILIES
+    ops = []
+    for _ in range(n_ops):
+        op = random_gate_op(rng, qubits, list(symbols), fams[rng.integers(len(fams))],
+                            p_symbol, p_control)
+        ops.append(op)
+        if rng.uniform() < p_channel:
+            q = op.target_qubits[rng.integers(len(op.target_qubits))]
+            if channels is None:
+                ops.append(random_channel(rng).on(q))
+            else:
+                ops.append(channels[rng.integers(len(channels))].on(q))
+    return C.Circuit(ops)
+
+
+def random_channel(rng):
+    """A channel of a random supported kind with strengths in [0, 0.3)."""
+    u = lambda: float(np.round(rng.uniform(0.0, 0.3), 4))
+    kind = int(rng.integers(8))
+    if kind == 0:
+        return C.depolarize(u())
+    if kind == 1:
+        return C.asymmetric_depolarize(u() / 3, u() / 3, u() / 3)
+    if kind == 2:
+        return C.generalized_amplitude_damp(float(np.round(rng.uniform(), 4)), u())
+    if kind == 3:
+        return C.amplitude_damp(u())
+    if kind == 4:
+        return C.ResetChannel()
+    if kind == 5:
+        return C.phase_damp(u())
+    if kind == 6:
+        return C.phase_flip(u())
+    return C.bit_flip(u())
+
+
 def random_pauli_sum(rng, qubits, n_terms, max_weight=3):
     terms = []
     for _ in range(n_terms):

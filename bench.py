@@ -182,6 +182,69 @@ def inner_product_grad_measure(stream, steps=20, warmup=3, oracle_rows=2):
             "max_abs_err_vs_oracle": err, "oracle_rows_checked": oracle_rows}
 
 
+def noisy_measure(stream, steps=10, warmup=2, trajectories=2000, cpu_rows=8, cpu_threads=16,
+                  check=True):
+    """tfq.noise expectation on 32 noisy 8-qubit circuits (8 layers of ry + CZ
+    ladder, depolarize + amplitude damping on every qubit after each layer),
+    ``trajectories`` per (circuit, observable): one persistent launch per step
+    from a hipGraph.  The CPU leg runs the float32 C++ trajectory loop
+    (oracle/cpu_tfq.cc, same Philox draws) on the first ``cpu_rows`` rows."""
+    import torch
+    from quantum_amd import noise
+    from quantum_amd import serialize as S
+    from quantum_amd import workloads as W
+    circuits, names, values, ops = W.noisy_layers()
+    progs = S.convert_to_tensor(circuits)
+    ps = S.convert_to_tensor(ops)
+    B = len(circuits)
+    ns = np.full((B, 2), trajectories, np.int32)
+    seed = 20240
+    plan, shape = noise.plan_expectation(progs, names, values, ps, ns, seed=seed)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    out = torch.empty(shape, dtype=torch.float32, device=dev)
+    plan.upload(dev, stream)
+    elapsed, _, graph = time_graph(plan, out, stream, steps, warmup)
+    per = elapsed / steps
+    total = B * trajectories
+    flops = plan.stats()["flops"]
+    res = out.cpu().numpy()
+    log(f"noisy expectation: {per * 1e3:.3f} ms per launch of {total} trajectories, "
+        f"{total / per:.3e} trajectories/s, {flops / per / 1e12:.2f} TFLOP/s")
+    result = {"workload": "32 circuits x 8 qubits, 8 layers of ry(symbol) + CZ ladder, each followed "
+                          "by depolarize(0.01) and amplitude_damp(0.02) on every qubit; "
+                          "observables [sum Z_i Z_i+1, X_0]",
+              "trajectories_per_launch": total, "ms_per_launch": round(per * 1e3, 4),
+              "trajectories_per_s": round(total / per, 1),
+              "circuits_per_s": round(B / per, 2),
+              "roofline": {"bound": "mfma", "achieved": round(flops / per / 1e12, 3),
+                           "peak": 157.3, "unit": "TFLOP/s",
+                           "frac": round(flops / per / 1e12 / 157.3, 4),
+                           "flops_per_launch": flops,
+                           "kernel": "noisy_kernel (wave-mode register engine, FP32)"}}
+    del graph, plan
+    if check:
+        from oracle import cpu_tfq
+        from oracle import noise_oracle as N
+        low = cpu_tfq.LoweredNoisy(list(progs[:cpu_rows]), names, values[:cpu_rows],
+                                   ps[:cpu_rows])
+        t0 = time.perf_counter()
+        cpu = cpu_tfq.run_noisy(low, ns[:cpu_rows], seed, threads=cpu_threads)
+        cpu_s = time.perf_counter() - t0
+        exact = N.density_expectation(list(progs[:1]), names, values[:1], ps[:1])
+        result["max_abs_diff_vs_cpu_port"] = float(np.abs(res[:cpu_rows] - cpu).max())
+        result["max_abs_err_vs_density_matrix_row0"] = float(np.abs(res[:1] - exact).max())
+        result["cpu_baseline"] = {
+            "value": round(cpu_rows * trajectories / cpu_s, 1), "unit": "trajectories/s",
+            "cores": cpu_threads, "kind": "port",
+            "sample": f"first {cpu_rows} circuits x {trajectories} trajectories of the same batch; "
+                      f"float32 C++ trajectory loop of TFQ's noisy CPU schedule "
+                      f"(oracle/cpu_tfq.cc), {cpu_threads} threads, {cpu_s:.2f}s"}
+        log(f"noisy: max |gpu - cpu port| = {result['max_abs_diff_vs_cpu_port']:.2e}, "
+            f"|gpu - density matrix| (row 0) = {result['max_abs_err_vs_density_matrix_row0']:.3f}, "
+            f"cpu {result['cpu_baseline']['value']:.0f} trajectories/s")
+    return result
+
+
 def measured_traffic(circuits, flops):
     """HBM bytes per launch from the committed rocprofv3 counter summary, if it
     was taken on this same workload (same circuit count and flop count)."""
@@ -234,7 +297,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the autotuner (the --only profiling runs never tune: they use "
                          "the planner defaults)")
-    ap.add_argument("--only", choices=["all", "batch", "hbm", "copy", "ipgrad"], default="all",
+    ap.add_argument("--only", choices=["all", "batch", "hbm", "copy", "ipgrad", "noisy"], default="all",
                     help="profiling runs: 'batch' launches only the batched kernel (graph "
                          "replays, no plain launches, no CPU baseline); 'hbm' only the 30-qubit "
                          "streaming passes; 'copy' a 1 GiB device copy (counter calibration); "
@@ -267,6 +330,10 @@ def main():
         return
     if args.only == "ipgrad":
         print(json.dumps({"inner_product_grad": inner_product_grad_measure(stream)}), flush=True)
+        return
+    if args.only == "noisy":
+        print(json.dumps({"noisy": noisy_measure(stream, steps=args.steps, warmup=args.warmup)}),
+              flush=True)
         return
     progs, names, values, ps, weights = build_uneven(args.circuits)
     tuned = None
@@ -336,6 +403,7 @@ def main():
             from quantum_amd import autotune as AT
             AT.apply(batch_setting)
     ipg = inner_product_grad_measure(stream)
+    noisy = noisy_measure(stream)
 
     done, cpu_dt, cores, err, checked, err_cpu = cpu_baseline(
         progs, names, values, ps, weights, gpu_exp, gpu_grad, args.cpu_sample, args.oracle_check)
@@ -370,6 +438,7 @@ def main():
         "roofline": roof,
         "hbm_pass": hbm,
         "inner_product_grad": ipg,
+        "noisy": noisy,
         "cpu_baseline": {"value": round(done / cpu_dt, 3), "unit": "circuits/s", "cores": cores,
                          "kind": "port",
                          "sample": f"first {done} circuits of the same batch, expectation (basic "

@@ -224,8 +224,8 @@ int tfqx_debug_fuse(const uint8_t* program, int64_t program_len, const uint8_t* 
  * HBM-streaming path is used, [7] workspace bytes, [8] algorithmic real flops
  * of one launch (batch path; 0 on the streaming path), [9] the part of those
  * flops that runs on FP32 MFMA (both paths).  Noisy plans: [0] non-empty
- * circuits, [1] ops (gates and channels), [2] trajectories per launch.  No
- * reference counterpart. */
+ * circuits, [1] ops (gates and channels), [2] trajectories per launch, [8]
+ * expected algorithmic real flops of one launch.  No reference counterpart. */
 int tfqx_plan_stats(const tfqx_plan* plan, int64_t* out, int64_t n);
 
 /* Largest qubit count in the batch (sizes the samples / state outputs). */

@@ -21,6 +21,7 @@
 // oracle's parser and supplies matrices and derivative matrices.
 #include <stdint.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <complex>
@@ -198,6 +199,148 @@ int cpu_tfq_run(int mode, int64_t B, const int32_t* circ, const int32_t* gates_i
     std::vector<std::thread> pool;
     for (int t = 0; t < threads; ++t) pool.emplace_back(work);
     for (auto& th : pool) th.join();
+  }
+  return 0;
+}
+
+// Noisy trajectories (TFQ's ComputeSmall schedule of
+// core/ops/noise/tfq_noisy_expectation.cc:284-398: trajectories spread over
+// the worker threads, one float32 state each).  A gate record with nt == -1 is
+// a channel: b0 = bit, nderiv = Kraus count, mat = Kraus matrices, dmat0 =
+// cumulative probabilities (col1 == 1, unitary mixture; entries (cum, 0)) or
+// K^dag K matrices, col0 = channel ordinal.  Channel c of trajectory t of row
+// i draws Philox uniform (seed, t, (i, c)); operator k is the first with
+// u < p_0 + ... + p_k.  ns[B][O]: trajectories per (row, slot).
+// exp_out[B][O] = mean over trajectories + identity terms.
+static double philox_u(unsigned long long seed, unsigned long long shot, uint32_t s_lo, uint32_t s_hi) {
+  uint32_t c0 = uint32_t(shot), c1 = uint32_t(shot >> 32), c2 = s_lo, c3 = s_hi;
+  uint32_t k0 = uint32_t(seed), k1 = uint32_t(seed >> 32);
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = uint64_t(0xD2511F53u) * c0, p1 = uint64_t(0xCD9E8D57u) * c2;
+    const uint32_t n0 = uint32_t(p1 >> 32) ^ c1 ^ k0, n2 = uint32_t(p0 >> 32) ^ c3 ^ k1;
+    c0 = n0; c1 = uint32_t(p1); c2 = n2; c3 = uint32_t(p0);
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  const unsigned long long bits = (uint64_t(c0) << 32) | c1;
+  return double(bits >> 11) * (1.0 / 9007199254740992.0);
+}
+
+int cpu_tfq_noisy(int64_t B, const int32_t* circ, const int32_t* gates_i, const uint64_t* gates_m,
+                  const float* mats, const int32_t* terms_i, const float* terms_c,
+                  const int32_t* paulis, int64_t O, const int32_t* ns, uint64_t seed, int threads,
+                  float* exp_out) {
+  const c64* M = reinterpret_cast<const c64*>(mats);
+  // Work items: (circuit, trajectory).
+  std::vector<int64_t> first(size_t(B) + 1, 0);
+  for (int64_t i = 0; i < B; ++i) {
+    int T = 0;
+    for (int64_t j = 0; j < O; ++j) T = std::max(T, ns[i * O + j]);
+    first[size_t(i) + 1] = first[size_t(i)] + T;
+  }
+  const int64_t total = first[size_t(B)];
+  const int nt = std::max(1, threads);
+  std::vector<std::vector<double>> acc(size_t(nt), std::vector<double>(size_t(B * O), 0.0));
+  std::atomic<int64_t> next(0);
+  auto work = [&](int tid) {
+    std::vector<c64> psi, scratch;
+    std::vector<double>& my = acc[size_t(tid)];
+    while (true) {
+      const int64_t w = next.fetch_add(1);
+      if (w >= total) break;
+      int64_t i = 0;
+      while (first[size_t(i) + 1] <= w) ++i;
+      const int64_t t = w - first[size_t(i)];
+      const int32_t* c = circ + 8 * i;
+      const int n = c[0];
+      const int64_t row = c[7];
+      const uint64_t N = uint64_t(1) << n;
+      psi.assign(N, c64(0, 0));
+      psi[0] = c64(1, 0);
+      c64 U[16];
+      for (int k = c[1]; k < c[2]; ++k) {
+        const int32_t* a = gates_i + 9 * k;
+        if (a[0] >= 0) {
+          Gate g;
+          g.nt = a[0]; g.b0 = a[1]; g.b1 = a[2]; g.mat = a[3];
+          g.cmask = gates_m[2 * k]; g.cval = gates_m[2 * k + 1];
+          load_matrix(M + g.mat, 1 << g.nt, false, U);
+          apply(psi.data(), n, g, U);
+          continue;
+        }
+        const int b = a[1], nk = a[4];
+        const double u = philox_u(seed, uint64_t(t), uint32_t(row), uint32_t(a[7]));
+        int pick = nk - 1;
+        double pp = 1.0;
+        if (a[8] == 1) {
+          for (int q = nk - 2; q >= 0; --q)
+            if (u < double(M[a[5] + q].real())) pick = q;
+        } else {
+          double s00 = 0, s11 = 0, sr = 0, si = 0;
+          const uint64_t bit = uint64_t(1) << b;
+          for (uint64_t h = 0; h < N / 2; ++h) {
+            const uint64_t x0 = ins0(h, b);
+            const c64 a0 = psi[x0], a1 = psi[x0 | bit];
+            s00 += std::norm(a0);
+            s11 += std::norm(a1);
+            sr += double(a0.real()) * a1.real() + double(a0.imag()) * a1.imag();
+            si += double(a0.real()) * a1.imag() - double(a0.imag()) * a1.real();
+          }
+          double cum = 0;
+          int last = 0;
+          pick = -1;
+          for (int q = 0; q < nk; ++q) {
+            const c64* E = M + a[5] + 4 * q;
+            const double pk = E[0].real() * s00 + E[3].real() * s11 +
+                              2.0 * (double(E[1].real()) * sr - double(E[1].imag()) * si);
+            if (pk > 0) last = q;
+            cum += pk;
+            if (pick < 0 && u < cum) {
+              pick = q;
+              pp = pk;
+            }
+          }
+          if (pick < 0) {
+            pick = last;
+            const c64* E = M + a[5] + 4 * last;
+            pp = E[0].real() * s00 + E[3].real() * s11 + 2.0 * (double(E[1].real()) * sr - double(E[1].imag()) * si);
+          }
+        }
+        Gate g{};
+        g.nt = 1;
+        g.b0 = b;
+        const float sc = a[8] == 1 ? 1.0f : float(1.0 / std::sqrt(pp));
+        for (int e = 0; e < 4; ++e) U[e] = M[a[3] + 4 * pick + e] * sc;
+        apply(psi.data(), n, g, U);
+      }
+      for (int tt = c[5]; tt < c[6]; ++tt) {
+        const int32_t* ti = terms_i + 3 * tt;
+        const int np = ti[2] - ti[1];
+        if (np == 0 || t >= ns[row * O + ti[0]]) continue;
+        scratch = psi;
+        apply_paulis(scratch.data(), n, paulis + 2 * ti[1], np);
+        my[size_t(row * O + ti[0])] += double(terms_c[tt]) * real_inner(psi.data(), scratch.data(), N);
+      }
+    }
+  };
+  if (nt <= 1) {
+    work(0);
+  } else {
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; ++t) pool.emplace_back(work, t);
+    for (auto& th : pool) th.join();
+  }
+  for (int64_t i = 0; i < B; ++i) {
+    const int32_t* c = circ + 8 * i;
+    const int64_t row = c[7];
+    for (int64_t j = 0; j < O; ++j) {
+      double s = 0;
+      for (int t = 0; t < nt; ++t) s += acc[size_t(t)][size_t(row * O + j)];
+      double ident = 0;
+      for (int tt = c[5]; tt < c[6]; ++tt)
+        if (terms_i[3 * tt] == j && terms_i[3 * tt + 2] == terms_i[3 * tt + 1]) ident += terms_c[tt];
+      exp_out[row * O + j] = float(s / double(ns[row * O + j]) + ident);
+    }
   }
   return 0;
 }

@@ -45,6 +45,9 @@ def lib():
         L.cpu_tfq_run.argtypes = [ctypes.c_int, ctypes.c_int64, P, P, P, P, P, P, P,
                                   ctypes.c_int64, ctypes.c_int64, P, ctypes.c_int, P, P]
         L.cpu_tfq_run.restype = ctypes.c_int
+        L.cpu_tfq_noisy.argtypes = [ctypes.c_int64, P, P, P, P, P, P, P, ctypes.c_int64, P,
+                                    ctypes.c_uint64, ctypes.c_int, P]
+        L.cpu_tfq_noisy.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -158,3 +161,78 @@ def run(low, mode, downstream=None, threads=1):
     exp = exp[:, :low.O]
     exp[low.empty] = -2.0
     return exp if mode == 0 else grad[:, :low.S]
+
+
+class LoweredNoisy:
+    """Flat arrays of a noisy batch for ``cpu_tfq_noisy``: gates one by one and
+    channels as records with nt = -1 (programs parsed by ``noise_oracle``)."""
+
+    def __init__(self, programs, symbol_names, symbol_values, pauli_sums):
+        from . import noise_oracle as N
+        names = [T._name(s) for s in symbol_names]
+        maps = T._symbol_maps(names, np.asarray(symbol_values, dtype=np.float32).reshape(len(programs), -1))
+        self.B = len(programs)
+        ps = np.asarray(pauli_sums, dtype=object)
+        self.O = ps.shape[1] if self.B else 0
+        circ, gi, gm, mats, ti, tc, pl = [], [], [], [], [], [], []
+        moff = [0]
+
+        def push(arr):
+            off = moff[0]
+            m = np.asarray(arr, dtype=np.complex64).reshape(-1)
+            mats.append(m)
+            moff[0] += m.size
+            return off
+
+        for i in range(self.B):
+            prog = N.NoisyProgram(programs[i], maps[i])
+            gb = len(gi)
+            ch = 0
+            for kind, it in prog.items:
+                if kind == "g":
+                    cmask = sum(1 << b for b in it.cbits)
+                    cval = sum(int(v) << b for b, v in zip(it.cbits, it.cvals))
+                    bits = list(it.bits)
+                    gi.append([len(bits), bits[0], bits[1] if len(bits) > 1 else 0,
+                               push(it.unitary()), 0, 0, 0, 0, 0])
+                    gm.append([cmask, cval])
+                    continue
+                mo = push(np.concatenate([K.reshape(-1) for K, _ in it.kraus]))
+                if it.cid in N.MIXTURES:
+                    aux = push(N._mixture_cum(it.kraus).astype(np.complex64))
+                    mix = 1
+                else:
+                    aux = push(np.concatenate([(K.conj().T @ K).reshape(-1) for K, _ in it.kraus]))
+                    mix = 0
+                gi.append([-1, it.bit, 0, mo, len(it.kraus), aux, 0, ch, mix])
+                gm.append([0, 0])
+                ch += 1
+            ge = len(gi)
+            tb = len(ti)
+            for j in range(self.O):
+                for c, paulis in T.parse_paulisum(ps[i][j], prog.qmap, prog.n):
+                    ti.append([j, len(pl), len(pl) + len(paulis)])
+                    tc.append(c)
+                    pl.extend([b, _PT[t]] for b, t in paulis)
+            circ.append([prog.n, gb, ge, 0, 0, tb, len(ti), i])
+        self.circ = np.array(circ, np.int32).reshape(-1, 8)
+        self.gi = np.array(gi, np.int32).reshape(-1, 9)
+        self.gm = np.array(gm, np.uint64).reshape(-1, 2)
+        self.mats = np.concatenate(mats) if mats else np.zeros(1, np.complex64)
+        self.ti = np.array(ti, np.int32).reshape(-1, 3)
+        self.tc = np.array(tc, np.float32)
+        self.pl = np.array(pl, np.int32).reshape(-1, 2)
+
+
+def run_noisy(low, num_samples, seed, threads=1):
+    """Noisy expectations [B, O]: term j of row i averaged over
+    num_samples[i][j] trajectories (same Philox draws as the device)."""
+    out = np.zeros((low.B, max(low.O, 1)), np.float32)
+    ns = np.ascontiguousarray(num_samples, dtype=np.int32)
+    mats = np.ascontiguousarray(low.mats.view(np.float32))
+    rc = lib().cpu_tfq_noisy(low.B, _p(low.circ), _p(low.gi), _p(low.gm), _p(mats), _p(low.ti),
+                             _p(low.tc), _p(low.pl), low.O, _p(ns), int(seed) & (2 ** 64 - 1),
+                             int(threads), _p(out))
+    if rc != 0:
+        raise RuntimeError("cpu_tfq_noisy failed")
+    return out[:, :low.O]

@@ -762,6 +762,29 @@ int tfqx_plan_stats(const tfqx_plan* p, int64_t* out, int64_t n) {
     v[1] = 0;
     for (const auto& op : np.ops) v[1] += op.type != kOpRemap;
     v[2] = np.trajectories;
+    // Algorithmic real flops per trajectory of each circuit: a dense k-qubit
+    // op 8 * 2^k per amplitude, a diagonal op 6, a channel its expected
+    // application (16 per amplitude, times the non-identity probability of a
+    // unitary mixture) plus 8 per amplitude for the reduced density matrix of
+    // other channels; each Pauli term 8 per amplitude.
+    std::vector<double> per(np.circuits.size(), 0.0);
+    for (size_t ci = 0; ci < np.circuits.size(); ++ci) {
+      const DevCircuit& c = np.circuits[ci];
+      const double A = std::ldexp(1.0, c.n);
+      double f = 8.0 * A * double(c.term_end - c.term_begin);
+      for (int i = c.fwd_begin; i < c.fwd_end; ++i) {
+        const DevOp& op = np.ops[size_t(i)];
+        if (op.type == kOpUnitary) f += 8.0 * A * double(1 << op.k);
+        else if (op.type == kOpDiag) f += 6.0 * A;
+        else if (op.type == kOpChannel)
+          f += (op.cval & kChanMixture) ? 16.0 * A * (1.0 - double(np.mats[op.dmat[0]].real()))
+                                         : 24.0 * A;
+      }
+      per[ci] = f;
+    }
+    double total = 0.0;
+    for (const auto& ch : np.chunks) total += per[size_t(ch.circuit)] * double(ch.t1 - ch.t0);
+    v[8] = int64_t(total);
   }
   v[7] = int64_t(p->total);
   for (int64_t i = 0; i < n && i < 10; ++i) out[i] = v[i];

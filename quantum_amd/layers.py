@@ -10,8 +10,12 @@
 repetitions=..., initializer=...)``: a single circuit is tiled over the batch
 of symbol values, a single operator (or flat list) over the batch of circuits;
 when ``symbol_values`` is omitted the layer owns learnable parameters
-initialised ``U(0, 2*pi)`` (``expectation.py:372-379``).  Only the noiseless
-native backend exists; its default differentiator is ``Adjoint``.
+initialised ``U(0, 2*pi)`` (``expectation.py:372-379``).  Backends: the
+native noiseless simulator (default differentiator ``Adjoint``) and
+``backend="noisy"`` (Monte-Carlo trajectories, ``tfq.noise``), which needs
+``repetitions`` trajectories per operator in the call (``expectation.py:278-345``)
+and defaults to ``CentralDifference`` (the reference defaults to
+ParameterShift, which is not built here).
 """
 
 import math
@@ -21,6 +25,7 @@ import numpy as np
 from . import circuits as C
 from . import circuit_execution_ops as ceo
 from . import differentiators as diff
+from . import noise
 from . import serialize
 from . import tfq_simulate_ops as sim
 
@@ -93,42 +98,75 @@ class _LearnableMixin:
         return self._w.unsqueeze(0).expand(batch, -1)
 
 
+def _check_backend(backend):
+    if backend not in (None, "noiseless", "noisy"):
+        raise NotImplementedError("Only the native noiseless and noisy backends are available.")
+    return backend == "noisy"
+
+
+def _noisy_op(fn, seed):
+    def op(programs, symbol_names, symbol_values, pauli_sums, num_samples):
+        return fn(programs, symbol_names, symbol_values, pauli_sums, num_samples, seed=seed)
+    return op
+
+
+def _repetitions(repetitions, shape, batch):
+    """Trajectories per (circuit, operator) from an int, a list per operator
+    or a full [batch, n_ops] array (``expectation.py:333-352``)."""
+    if isinstance(repetitions, (int, np.integer)):
+        return np.full(shape, int(repetitions), np.int32)
+    reps = np.asarray(repetitions, np.int32)
+    if reps.ndim == 1:
+        reps = np.tile(reps[None], (batch, 1))
+    return reps
+
+
 class Expectation(_module_base(), _LearnableMixin):
-    def __init__(self, backend="noiseless", differentiator=None):
+    def __init__(self, backend="noiseless", differentiator=None, seed=None):
         super().__init__()
-        if backend not in (None, "noiseless"):
-            raise NotImplementedError("Only the noiseless native backend is available.")
+        self.noisy = _check_backend(backend)
         if differentiator is None:
-            differentiator = diff.Adjoint()
+            differentiator = diff.CentralDifference() if self.noisy else diff.Adjoint()
         if not isinstance(differentiator, diff.Differentiator):
             raise TypeError("Differentiator must inherit from Differentiator")
-        self._op = differentiator.generate_differentiable_op(analytic_op=ceo.get_expectation_op())
+        if self.noisy:
+            self._op = differentiator.generate_differentiable_op(
+                sampled_op=_noisy_op(noise.expectation, seed))
+        else:
+            self._op = differentiator.generate_differentiable_op(
+                analytic_op=ceo.get_expectation_op())
         self._w = None
 
     def forward(self, inputs, *, symbol_names=None, symbol_values=None, operators=None,
                 repetitions=None, initializer=None):
-        if repetitions is not None:
+        if not self.noisy and repetitions is not None:
             raise RuntimeError("repetitions value provided for analytic expectation calculation "
                                "that is noiseless.")
+        if self.noisy and repetitions is None:
+            raise RuntimeError("Value for repetitions not provided. With backend='noisy' a number "
+                               "of trajectory repetitions must be provided in the layer call "
+                               "method.")
         given = symbol_values
         progs, names, values = expand_circuits(inputs, symbol_names, symbol_values)
         ops = expand_operators(operators, len(progs))
         if given is None and names:
             values = self._values(names, None, len(progs), initializer)
+        if self.noisy:
+            return self._op(progs, names, values, ops, _repetitions(repetitions, ops.shape, len(progs)))
         return self._op(progs, names, values, ops)
 
 
 class SampledExpectation(_module_base(), _LearnableMixin):
     def __init__(self, backend="noiseless", differentiator=None, seed=None):
         super().__init__()
-        if backend not in (None, "noiseless"):
-            raise NotImplementedError("Only the noiseless native backend is available.")
+        noisy = _check_backend(backend)
         if differentiator is None:
             differentiator = diff.CentralDifference()
         if isinstance(differentiator, diff.Adjoint):
             raise ValueError("sample base backends are not supported by the Adjoint method.")
-        self._op = differentiator.generate_differentiable_op(
-            sampled_op=ceo.get_sampled_expectation_op(seed=seed))
+        op = (_noisy_op(noise.sampled_expectation, seed) if noisy
+              else ceo.get_sampled_expectation_op(seed=seed))
+        self._op = differentiator.generate_differentiable_op(sampled_op=op)
         self._w = None
 
     def forward(self, inputs, *, symbol_names=None, symbol_values=None, operators=None,
@@ -154,9 +192,10 @@ class Sample(_module_base()):
 
     def __init__(self, backend="noiseless", seed=None):
         super().__init__()
-        if backend not in (None, "noiseless"):
-            raise NotImplementedError("Only the noiseless native backend is available.")
-        self._op = ceo.get_sampling_op(seed=seed)
+        if _check_backend(backend):
+            self._op = lambda p, n, v, r: noise.samples(p, n, v, r, seed=seed)
+        else:
+            self._op = ceo.get_sampling_op(seed=seed)
 
     def forward(self, inputs, *, symbol_names=None, symbol_values=None, repetitions=None):
         if repetitions is None:

@@ -309,3 +309,24 @@ def config5_vqe(n=12, layers=6):
     H = sum Z_i Z_i+1 (ring) + 1.0 sum X_i."""
     circuit, names, q = hea_ring(n, layers)
     return circuit, names, tfim(q, 1.0, ring=True)
+
+
+def noisy_layers(seed=1010, batch=32, n=8, layers=8, p_dep=0.01, gamma=0.02):
+    """Noisy variational circuits: ``layers`` layers of ry(symbol) on every
+    qubit and a CZ ladder, each followed by depolarize(p_dep) and
+    amplitude_damp(gamma) on every qubit; observables [sum Z_i Z_i+1, X_0]."""
+    rng = np.random.default_rng(seed)
+    q = C.LineQubit.range(n)
+    names = [f"t{l}_{i}" for l in range(layers) for i in range(n)]
+    moments = []
+    for l in range(layers):
+        moments.append([C.ry(C.Symbol(f"t{l}_{i}")).on(q[i]) for i in range(n)])
+        moments.append([C.CZ(q[i], q[i + 1]) for i in range(0, n - 1, 2)])
+        moments.append([C.CZ(q[i], q[i + 1]) for i in range(1, n - 1, 2)])
+        moments.append([C.depolarize(p_dep).on(x) for x in q])
+        moments.append([C.amplitude_damp(gamma).on(x) for x in q])
+    circuit = C.Circuit.from_moments(moments)
+    values = rng.uniform(-math.pi, math.pi, size=(batch, len(names))).astype(np.float32)
+    zz = C.PauliSum([C.pauli_z(q[i]) * C.pauli_z(q[i + 1]) for i in range(n - 1)])
+    ops = [zz, C.PauliSum([C.pauli_x(q[0])])]
+    return [circuit] * batch, names, values, [ops] * batch

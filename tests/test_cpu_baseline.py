@@ -55,3 +55,25 @@ def test_basic_fusion_preserves_the_state():
             s = O.apply_matrix(s, U, bits, prog.n, cb, [(cv >> b) & 1 for b in cb])
         return s
     np.testing.assert_allclose(run(fused), run(plain), atol=1e-12)
+
+
+def test_noisy_baseline_matches_trajectory_replica():
+    """The float32 C++ trajectory loop (the noisy op's CPU baseline) draws the
+    same Philox variates as the oracle's replica, so the averages agree."""
+    from oracle import noise_oracle as N
+    rng = np.random.default_rng(41)
+    names = ["a", "b"]
+    circuits, sums = [], []
+    for n in (3, 5, 7):
+        q = C.LineQubit.range(n)
+        circuits.append(W.random_noisy_circuit(rng, q, 3 * n, symbols=names, p_channel=0.6,
+                                               p_control=0.1) + C.Circuit([C.I(x) for x in q]))
+        sums.append([W.random_pauli_sum(rng, q, 3), W.random_pauli_sum(rng, q, 2)])
+    progs = [S.serialize_circuit(c) for c in circuits]
+    ps = [[S.serialize_paulisum(x) for x in row] for row in sums]
+    vals = rng.uniform(-2, 2, size=(3, 2)).astype(np.float32)
+    ns = np.array([[40, 25], [30, 30], [12, 40]], np.int32)
+    low = cpu_tfq.LoweredNoisy(progs, names, vals, ps)
+    got = cpu_tfq.run_noisy(low, ns, seed=99, threads=4)
+    ref = N.trajectory_expectation(progs, names, vals, ps, ns, seed=99)
+    np.testing.assert_allclose(got, ref, atol=2e-4)

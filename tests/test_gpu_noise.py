@@ -212,3 +212,43 @@ def test_noisy_empty_inputs_and_errors():
     wide = S.convert_to_tensor([C.Circuit([C.bit_flip(0.1).on(x) for x in C.LineQubit.range(14)])])
     with pytest.raises(NotImplementedError, match="at most 13 qubits"):
         noise.samples(wide, [], np.zeros((1, 0)), [4])
+
+
+def test_noisy_layers():
+    """``backend='noisy'`` on the Expectation, SampledExpectation and Sample
+    layers (``expectation.py:278-345``)."""
+    import torch
+    from quantum_amd import layers
+    q = C.LineQubit.range(3)
+    a = C.Symbol("a")
+    circ = C.Circuit([C.rx(a).on(q[0]), C.CNOT(q[0], q[1]), C.amplitude_damp(0.2).on(q[1]),
+                      C.depolarize(0.05).on(q[2]), C.H(q[2])])
+    op = C.PauliSum([C.pauli_z(q[1])])
+    _say("noisy layers")
+    exp = layers.Expectation(backend="noisy", seed=5)
+    with pytest.raises(RuntimeError, match="repetitions"):
+        exp([circ], symbol_names=["a"], symbol_values=np.array([[0.7]], np.float32), operators=op)
+    v = torch.tensor([[0.7]], device="cuda", requires_grad=True)
+    out = exp([circ], symbol_names=["a"], symbol_values=v, operators=op, repetitions=20000)
+    prog = [S.serialize_circuit(circ)]
+    ps = [[S.serialize_paulisum(op)]]
+    exact = N.density_expectation(prog, ["a"], np.array([[0.7]]), ps)[0, 0]
+    assert abs(float(out[0, 0]) - exact) < 0.03
+    out.sum().backward()
+    h = 1e-3
+    fd = (N.density_expectation(prog, ["a"], np.array([[0.7 + h]]), ps)[0, 0] -
+          N.density_expectation(prog, ["a"], np.array([[0.7 - h]]), ps)[0, 0]) / (2 * h)
+    assert np.isfinite(float(v.grad[0, 0]))
+    assert abs(float(v.grad[0, 0]) - fd) < 0.25
+    sexp = layers.SampledExpectation(backend="noisy", seed=6)
+    s_out = sexp([circ], symbol_names=["a"], symbol_values=np.array([[0.7]], np.float32),
+                 operators=op, repetitions=20000)
+    assert abs(float(s_out[0, 0]) - exact) < 0.05
+    smp = layers.Sample(backend="noisy", seed=7)
+    bits = smp([circ], symbol_names=["a"], symbol_values=np.array([[0.7]], np.float32),
+               repetitions=100)
+    assert tuple(bits[0].shape) == (100, 3)
+    with pytest.raises(RuntimeError, match="noiseless"):
+        layers.Expectation()([circ], symbol_names=["a"],
+                             symbol_values=np.array([[0.7]], np.float32), operators=op,
+                             repetitions=10)

@@ -219,6 +219,43 @@ int tfqx_debug_fuse(const uint8_t* program, int64_t program_len, const uint8_t* 
                     int64_t gates_cap, int64_t* n_ops, int64_t* n_gates, char* err,
                     int64_t err_len);
 
+/* Parameter-shift program rewriting (host only).  String results come back
+ * in a tfqx_strings handle: count, bytes + offsets[count+1], and the result
+ * shape; free it with tfqx_strings_destroy. */
+typedef struct tfqx_strings tfqx_strings;
+int64_t tfqx_strings_count(const tfqx_strings* s);
+const uint8_t* tfqx_strings_data(const tfqx_strings* s);
+const int64_t* tfqx_strings_offsets(const tfqx_strings* s);
+int64_t tfqx_strings_dim(const tfqx_strings* s, int64_t i);
+void tfqx_strings_destroy(tfqx_strings* s);
+
+/* TfqPsDecompose(programs[B]) -> programs[B].  Replaces
+ * core/ops/tfq_ps_decompose_op.cc:39-160: symbolic PhasedISwap, ISwap,
+ * PhasedX and FSim gates become Z / XX / YY / X / CZ eigen gates with one
+ * symbol each, in extra moments after their own. */
+int tfqx_ps_decompose(const uint8_t* programs, const int64_t* program_offsets, int64_t batch,
+                      tfqx_strings** out, char* err, int64_t err_len);
+
+/* TfqPsSymbolReplace(programs[B], symbols[S], replacement_symbols[S])
+ * -> programs[B, S, pad].  Replaces core/ops/tfq_ps_symbol_replace_op.cc:
+ * 39-185: entry (i, j, k) is programs[i] with the k-th occurrence of
+ * symbols[j] (moment, op, argument order) renamed to replacement_symbols[j];
+ * padded with the empty program. */
+int tfqx_ps_symbol_replace(const uint8_t* programs, const int64_t* program_offsets, int64_t batch,
+                           const uint8_t* symbols, const int64_t* symbol_offsets, int64_t num_symbols,
+                           const uint8_t* replacements, const int64_t* replacement_offsets,
+                           int64_t num_replacements, tfqx_strings** out, char* err,
+                           int64_t err_len);
+
+/* TfqPsWeightsFromSymbols(programs[B], symbols[S]) -> float[B, S, pad].
+ * Replaces core/ops/tfq_ps_weights_from_symbols_op.cc:39-170: the
+ * exponent_scalar of every gate whose exponent is symbols[j], zero padded.
+ * Call with out = NULL to get *pad, then with capacity >= B * S * pad. */
+int tfqx_ps_weights_from_symbols(const uint8_t* programs, const int64_t* program_offsets,
+                                 int64_t batch, const uint8_t* symbols,
+                                 const int64_t* symbol_offsets, int64_t num_symbols, float* out,
+                                 int64_t capacity, int64_t* pad, char* err, int64_t err_len);
+
 /* Plan statistics: out[0] circuits, [1] forward ops, [2] forward HBM passes,
  * [3] max qubits, [4] adjoint ops, [5] adjoint HBM passes, [6] 1 if the
  * HBM-streaming path is used, [7] workspace bytes, [8] algorithmic real flops

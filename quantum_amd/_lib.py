@@ -49,6 +49,21 @@ def _declare(lib):
     lib.tfqx_plan_noisy_expectation.argtypes = s + [P, P, i64, P, u64, P, c.c_char_p, i64]
     lib.tfqx_plan_noisy_sampled_expectation.argtypes = s + [P, P, i64, P, u64, P, c.c_char_p, i64]
     lib.tfqx_plan_noisy_samples.argtypes = s + [i32, u64, P, c.c_char_p, i64]
+    lib.tfqx_ps_decompose.argtypes = [P, P, i64, P, c.c_char_p, i64]
+    lib.tfqx_ps_symbol_replace.argtypes = [P, P, i64, P, P, i64, P, P, i64, P, c.c_char_p, i64]
+    lib.tfqx_ps_weights_from_symbols.argtypes = [P, P, i64, P, P, i64, P, i64, P, c.c_char_p, i64]
+    for name in ("tfqx_ps_decompose", "tfqx_ps_symbol_replace", "tfqx_ps_weights_from_symbols"):
+        getattr(lib, name).restype = c.c_int
+    lib.tfqx_strings_count.argtypes = [P]
+    lib.tfqx_strings_count.restype = i64
+    lib.tfqx_strings_data.argtypes = [P]
+    lib.tfqx_strings_data.restype = P
+    lib.tfqx_strings_offsets.argtypes = [P]
+    lib.tfqx_strings_offsets.restype = P
+    lib.tfqx_strings_dim.argtypes = [P, i64]
+    lib.tfqx_strings_dim.restype = i64
+    lib.tfqx_strings_destroy.argtypes = [P]
+    lib.tfqx_strings_destroy.restype = None
     lib.tfqx_plan_inner_product.argtypes = s + [P, P, i64, P, c.c_char_p, i64]
     lib.tfqx_plan_inner_product_grad.argtypes = s + [P, P, i64, P, i32, P, c.c_char_p, i64]
     lib.tfqx_plan_adjoint_gradient.argtypes = s + [P, P, i64, P, i32, P, c.c_char_p, i64]
@@ -114,7 +129,10 @@ def lib():
 EXPORTED_SYMBOLS = (
     "tfqx_plan_simulate_expectation", "tfqx_plan_simulate_sampled_expectation",
     "tfqx_plan_simulate_samples", "tfqx_plan_simulate_state", "tfqx_plan_noisy_expectation",
-    "tfqx_plan_noisy_sampled_expectation", "tfqx_plan_noisy_samples", "tfqx_plan_inner_product",
+    "tfqx_plan_noisy_sampled_expectation", "tfqx_plan_noisy_samples", "tfqx_ps_decompose",
+    "tfqx_ps_symbol_replace", "tfqx_ps_weights_from_symbols", "tfqx_strings_count",
+    "tfqx_strings_data", "tfqx_strings_offsets", "tfqx_strings_dim", "tfqx_strings_destroy",
+    "tfqx_plan_inner_product",
     "tfqx_plan_inner_product_grad", "tfqx_plan_adjoint_gradient", "tfqx_plan_expectation_and_gradient", "tfqx_plan_rebind",
     "tfqx_debug_fuse",
     "tfqx_plan_stats",

@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "libtfqx.so")
 ARCH = os.environ.get("TFQX_OFFLOAD_ARCH", "gfx950")
 
 HOST_SOURCES = ["wire.cc", "circuit.cc", "plan.cc", "large.cc", "noisy.cc", "api.cc",
-                "surrogate_api.cc"]
+                "ps_ops.cc", "surrogate_api.cc"]
 # The batch kernel's modes are separate translation units so they compile in
 # parallel (each instantiates the register engine many times).
 HIP_SOURCES = ["batch_mode_adjoint.hip", "batch_mode_expectation.hip", "batch_mode_other.hip",

@@ -85,7 +85,10 @@ float AsFloat(uint64_t v) {
 bool DecodeArgValue(const uint8_t* b, size_t n, Arg* a) {
   FOR_FIELDS(b, n) {
     if (num == 1 && wt == 5) a->f = AsFloat(v);
-    else if (num == 3 && wt == 2) a->str.assign(reinterpret_cast<const char*>(sub), sl);
+    else if (num == 3 && wt == 2) {
+      a->str.assign(reinterpret_cast<const char*>(sub), sl);
+      a->is_str = true;
+    }
     else if (num == 4 && wt == 1) {
       double d;
       memcpy(&d, &v, 8);
@@ -200,6 +203,89 @@ bool DecodeProgram(const uint8_t* data, size_t len, Program* out) {
     }
   }
   return rd.ok;
+}
+
+namespace {
+
+void PutVarint(std::string* o, uint64_t v) {
+  while (v >= 0x80) {
+    o->push_back(char(uint8_t(v) | 0x80));
+    v >>= 7;
+  }
+  o->push_back(char(uint8_t(v)));
+}
+
+void PutLen(std::string* o, uint32_t field, const std::string& payload) {
+  PutVarint(o, (uint64_t(field) << 3) | 2);
+  PutVarint(o, payload.size());
+  o->append(payload);
+}
+
+std::string EncodeArg(const Arg& a) {
+  std::string out;
+  if (a.has_symbol) {
+    PutLen(&out, 2, a.symbol);
+  } else if (a.has_value) {
+    std::string v;
+    if (a.is_str) {
+      PutLen(&v, 3, a.str);
+    } else {
+      PutVarint(&v, (1u << 3) | 5);
+      uint32_t u;
+      memcpy(&u, &a.f, 4);
+      for (int i = 0; i < 4; ++i) v.push_back(char(uint8_t(u >> (8 * i))));
+    }
+    PutLen(&out, 1, v);
+  }
+  return out;
+}
+
+std::string EncodeOperation(const Operation& op) {
+  std::string out, gate;
+  PutLen(&gate, 1, op.gate_id);
+  PutLen(&out, 1, gate);
+  for (const auto& kv : op.args) {
+    std::string entry;
+    PutLen(&entry, 1, kv.first);
+    PutLen(&entry, 2, EncodeArg(kv.second));
+    PutLen(&out, 2, entry);
+  }
+  for (const auto& q : op.qubits) {
+    std::string qb;
+    PutLen(&qb, 2, q);
+    PutLen(&out, 3, qb);
+  }
+  return out;
+}
+
+std::string Language() {
+  std::string lang;
+  PutLen(&lang, 1, "tfq_gate_set");
+  return lang;
+}
+
+}  // namespace
+
+std::string EncodeProgram(const Program& p) {
+  std::string circ;
+  PutVarint(&circ, (1u << 3) | 0);  // scheduling_strategy = MOMENT_BY_MOMENT
+  PutVarint(&circ, 1);
+  for (const auto& m : p.moments) {
+    std::string mom;
+    for (const auto& op : m) PutLen(&mom, 1, EncodeOperation(op));
+    PutLen(&circ, 2, mom);
+  }
+  std::string out;
+  PutLen(&out, 1, Language());
+  PutLen(&out, 2, circ);
+  return out;
+}
+
+std::string EncodeEmptyProgram() {
+  std::string out;
+  PutLen(&out, 1, Language());
+  PutLen(&out, 2, std::string());
+  return out;
 }
 
 bool DecodePauliSum(const uint8_t* data, size_t len, PauliSum* out) {

@@ -18,6 +18,7 @@ namespace tfqx {
 struct Arg {
   bool has_value = false;   // arg_value set (oneof)
   bool has_symbol = false;  // symbol set (oneof)
+  bool is_str = false;      // arg_value holds string_value
   float f = 0.0f;           // arg_value.float_value (or double_value narrowed)
   std::string str;          // arg_value.string_value
   std::string symbol;
@@ -51,6 +52,11 @@ struct PauliSum {
 
 // Return false on malformed input.
 bool DecodeProgram(const uint8_t* data, size_t len, Program* out);
+// Program -> proto3 bytes: language.gate_set "tfq_gate_set", scheduling
+// MOMENT_BY_MOMENT, every kept field (float values as float_value).
+std::string EncodeProgram(const Program& p);
+// The empty program of TFQ's padding (gate set and an empty circuit).
+std::string EncodeEmptyProgram();
 bool DecodePauliSum(const uint8_t* data, size_t len, PauliSum* out);
 
 }  // namespace tfqx

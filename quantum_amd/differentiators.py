@@ -6,15 +6,20 @@ Mirrors ``python/differentiators/differentiator.py:58-187`` (the
 ``linear_combination.py`` (``ForwardDifference``, ``CentralDifference``).
 ``tf.custom_gradient`` becomes a ``torch.autograd.Function``: the forward runs
 the wrapped op, the backward calls ``differentiate_analytic`` /
-``differentiate_sampled`` with the upstream gradient.
+``differentiate_sampled`` with the upstream gradient.  ``ParameterShift``
+(``parameter_shift.py``) builds its shifted circuits with the native
+rewriting ops of ``ps_util_ops``.
 """
 
 import inspect
 
 import numpy as np
 
+from . import ps_util_ops
 from . import tfq_adj_grad_op
 from .tfq_simulate_ops import as_names, as_string_tensor, as_values
+
+PARAMETER_IMPURITY_NAME = "_impurity_for_param_shift"
 
 
 def _torch():
@@ -242,3 +247,83 @@ class CentralDifference(LinearCombination):
             raise ValueError("error_order must be 2 or 4.")
         h = float(grid_spacing)
         super().__init__([x / h for x in w], [x * h for x in p])
+
+
+class ParameterShift(Differentiator):
+    """Parameter-shift rule (``parameter_shift.py:28-150``,
+    ``parameter_shift_util.py:25-110``).
+
+    Programs are decomposed into single-symbol eigen gates with two
+    eigenvalues (``tfq_ps_decompose``); each occurrence g of symbol s (with
+    exponent scalar c_g) is renamed to an impurity symbol
+    (``tfq_ps_symbol_replace``) and evaluated at v_s +- 0.5 / c_g.  Then
+    d<O>/dv_s = sum_g (pi c_g / 2) (<O>(v_s + 0.5/c_g) - <O>(v_s - 0.5/c_g)).
+    All shifted circuits of a batch go to the expectation op in one call.
+    """
+
+    def get_gradient_circuits(self, programs, symbol_names, symbol_values):
+        progs = as_string_tensor(programs, 1, "programs")
+        names = as_names(symbol_names)
+        vals = as_values(symbol_values)
+        B, S = vals.shape
+        decomposed = ps_util_ops.tfq_ps_decompose(progs)
+        new = ps_util_ops.tfq_ps_symbol_replace(decomposed, names,
+                                                [PARAMETER_IMPURITY_NAME] * S)  # [B, S, G]
+        coeff = ps_util_ops.tfq_ps_weights_from_symbols(decomposed, names)  # [B, S, G]
+        if new.shape[2] != coeff.shape[2]:
+            raise ValueError("parameter-shift programs and weights do not match: "
+                             f"{new.shape[2]} != {coeff.shape[2]} occurrences.")
+        G = new.shape[2]
+        f32 = np.float32
+        # weights = c * pi * 0.5 * 0.5 * delta_eig, shifts = v +- (1 / delta_eig) / c
+        # (divide_no_nan), all in float32 as TensorFlow evaluates them.
+        w_plus = (((coeff * f32(np.pi)) * f32(0.5)) * f32(0.5)) * f32(2.0)
+        weights = np.concatenate([w_plus[..., None], -w_plus[..., None]], axis=-1)  # [B,S,G,2]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            s_plus = np.where(coeff != 0, f32(0.5) / coeff, f32(0.0)).astype(f32)
+        shifts = vals[:, :, None, None] + np.concatenate([s_plus[..., None], -s_plus[..., None]],
+                                                         axis=-1)
+        M = S * G * 2
+        batch_programs = np.repeat(new[..., None], 2, axis=-1).reshape(B, M)
+        batch_values = np.concatenate(
+            [np.repeat(vals[:, None, :], M, axis=1), shifts.reshape(B, M, 1)], axis=2).astype(f32)
+        batch_weights = weights.reshape(B, S, G * 2)
+        batch_mapper = np.tile(np.arange(M).reshape(1, S, G * 2), (B, 1, 1))
+        new_names = np.array(list(names) + [PARAMETER_IMPURITY_NAME], dtype=object)
+        return batch_programs, new_names, batch_values, batch_weights, batch_mapper
+
+    def _jacobian_contract(self, programs, symbol_names, symbol_values, pauli_sums, grad,
+                           evaluate):
+        torch = _torch()
+        bp, nn, bv, bw, bm = self.get_gradient_circuits(programs, symbol_names, symbol_values)
+        psums = as_string_tensor(pauli_sums, 2, "pauli_sums")
+        B, M = bp.shape
+        S = bw.shape[1]
+        if M == 0:
+            return torch.zeros((B, S), dtype=torch.float32, device=grad.device)
+        exps = evaluate(bp.reshape(B * M), nn, bv.reshape(B * M, -1), np.repeat(psums, M, axis=0),
+                        M)
+        exps = exps.reshape(B, M, -1).float()
+        idx = torch.as_tensor(bm, device=exps.device, dtype=torch.long)  # [B, S, K]
+        w = torch.as_tensor(bw, device=exps.device)  # [B, S, K]
+        picked = torch.gather(exps.unsqueeze(1).expand(B, S, M, exps.shape[-1]), 2,
+                              idx.unsqueeze(-1).expand(B, S, idx.shape[2], exps.shape[-1]))
+        jac = (w.unsqueeze(-1) * picked).sum(2)  # [B, S, O]
+        g = grad.to(exps.device).float()
+        return (jac * g.unsqueeze(1)).sum(-1)
+
+    @catch_empty_inputs
+    def differentiate_analytic(self, programs, symbol_names, symbol_values, pauli_sums,
+                               forward_pass_vals, grad):
+        op = self.expectation_op
+        return self._jacobian_contract(programs, symbol_names, symbol_values, pauli_sums, grad,
+                                       lambda p, n, v, s, reps: op(p, n, v, s))
+
+    @catch_empty_inputs
+    def differentiate_sampled(self, programs, symbol_names, symbol_values, pauli_sums,
+                              num_samples, forward_pass_vals, grad):
+        op = self.expectation_op
+        ns = as_values(num_samples, "num_samples", 2, np.int32)
+        return self._jacobian_contract(programs, symbol_names, symbol_values, pauli_sums, grad,
+                                       lambda p, n, v, s, reps: op(p, n, v, s,
+                                                                   np.repeat(ns, reps, axis=0)))

@@ -13,9 +13,8 @@ when ``symbol_values`` is omitted the layer owns learnable parameters
 initialised ``U(0, 2*pi)`` (``expectation.py:372-379``).  Backends: the
 native noiseless simulator (default differentiator ``Adjoint``) and
 ``backend="noisy"`` (Monte-Carlo trajectories, ``tfq.noise``), which needs
-``repetitions`` trajectories per operator in the call (``expectation.py:278-345``)
-and defaults to ``CentralDifference`` (the reference defaults to
-ParameterShift, which is not built here).
+``repetitions`` trajectories per operator in the call (``expectation.py:278-345``).
+Sample-based layers default to ``ParameterShift``, as the reference's do.
 """
 
 import math
@@ -126,7 +125,7 @@ class Expectation(_module_base(), _LearnableMixin):
         super().__init__()
         self.noisy = _check_backend(backend)
         if differentiator is None:
-            differentiator = diff.CentralDifference() if self.noisy else diff.Adjoint()
+            differentiator = diff.ParameterShift() if self.noisy else diff.Adjoint()
         if not isinstance(differentiator, diff.Differentiator):
             raise TypeError("Differentiator must inherit from Differentiator")
         if self.noisy:
@@ -161,7 +160,7 @@ class SampledExpectation(_module_base(), _LearnableMixin):
         super().__init__()
         noisy = _check_backend(backend)
         if differentiator is None:
-            differentiator = diff.CentralDifference()
+            differentiator = diff.ParameterShift()
         if isinstance(differentiator, diff.Adjoint):
             raise ValueError("sample base backends are not supported by the Adjoint method.")
         op = (_noisy_op(noise.sampled_expectation, seed) if noisy

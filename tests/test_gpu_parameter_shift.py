@@ -40,7 +40,12 @@ def _batch(seed, n, batch, depth):
     q = C.LineQubit.range(n)
     circuits = []
     for _ in range(batch):
-        c = W.random_circuit(rng, q, depth, symbols=names, p_symbol=0.6, p_control=0.0)
+        # Random PhasedISwaps are left out: one with a symbolic phase and a
+        # numeric exponent decomposes (as in the reference) into parts with an
+        # empty symbol, which the weights op rejects (test below).
+        fams = [f for f in W.ALL_GATE_FAMILIES if f != "PISP"]
+        c = W.random_circuit(rng, q, depth, symbols=names, p_symbol=0.6, p_control=0.0,
+                             families=fams)
         c = c + C.Circuit([C.CZ(q[0], q[1]).controlled_by(q[2]), C.H(q[2]).controlled_by(q[0])])
         t = C.Symbol("a")
         extra = C.Circuit([C.PhasedISwapPowGate(phase_exponent=t * 0.7,
@@ -119,3 +124,21 @@ def test_parameter_shift_rule_on_a_controlled_rotation():
     f = O.expectation_function(list(progs), ["v"], ps)
     rule = 0.5 * (f(vals + math.pi / 2) - f(vals - math.pi / 2))[0, 0]
     np.testing.assert_allclose(float(t.grad[0, 0]), rule, atol=1e-5)
+
+
+def test_parameter_shift_rejects_a_phased_iswap_with_numeric_exponent():
+    """The reference's decomposition gives the ISWAP part of such a gate an
+    empty symbol (tfq_ps_decompose_op.cc:84-91), which
+    tfq_ps_weights_from_symbols does not find among the symbols."""
+    import torch
+    from quantum_amd import circuit_execution_ops as ceo
+    from quantum_amd import differentiators as diff
+    from quantum_amd._lib import InvalidArgumentError
+    q = C.LineQubit.range(2)
+    circ = C.Circuit([C.PhasedISwapPowGate(phase_exponent=C.Symbol("p"), exponent=0.5).on(q[0], q[1])])
+    op = diff.ParameterShift().generate_differentiable_op(analytic_op=ceo.get_expectation_op())
+    t = torch.tensor([[0.3]], device="cuda", requires_grad=True)
+    out = op(S.convert_to_tensor([circ]), ["p"], t,
+             S.convert_to_tensor([[C.PauliSum([C.pauli_z(q[0])])]]))
+    with pytest.raises(InvalidArgumentError, match="sympy.Symbol"):
+        out.sum().backward()

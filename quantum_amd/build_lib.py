@@ -16,7 +16,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtfqx.so")
 ARCH = os.environ.get("TFQX_OFFLOAD_ARCH", "gfx950")
 
-HOST_SOURCES = ["wire.cc", "circuit.cc", "plan.cc", "large.cc", "noisy.cc", "api.cc",
+HOST_SOURCES = ["wire.cc", "textproto.cc", "circuit.cc", "plan.cc", "large.cc", "noisy.cc", "api.cc",
                 "ps_ops.cc", "surrogate_api.cc"]
 # The batch kernel's modes are separate translation units so they compile in
 # parallel (each instantiates the register engine many times).

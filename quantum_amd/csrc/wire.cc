@@ -3,6 +3,8 @@
 
 #include <string.h>
 
+#include "textproto.h"
+
 namespace tfqx {
 namespace {
 
@@ -34,6 +36,7 @@ struct Reader {
              uint64_t* v) {
     uint64_t key = varint();
     if (!ok) return false;
+    if ((key >> 3) == 0) return ok = false;  // field number 0 is invalid
     *num = uint32_t(key >> 3);
     *wire = uint32_t(key & 7);
     switch (*wire) {
@@ -172,7 +175,7 @@ const Arg* Operation::find(const char* name) const {
   return nullptr;
 }
 
-bool DecodeProgram(const uint8_t* data, size_t len, Program* out) {
+bool DecodeProgramBinary(const uint8_t* data, size_t len, Program* out) {
   out->moments.clear();
   FOR_FIELDS(data, len) {
     if (num == 2 && wt == 2) {  // Circuit
@@ -288,7 +291,7 @@ std::string EncodeEmptyProgram() {
   return out;
 }
 
-bool DecodePauliSum(const uint8_t* data, size_t len, PauliSum* out) {
+bool DecodePauliSumBinary(const uint8_t* data, size_t len, PauliSum* out) {
   out->terms.clear();
   FOR_FIELDS(data, len) {
     if (num == 1 && wt == 2) {
@@ -321,6 +324,25 @@ bool DecodePauliSum(const uint8_t* data, size_t len, PauliSum* out) {
     }
   }
   return rd.ok;
+}
+
+}  // namespace tfqx
+
+namespace tfqx {
+
+// ParseProto (core/ops/parse_context.cc:41-56): binary first, then text.
+bool DecodeProgram(const uint8_t* data, size_t len, Program* out) {
+  if (DecodeProgramBinary(data, len, out)) return true;
+  std::string bin;
+  if (!TextProgramToBinary(std::string(reinterpret_cast<const char*>(data), len), &bin)) return false;
+  return DecodeProgramBinary(reinterpret_cast<const uint8_t*>(bin.data()), bin.size(), out);
+}
+
+bool DecodePauliSum(const uint8_t* data, size_t len, PauliSum* out) {
+  if (DecodePauliSumBinary(data, len, out)) return true;
+  std::string bin;
+  if (!TextPauliSumToBinary(std::string(reinterpret_cast<const char*>(data), len), &bin)) return false;
+  return DecodePauliSumBinary(reinterpret_cast<const uint8_t*>(bin.data()), bin.size(), out);
 }
 
 }  // namespace tfqx

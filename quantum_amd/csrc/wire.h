@@ -50,8 +50,11 @@ struct PauliSum {
   std::vector<PauliTerm> terms;
 };
 
-// Return false on malformed input.
+// Return false on malformed input.  The binary wire format is tried first,
+// then the protobuf text format (textproto.h), as TFQ's ParseProto does.
 bool DecodeProgram(const uint8_t* data, size_t len, Program* out);
+bool DecodeProgramBinary(const uint8_t* data, size_t len, Program* out);
+bool DecodePauliSumBinary(const uint8_t* data, size_t len, PauliSum* out);
 // Program -> proto3 bytes: language.gate_set "tfq_gate_set", scheduling
 // MOMENT_BY_MOMENT, every kept field (float values as float_value).
 std::string EncodeProgram(const Program& p);

@@ -46,6 +46,9 @@ def as_values(v, name="symbol_values", rank=2, dtype=np.float32):
     if isinstance(v, torch.Tensor):
         v = v.detach().to("cpu").numpy()
     arr = np.asarray(v)
+    if arr.dtype.kind in "SUO" and any(isinstance(x, (str, bytes)) for x in arr.reshape(-1)):
+        # tf.cast of a string tensor (UnimplementedError in the reference).
+        raise NotImplementedError("Cast string to float is not supported")
     if arr.dtype == object:
         arr = arr.astype(np.float64)
     if np.iscomplexobj(arr):
@@ -61,8 +64,17 @@ def as_names(names):
         names = list(names.reshape(-1)) if names.ndim <= 1 else None
         if names is None:
             raise InvalidArgumentError("symbol_names must be rank 1.")
-    names = [n.decode("utf-8") if isinstance(n, bytes) else str(n) for n in list(names)]
-    return np.array(names, dtype=object)
+    out = []
+    for n in list(names):
+        if isinstance(n, bytes):
+            out.append(n.decode("utf-8"))
+        elif isinstance(n, str):
+            out.append(n)
+        else:
+            # A string tensor cannot hold numbers (TypeError from
+            # tf.convert_to_tensor in the reference).
+            raise TypeError(f"Cannot convert {list(names)!r} to EagerTensor of dtype string")
+    return np.array(out, dtype=object)
 
 
 def _device_and_stream():

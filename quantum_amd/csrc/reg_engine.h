@@ -607,13 +607,28 @@ __device__ __forceinline__ double wave_sum(double v) {
 // uniform(seed, shot = traj, stream = (row, c)).  In wave mode a wavefront
 // holds 2^(6 - gbits) trajectories side by side, each on 2^gbits consecutive
 // lanes, and traj is this lane's.
+//
+// The variates of a trajectory are drawn 2^gbits at a time: lane li of the
+// trajectory's group computes the variate of channel base + li, and channel
+// c (base <= c < base + 2^gbits) reads it from lane c - base of its group.
+// Channel ordinals only grow along the op list and every lane of a wavefront
+// meets the same channels, so `base` is wave-uniform.
 struct NoiseDraw {
   unsigned long long seed;
   unsigned long long traj;
   uint32_t row;
   int gbits;
-  __device__ __forceinline__ double operator()(int ch) const {
-    return philox_uniform(seed, traj, row, uint32_t(ch));
+  int base = -1;
+  double cached = 0.0;
+  __device__ __forceinline__ double operator()(int ch) {
+    const int width = 1 << gbits;
+    if (base < 0 || ch < base || ch >= base + width) {
+      base = ch;
+      const uint32_t li = (threadIdx.x & 63u) & uint32_t(width - 1);
+      cached = philox_uniform(seed, traj, row, uint32_t(ch) + li);
+    }
+    const int src = int((threadIdx.x & 63u) & ~uint32_t(width - 1)) + (ch - base);
+    return __shfl(cached, src, 64);
   }
   // Gradient partials (reverse runs only; never called on a forward run).
   __device__ __forceinline__ void operator()(int, int, double, double) const {}
@@ -639,7 +654,7 @@ __device__ __forceinline__ double group_sum(double v, int gbits) {
 template <int R, int J>
 __device__ __forceinline__ void reg_channel_j(cf (&a)[1 << R], const DevOp& op,
                                               const float2* __restrict__ mats,
-                                              const NoiseDraw& nd) {
+                                              NoiseDraw& nd) {
   constexpr int G = 1 << (R - 1);
   const int nk = ChanKraus(op.cval);
   const double u = nd(op.col[0]);
@@ -698,7 +713,7 @@ __device__ __forceinline__ void reg_channel_j(cf (&a)[1 << R], const DevOp& op,
 
 template <int R, int J = 0>
 __device__ __forceinline__ void reg_channel(int j, cf (&a)[1 << R], const DevOp& op,
-                                            const float2* mats, const NoiseDraw& nd) {
+                                            const float2* mats, NoiseDraw& nd) {
   if constexpr (J < R) {
     if (j == J) reg_channel_j<R, J>(a, op, mats, nd);
     else reg_channel<R, J + 1>(j, a, op, mats, nd);

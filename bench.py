@@ -182,6 +182,76 @@ def inner_product_grad_measure(stream, steps=20, warmup=3, oracle_rows=2):
             "max_abs_err_vs_oracle": err, "oracle_rows_checked": oracle_rows}
 
 
+def ops_measure(stream, steps=20, warmup=3, shots=1000):
+    """The other ops of the family on the 64 x 10-qubit rotation batch (600
+    symbols, observables [sum Z_i Z_i+1, X_0]): one launch per step from a
+    hipGraph, with 2 rows checked against the oracle (sampled ops against the
+    oracle's replica of the same Philox stream)."""
+    import torch
+    from oracle import tfq_oracle as O
+    from quantum_amd import _lib
+    from quantum_amd import circuits as C
+    from quantum_amd import serialize as S
+    from quantum_amd import workloads as W
+    from quantum_amd.tfq_simulate_ops import _common, _plan_args
+    circuits, names, values, ops = W.config1_line_rotations()
+    progs, pnames, pvalues = _common(S.convert_to_tensor(circuits), names, values)
+    ps = S.convert_to_tensor(ops)
+    B, O_ = len(circuits), ps.shape[1]
+    rng = np.random.default_rng(2024)
+    q = C.LineQubit.range(10)
+    others = np.array([[S.serialize_circuit(W.random_circuit(rng, q, 40) +
+                                            C.Circuit([C.I(x) for x in q]))
+                        for _ in range(2)] for _ in circuits], dtype=object)
+    ns = np.full((B, O_), shots, np.int32)
+    seed = 777
+    L = _lib.lib()
+    dev = torch.device("cuda", torch.cuda.current_device())
+    res = {}
+
+    def run(name, make, shape, dtype, check):
+        P, N, args = _plan_args(progs, pnames, pvalues)
+        plan = make(args)
+        out = torch.empty(shape, dtype=dtype, device=dev)
+        plan.upload(dev, stream)
+        elapsed, _, graph = time_graph(plan, out, stream, steps, warmup)
+        per = elapsed / steps
+        err = check(out.cpu().numpy())
+        res[name] = {"ms_per_launch": round(per * 1e3, 4), "circuits_per_s": round(B / per, 1),
+                     "max_abs_err_vs_oracle_2_rows": err}
+        log(f"{name}: {per * 1e3:.3f} ms per {B}-circuit launch, {B / per:.0f} circuits/s, "
+            f"err {err}")
+        del graph, plan
+
+    Ps = _lib.StringTensor(ps)
+    Ot = _lib.StringTensor(others)
+    two = list(progs[:2])
+    run("expectation",
+        lambda a: _lib.Plan(L.tfqx_plan_simulate_expectation, *a, Ps.ptr, Ps.off, O_),
+        (B, O_), torch.float32,
+        lambda r: float(np.abs(r[:2] - O.simulate_expectation(two, names, values[:2], ps[:2])).max()))
+    run("state", lambda a: _lib.Plan(L.tfqx_plan_simulate_state, *a), (B, 1024), torch.complex64,
+        lambda r: float(np.abs(r[:2] - O.simulate_state(two, names, values[:2])).max()))
+    run("inner_product",
+        lambda a: _lib.Plan(L.tfqx_plan_inner_product, *a, Ot.ptr, Ot.off, 2), (B, 2),
+        torch.complex64,
+        lambda r: float(np.abs(r[:2] - O.inner_product(two, names, values[:2], others[:2])).max()))
+    run("samples",
+        lambda a: _lib.Plan(L.tfqx_plan_simulate_samples, *a, shots, seed), (B, shots, 10),
+        torch.int8,
+        lambda r: float(np.mean(r[:2] != O.simulate_samples(two, names, values[:2], shots, seed))))
+    run("sampled_expectation",
+        lambda a: _lib.Plan(L.tfqx_plan_simulate_sampled_expectation, *a, Ps.ptr, Ps.off, O_,
+                            ns.ctypes.data, seed), (B, O_), torch.float32,
+        lambda r: float(np.abs(r[:2] - O.simulate_sampled_expectation(
+            two, names, values[:2], ps[:2], ns[:2], seed)).max()))
+    res["workload"] = ("64 circuits x 10 qubits (20 layers rx ry rz + CZ ladder, 600 symbols); "
+                       "observables [sum Z_i Z_i+1, X_0]; inner product: 2 paired programs; "
+                       f"samples / sampled expectation: {shots} shots (samples: fraction of "
+                       "differing bits vs the oracle's replica)")
+    return res
+
+
 def noisy_measure(stream, steps=10, warmup=2, trajectories=2000, cpu_rows=8, cpu_threads=16,
                   check=True):
     """tfq.noise expectation on 32 noisy 8-qubit circuits (8 layers of ry + CZ
@@ -297,7 +367,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the autotuner (the --only profiling runs never tune: they use "
                          "the planner defaults)")
-    ap.add_argument("--only", choices=["all", "batch", "hbm", "copy", "ipgrad", "noisy"], default="all",
+    ap.add_argument("--only", choices=["all", "batch", "hbm", "copy", "ipgrad", "noisy", "ops"],
+                    default="all",
                     help="profiling runs: 'batch' launches only the batched kernel (graph "
                          "replays, no plain launches, no CPU baseline); 'hbm' only the 30-qubit "
                          "streaming passes; 'copy' a 1 GiB device copy (counter calibration); "
@@ -330,6 +401,9 @@ def main():
         return
     if args.only == "ipgrad":
         print(json.dumps({"inner_product_grad": inner_product_grad_measure(stream)}), flush=True)
+        return
+    if args.only == "ops":
+        print(json.dumps({"ops": ops_measure(stream)}), flush=True)
         return
     if args.only == "noisy":
         print(json.dumps({"noisy": noisy_measure(stream, steps=args.steps, warmup=args.warmup)}),
@@ -404,6 +478,7 @@ def main():
             AT.apply(batch_setting)
     ipg = inner_product_grad_measure(stream)
     noisy = noisy_measure(stream)
+    other_ops = ops_measure(stream)
 
     done, cpu_dt, cores, err, checked, err_cpu = cpu_baseline(
         progs, names, values, ps, weights, gpu_exp, gpu_grad, args.cpu_sample, args.oracle_check)
@@ -439,6 +514,7 @@ def main():
         "hbm_pass": hbm,
         "inner_product_grad": ipg,
         "noisy": noisy,
+        "ops": other_ops,
         "cpu_baseline": {"value": round(done / cpu_dt, 3), "unit": "circuits/s", "cores": cores,
                          "kind": "port",
                          "sample": f"first {done} circuits of the same batch, expectation (basic "

@@ -324,6 +324,13 @@ int tfqx_gemm_bf16(int32_t M, int32_t N, int32_t K, const void* A, int32_t lda, 
  * 14 and 13).  No reference counterpart (qsim's CPU path has no tiles). */
 void tfqx_set_tuning(int32_t register_engine, int32_t tile_single, int32_t tile_two);
 
+/* Smallest number of 4-10 qubit circuits in a batch for which the planner
+ * runs them one per wavefront (expectation and adjoint-gradient plans);
+ * smaller batches run every circuit on a whole workgroup.  Negative restores
+ * the default, 1024 (four wavefronts per CU); 0 always uses wavefronts.
+ * No reference counterpart. */
+void tfqx_set_wave_min_circuits(int32_t count);
+
 /* Fusion width (qubits per fused block) the planner uses; 0 = autotuned. */
 void tfqx_set_fusion_width(int32_t width);
 const char* tfqx_version(void);

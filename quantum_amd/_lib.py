@@ -97,6 +97,8 @@ def _declare(lib):
     lib.tfqx_plan_destroy.restype = None
     lib.tfqx_set_tuning.argtypes = [i32, i32, i32]
     lib.tfqx_set_tuning.restype = None
+    lib.tfqx_set_wave_min_circuits.argtypes = [i32]
+    lib.tfqx_set_wave_min_circuits.restype = None
     lib.tfqx_set_fusion_width.argtypes = [i32]
     lib.tfqx_set_fusion_width.restype = None
     lib.tfqx_version.restype = c.c_char_p
@@ -139,7 +141,7 @@ EXPORTED_SYMBOLS = (
     "tfqx_plan_max_num_qubits", "tfqx_plan_workspace_bytes",
     "tfqx_plan_output_bytes", "tfqx_plan_upload", "tfqx_plan_launch", "tfqx_plan_launch_plain",
     "tfqx_graph_capture", "tfqx_graph_launch", "tfqx_graph_destroy", "tfqx_plan_destroy",
-    "tfqx_set_fusion_width", "tfqx_set_tuning", "tfqx_version", "tfqx_mlp_layout", "tfqx_mlp_set_params",
+    "tfqx_set_fusion_width", "tfqx_set_wave_min_circuits", "tfqx_set_tuning", "tfqx_version", "tfqx_mlp_layout", "tfqx_mlp_set_params",
     "tfqx_mlp_predict", "tfqx_mlp_loss_grad", "tfqx_mlp_train_step", "tfqx_gemm_bf16")
 
 
@@ -282,6 +284,12 @@ def fusion_decisions(program, symbol_names=(), symbol_values=(), width=2):
         bits = tuple(int(b) for b in r[1:1 + r[0]])
         out.append((bits, bool(r[5]), tuple(int(g) for g in gates[r[6]:r[7]])))
     return out
+
+
+def set_wave_min_circuits(count=-1):
+    """Smallest count of 4-10 qubit circuits run one per wavefront (negative:
+    the default 1024; 0: always)."""
+    lib().tfqx_set_wave_min_circuits(int(count))
 
 
 def set_tuning(register_engine=2, tile_single=0, tile_two=0):

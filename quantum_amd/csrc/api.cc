@@ -266,6 +266,8 @@ void Layout(tfqx_plan* p) {
   put(p->off_passes, g.passes.data(), g.passes.size() * sizeof(DevPass));
 }
 
+std::atomic<int> g_wave_min{kWaveMinCircuits};
+
 int BuildPlan(int mode, Inputs& in, int64_t B, int64_t S, int64_t O, int64_t K,
               const float* downstream, bool tfq_fd, tfqx_plan** out, char* err, int64_t len,
               bool with_expectation = false) {
@@ -287,7 +289,14 @@ int BuildPlan(int mode, Inputs& in, int64_t B, int64_t S, int64_t O, int64_t K,
   const bool tiled_ok = (mode == kModeExpectation || mode == kModeAdjoint || mode == kModeState) &&
                         p->max_nq <= kTiledMaxQubits;
   opt.tile_log2 = resident;
-  opt.wave = mode == kModeExpectation || mode == kModeAdjoint;
+  // Wave mode gives each small circuit one wavefront.  It pays when there
+  // are enough such circuits to occupy the chip (256 CUs x 4 SIMDs); fewer
+  // run faster as whole workgroups, 8 wavefronts per circuit.
+  int wave_eligible = 0;
+  for (const auto& c : in.circuits)
+    wave_eligible += !c.gates.empty() && c.num_qubits >= kWaveRegBits && c.num_qubits <= kWaveMaxQubits;
+  if (in.shared) wave_eligible *= int(B);
+  opt.wave = (mode == kModeExpectation || mode == kModeAdjoint) && wave_eligible >= g_wave_min.load();
   if (p->max_nq > resident && !tiled_ok) {
     p->large = true;
     Status s = BuildLargePlan(mode, in.circuits, in.sums, downstream, in.others, B, S, O, K,
@@ -465,6 +474,10 @@ extern "C" {
 const char* tfqx_version(void) { return "tfqx 0.1 (gfx950)"; }
 
 void tfqx_set_fusion_width(int32_t width) { g_fusion_width.store(width); }
+
+void tfqx_set_wave_min_circuits(int32_t count) {
+  g_wave_min.store(count < 0 ? kWaveMinCircuits : int(count));
+}
 
 void tfqx_set_tuning(int32_t register_engine, int32_t tile_single, int32_t tile_two) {
   Tuning t;

@@ -52,6 +52,8 @@ constexpr int RegBitsForTile(int L) { return (L >= 12 && L <= 13) ? L - kRegThre
 // registers, kWaveRegBits register qubits per lane and the rest on lane bits.
 constexpr int kWaveRegBits = 4;
 constexpr int kWaveMaxQubits = 10;
+// Batches with fewer wave-eligible circuits than this run them as workgroups.
+constexpr int kWaveMinCircuits = 1024;
 
 // One op on a tile.  Qubit positions are tile-local bit indices (ascending).
 // Matrix layout: row-major 2^k x 2^k complex64, index bit j <-> q[j].

@@ -403,19 +403,27 @@ def test_register_engine_levels_match_oracle(streaming_case, tiled_case, level, 
 # Wave mode: circuits of 4..10 qubits run one per wavefront after the
 # workgroup-sized ones; 4 qubits leaves every lane a replica of lane 0, 10 uses
 # all 64 lanes.  Controls exercise the predicated register paths.
+# The planner uses wave mode only for batches with many such circuits; the
+# test lowers that threshold to 0 (always) and also runs the same small batch
+# on whole workgroups (the default for it).
 @pytest.mark.parametrize("widths", [[4, 5, 10, 14, 7, 13], [4, 6, 8, 10]])
-def test_wave_mode_batches(widths):
+@pytest.mark.parametrize("wave_min", [0, -1])
+def test_wave_mode_batches(widths, wave_min):
     sim, adj, _ = ops()
     circuits, names, vals, psums = _mixed_batch(803 + len(widths), widths)
     progs, ps = _ser(circuits), _ser(psums)
     g = np.random.default_rng(7).uniform(-1, 1, size=(len(circuits), 2)).astype(np.float32)
-    _say(f"wave mode widths {widths}")
-    exp, grad = adj.expectation_and_gradient(progs, names, vals, ps, g)
+    _say(f"wave mode widths {widths}, wave_min {wave_min}")
+    _lib.set_wave_min_circuits(wave_min)
+    try:
+        exp, grad = adj.expectation_and_gradient(progs, names, vals, ps, g)
+        out = _np(sim.tfq_simulate_expectation(progs, names, vals, ps))
+    finally:
+        _lib.set_wave_min_circuits()
     ref_e = O.simulate_expectation(list(progs), names, vals, ps)
     ref_g = O.adjoint_gradient(list(progs), names, vals, ps, g)
     np.testing.assert_allclose(_np(exp), ref_e, atol=TOL * max(1, np.abs(ref_e).max()))
     np.testing.assert_allclose(_np(grad), ref_g, atol=TOL * max(1, np.abs(ref_g).max()))
-    out = _np(sim.tfq_simulate_expectation(progs, names, vals, ps))
     np.testing.assert_allclose(out, ref_e, atol=TOL * max(1, np.abs(ref_e).max()))
 
 
@@ -429,7 +437,11 @@ def test_plain_launch_matches_batched_launch():
     g = np.random.default_rng(9).uniform(-1, 1, size=(len(circuits), 2)).astype(np.float32)
     dev = torch.device("cuda", 0)
     stream = torch.cuda.current_stream().cuda_stream
-    plan, shape = A.plan_adj_grad(progs, names, vals, ps, g, with_expectation=True)
+    _lib.set_wave_min_circuits(0)
+    try:
+        plan, shape = A.plan_adj_grad(progs, names, vals, ps, g, with_expectation=True)
+    finally:
+        _lib.set_wave_min_circuits()
     out = torch.empty(shape, dtype=torch.float32, device=dev)
     out_plain = torch.empty_like(out)
     plan.upload(dev, stream)

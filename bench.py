@@ -183,7 +183,7 @@ def inner_product_grad_measure(stream, steps=20, warmup=3, oracle_rows=2):
 
 
 def ops_measure(stream, steps=20, warmup=3, shots=1000):
-    """The other ops of the family on the 64 x 10-qubit rotation batch (600
+    """Each op of the family on the 64 x 10-qubit rotation batch (600
     symbols, observables [sum Z_i Z_i+1, X_0]): one launch per step from a
     hipGraph, with 2 rows checked against the oracle (sampled ops against the
     oracle's replica of the same Philox stream)."""
@@ -230,6 +230,13 @@ def ops_measure(stream, steps=20, warmup=3, shots=1000):
         lambda a: _lib.Plan(L.tfqx_plan_simulate_expectation, *a, Ps.ptr, Ps.off, O_),
         (B, O_), torch.float32,
         lambda r: float(np.abs(r[:2] - O.simulate_expectation(two, names, values[:2], ps[:2])).max()))
+    from quantum_amd import tfq_adj_grad_op as A
+    g = np.random.default_rng(2025).uniform(-1, 1, size=(B, O_)).astype(np.float32)
+    run("adjoint_gradient",
+        lambda a: A.plan_adj_grad(progs, pnames, pvalues, ps, g)[0], (B, len(names)),
+        torch.float32,
+        lambda r: float(np.abs(r[:2] - O.adjoint_gradient(two, names, values[:2], ps[:2],
+                                                          g[:2])).max()))
     run("state", lambda a: _lib.Plan(L.tfqx_plan_simulate_state, *a), (B, 1024), torch.complex64,
         lambda r: float(np.abs(r[:2] - O.simulate_state(two, names, values[:2])).max()))
     run("inner_product",
@@ -246,7 +253,8 @@ def ops_measure(stream, steps=20, warmup=3, shots=1000):
         lambda r: float(np.abs(r[:2] - O.simulate_sampled_expectation(
             two, names, values[:2], ps[:2], ns[:2], seed)).max()))
     res["workload"] = ("64 circuits x 10 qubits (20 layers rx ry rz + CZ ladder, 600 symbols); "
-                       "observables [sum Z_i Z_i+1, X_0]; inner product: 2 paired programs; "
+                       "observables [sum Z_i Z_i+1, X_0]; adjoint gradient: 600 symbols with "
+                       "random upstream weights; inner product: 2 paired programs; "
                        f"samples / sampled expectation: {shots} shots (samples: fraction of "
                        "differing bits vs the oracle's replica)")
     return res

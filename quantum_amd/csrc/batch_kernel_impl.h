@@ -131,27 +131,29 @@ __device__ __forceinline__ void build_lambda(const float2* __restrict__ S, float
 
 // CPLX = false: gacc[col] += 2 Re<dG psi | lam> (adjoint gradient).
 // CPLX = true: (gacc[2 col], gacc[2 col + 1]) += <dG psi | lam> (inner-product
-// gradient).
+// gradient).  Each wavefront adds its own partial with an FP64 atomic, so the
+// step needs no workgroup barrier beyond the one that follows every op.
 template <bool CPLX>
 __device__ __forceinline__ void param_step(float2* A, float2* B, int n, const DevOp& op,
-                                           const float2* M, double* gacc, double* red) {
+                                           const float2* M, double* gacc) {
   double acc[2 * kMaxDeriv] = {0.0, 0.0, 0.0, 0.0};
   switch (op.k) {
     case 1: apply_param<1>(A, B, n, op, M, acc); break;
     default: apply_param<2>(A, B, n, op, M, acc); break;
   }
+  const bool lead = (threadIdx.x & 63) == 0;
 #pragma unroll
   for (int d = 0; d < kMaxDeriv; ++d) {
     if (d >= op.nderiv) break;
-    const double tot = block_sum(acc[d], red);
+    const double tot = wave_sum(acc[d]);
     if (CPLX) {
-      const double tim = block_sum(acc[kMaxDeriv + d], red);
-      if (threadIdx.x == 0) {
-        gacc[2 * op.col[d]] += tot;
-        gacc[2 * op.col[d] + 1] += tim;
+      const double tim = wave_sum(acc[kMaxDeriv + d]);
+      if (lead) {
+        atomicAdd(&gacc[2 * op.col[d]], tot);
+        atomicAdd(&gacc[2 * op.col[d] + 1], tim);
       }
-    } else if (threadIdx.x == 0) {
-      gacc[op.col[d]] += 2.0 * tot;
+    } else if (lead) {
+      atomicAdd(&gacc[op.col[d]], 2.0 * tot);
     }
   }
 }
@@ -171,7 +173,7 @@ __device__ __forceinline__ void run_reverse(float2* A, float2* B, int n, const D
   if (reg_run_any<true>(A, B, n, ops, b, e, mc, acc)) return;
   for_ops(ops, b, e, mats, mbuf, [&](const DevOp& op, const float2* M) {
     if (op.type == kOpParam) {
-      param_step<CPLX>(A, B, n, op, M, gacc, red);
+      param_step<CPLX>(A, B, n, op, M, gacc);
     } else {
       apply_op<true, KMAX>(A, n, op, M);
       apply_op<true, KMAX>(B, n, op, M);
@@ -450,6 +452,7 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(BatchArgs a) {
     if (MODE == kModeAdjoint) {
       double* gacc = a.grad_acc + size_t(row) * a.nsym;
       for (int j = tid; j < a.nsym; j += kThreads) gacc[j] = 0.0;
+      __threadfence();  // the zeros reach L2 before any wavefront's atomics
       const int lb = uniform(c.lam_begin), le = uniform(c.lam_end);
       if (tiled) {
         build_lambda(gpsi, glam, N, a, lb, le);
@@ -474,6 +477,7 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(BatchArgs a) {
       // Adjoint sweep with complex accumulation (tfq_inner_product_grad.cc:259-314).
       double* gacc = a.grad_acc + size_t(row) * a.nsym * 2;
       for (int j = tid; j < 2 * a.nsym; j += kThreads) gacc[j] = 0.0;
+      __threadfence();  // the zeros reach L2 before any wavefront's atomics
       __syncthreads();
       run_reverse<KMAX, true>(psi, buf2, n, a.ops, uniform(c.rev_begin), uniform(c.rev_end), a.mats,
                               mbuf, gacc, red, mc);

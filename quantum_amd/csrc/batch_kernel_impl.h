@@ -34,20 +34,38 @@ struct NoAcc {
   __device__ void operator()(int, int, double, double) const {}
 };
 
+// Run ops [b, e): f(op, M) with M the op's matrices (unitary or diagonal, then
+// its derivatives, back to back) in the workgroup's LDS matrix window.  The
+// window is refilled in one coalesced block load when an op's matrices fall
+// outside it, so most ops start without waiting on global memory; the next
+// op's descriptor is read while the current one runs.  Every op ends with a
+// workgroup barrier.
+template <typename F>
+__device__ __forceinline__ void for_ops_cached(const DevOp* ops, int b, int e, MatCache& mc, F&& f) {
+  if (b >= e) return;
+  __syncthreads();
+  DevOp nx = load_op(ops + b);
+  for (int i = b; i < e; ++i) {
+    DevOp op = nx;
+    if (i + 1 < e) nx = load_op(ops + i + 1);
+    const float2* W = mats_for(mc, op);
+    f(op, W + op.mat);
+    __syncthreads();
+  }
+}
+
 // Forward ops [b, e) on an LDS tile of n qubits (register engine when planned).
 template <int KMAX>
-__device__ __forceinline__ void run_ops(float2* s, int n, const DevOp* ops, int b, int e, const float2* mats,
-                        float2* mbuf, MatCache& mc) {
+__device__ __forceinline__ void run_ops(float2* s, int n, const DevOp* ops, int b, int e, MatCache& mc) {
   if (reg_run_any<false>(s, s, n, ops, b, e, mc, NoAcc())) return;
-  for_ops(ops, b, e, mats, mbuf,
-          [&](const DevOp& op, const float2* M) { apply_op<false, KMAX>(s, n, op, M); });
+  for_ops_cached(ops, b, e, mc,
+                 [&](const DevOp& op, const float2* M) { apply_op<false, KMAX>(s, n, op, M); });
 }
 
 // Adjoint sweep of ops [b, e) on psi (A) and lambda (B), gradients into gacc.
 template <int KMAX, bool CPLX = false>
 __device__ __forceinline__ void run_reverse(float2* A, float2* B, int n, const DevOp* ops, int b, int e,
-                            const float2* mats, float2* mbuf, double* gacc, double* red,
-                            MatCache& mc);
+                                            double* gacc, MatCache& mc);
 
 __device__ __forceinline__ void set_zero_state(float2* s, uint32_t N) {
   for (uint32_t x = threadIdx.x; x < N; x += kThreads) s[x] = make_float2(x == 0 ? 1.f : 0.f, 0.f);
@@ -160,8 +178,7 @@ __device__ __forceinline__ void param_step(float2* A, float2* B, int n, const De
 
 template <int KMAX, bool CPLX>
 __device__ __forceinline__ void run_reverse(float2* A, float2* B, int n, const DevOp* ops, int b, int e,
-                            const float2* mats, float2* mbuf, double* gacc, double* red,
-                            MatCache& mc) {
+                                            double* gacc, MatCache& mc) {
   auto acc = [gacc](int, int col, double w, double v) {
     if (CPLX) {
       atomicAdd(&gacc[2 * col], w);
@@ -171,7 +188,7 @@ __device__ __forceinline__ void run_reverse(float2* A, float2* B, int n, const D
     }
   };
   if (reg_run_any<true>(A, B, n, ops, b, e, mc, acc)) return;
-  for_ops(ops, b, e, mats, mbuf, [&](const DevOp& op, const float2* M) {
+  for_ops_cached(ops, b, e, mc, [&](const DevOp& op, const float2* M) {
     if (op.type == kOpParam) {
       param_step<CPLX>(A, B, n, op, M, gacc);
     } else {
@@ -273,7 +290,7 @@ __device__ __forceinline__ void wave_circuit(const BatchArgs& a, int ci, float2*
 // tiles through LDS buffer A.  The first pass starts from |0>.
 template <int KMAX>
 __device__ __forceinline__ void tiled_forward(float2* G, int n, const BatchArgs& a, int pb, int pe, float2* A,
-                              float2* mbuf, uint64_t* tab, MatCache& mc) {
+                              uint64_t* tab, MatCache& mc) {
   const int tid = threadIdx.x;
   bool first = true;
   for (int p = pb; p < pe; ++p) {
@@ -307,7 +324,7 @@ __device__ __forceinline__ void tiled_forward(float2* G, int n, const BatchArgs&
         tile_gather(A, G, T, gidx);
       }
       __syncthreads();
-      run_ops<KMAX>(A, ps.L, a.ops, ps.op_begin, ps.op_end, a.mats, mbuf, mc);
+      run_ops<KMAX>(A, ps.L, a.ops, ps.op_begin, ps.op_end, mc);
       tile_scatter(G, A, T, gidx);
       __syncthreads();
     }
@@ -325,8 +342,8 @@ __device__ __forceinline__ void tiled_forward(float2* G, int n, const BatchArgs&
 // Tiled adjoint sweep on psi (G) and lambda (Lg) with LDS tiles A and B.
 template <int KMAX>
 __device__ __forceinline__ void tiled_reverse(float2* G, float2* Lg, int n, const BatchArgs& a, int pb, int pe,
-                              float2* A, float2* B, double* gacc, double* red, float2* mbuf,
-                              uint64_t* tab, MatCache& mc) {
+                              float2* A, float2* B, double* gacc, uint64_t* tab,
+                              MatCache& mc) {
   const int tid = threadIdx.x;
   for (int p = pb; p < pe; ++p) {
     const DevPass ps = a.passes[p];
@@ -356,7 +373,7 @@ __device__ __forceinline__ void tiled_reverse(float2* G, float2* Lg, int n, cons
       if (reg_run_any<true>(A, B, ps.L, a.ops, ps.op_begin, ps.op_end, mc, acc, io)) continue;
       tile_gather2(A, B, G, Lg, T, gidx);
       __syncthreads();
-      run_reverse<KMAX>(A, B, ps.L, a.ops, ps.op_begin, ps.op_end, a.mats, mbuf, gacc, red, mc);
+      run_reverse<KMAX>(A, B, ps.L, a.ops, ps.op_begin, ps.op_end, gacc, mc);
       tile_scatter(G, A, T, gidx);
       tile_scatter(Lg, B, T, gidx);
       __syncthreads();
@@ -370,7 +387,6 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(BatchArgs a) {
   extern __shared__ float2 lds[];
   __shared__ double red[kWaves];
   __shared__ int s_ci;
-  __shared__ float2 mbuf[2 * kMatSlot];
   __shared__ uint64_t tab[kTileTable];
   __shared__ float2 mcache[kMatCache];
   MatCache mc = mat_cache(mcache, a.mats);
@@ -427,7 +443,7 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(BatchArgs a) {
       for (int o = uniform(c.other_begin); o < c.other_end; ++o) {
         const DevOther ot = a.others[o];
         set_zero_state(psi, N);
-        run_ops<KMAX>(psi, n, a.ops, uniform(ot.op_begin), uniform(ot.op_end), a.mats, mbuf, mc);
+        run_ops<KMAX>(psi, n, a.ops, uniform(ot.op_begin), uniform(ot.op_end), mc);
         const float w = ot.weight;
         for (uint32_t x = tid; x < N; x += kThreads) {
           const float2 p = psi[x];
@@ -437,11 +453,11 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(BatchArgs a) {
       }
     }
     if (tiled) {
-      tiled_forward<KMAX>(gpsi, n, a, uniform(c.fwd_pass_begin), uniform(c.fwd_pass_end), psi, mbuf,
+      tiled_forward<KMAX>(gpsi, n, a, uniform(c.fwd_pass_begin), uniform(c.fwd_pass_end), psi,
                     tab, mc);
     } else {
       set_zero_state(psi, N);
-      run_ops<KMAX>(psi, n, a.ops, uniform(c.fwd_begin), uniform(c.fwd_end), a.mats, mbuf, mc);
+      run_ops<KMAX>(psi, n, a.ops, uniform(c.fwd_begin), uniform(c.fwd_end), mc);
     }
 
     if (MODE == kModeExpectation || (MODE == kModeAdjoint && a.exp_out)) {
@@ -457,15 +473,13 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(BatchArgs a) {
       if (tiled) {
         build_lambda(gpsi, glam, N, a, lb, le);
         tiled_reverse<KMAX>(gpsi, glam, n, a, uniform(c.rev_pass_begin), uniform(c.rev_pass_end), psi,
-                      buf2, gacc, red, mbuf, tab, mc);
+                      buf2, gacc, tab, mc);
       } else {
         float2* lam = buf2;
         build_lambda(psi, lam, N, a, lb, le);
-        run_reverse<KMAX>(psi, lam, n, a.ops, uniform(c.rev_begin), uniform(c.rev_end), a.mats,
-                          mbuf, gacc, red, mc);
+        run_reverse<KMAX>(psi, lam, n, a.ops, uniform(c.rev_begin), uniform(c.rev_end), gacc, mc);
       }
-      // Gradients arrive by atomics (register engine) or plain stores; read them
-      // back from L2.
+      // Gradients arrive by atomics; read them back from L2.
       __threadfence();
       __syncthreads();
       for (int j = tid; j < a.nsym; j += kThreads)
@@ -479,8 +493,8 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(BatchArgs a) {
       for (int j = tid; j < 2 * a.nsym; j += kThreads) gacc[j] = 0.0;
       __threadfence();  // the zeros reach L2 before any wavefront's atomics
       __syncthreads();
-      run_reverse<KMAX, true>(psi, buf2, n, a.ops, uniform(c.rev_begin), uniform(c.rev_end), a.mats,
-                              mbuf, gacc, red, mc);
+      run_reverse<KMAX, true>(psi, buf2, n, a.ops, uniform(c.rev_begin), uniform(c.rev_end), gacc,
+                              mc);
       __threadfence();
       __syncthreads();
       for (int j = tid; j < a.nsym; j += kThreads)
@@ -554,7 +568,7 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(BatchArgs a) {
       for (int o = uniform(c.other_begin); o < c.other_end; ++o) {
         const DevOther ot = a.others[o];
         set_zero_state(phi, N);
-        run_ops<KMAX>(phi, n, a.ops, uniform(ot.op_begin), uniform(ot.op_end), a.mats, mbuf, mc);
+        run_ops<KMAX>(phi, n, a.ops, uniform(ot.op_begin), uniform(ot.op_end), mc);
         double re = 0.0, im = 0.0;
         for (uint32_t x = tid; x < N; x += kThreads) {
           const float2 p = cmul_conj_a(psi[x], phi[x]);

@@ -115,8 +115,33 @@ __device__ __forceinline__ void for_ops(const DevOp* ops, int b, int e, const fl
   }
 }
 
+// v + (v of the lane that DPP control CTRL selects within the lane's row of 16).
+template <int CTRL>
+__device__ __forceinline__ double dpp_add(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return v + __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_value(double v, int lane) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                          __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+// Sum over the 64 lanes of a wavefront (all lanes active), returned in every
+// lane.  Within each row of 16 lanes: DPP quad permutations (lane ^ 1, lane ^
+// 2), then the half-row mirror (lane 7 - i) and the row mirror (lane 15 - i),
+// which pair each lane with lanes holding the other partial; the four row
+// sums are then read lane by lane.  All steps are VALU operations, with no LDS
+// round trip.
+__device__ __forceinline__ double wave_sum(double v) {
+  v = dpp_add<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+  v = dpp_add<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+  v = dpp_add<0x141>(v);  // row_half_mirror
+  v = dpp_add<0x140>(v);  // row_mirror
+  return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
+}
+
 __device__ __forceinline__ double block_sum(double v, double* red) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  v = wave_sum(v);
   __syncthreads();
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();

@@ -30,8 +30,9 @@
 
 namespace tfqx {
 
-// Widest noisy program: a workgroup trajectory keeps its state in LDS.
-constexpr int kNoisyMaxQubits = 13;
+// Widest noisy program: a workgroup trajectory keeps its state in LDS
+// (2^14 complex64 amplitudes = 128 KiB of gfx950's 160 KiB).
+constexpr int kNoisyMaxQubits = 14;
 // Random stream of the terminal measurement (samples) and of the one-shot
 // term measurements (sampled expectation, ORed with the term's index).
 constexpr uint32_t kStreamSample = 0xFFFFFFFFu;

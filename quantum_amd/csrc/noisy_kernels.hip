@@ -421,10 +421,13 @@ int NoisyLaunch(NoisyPlan* p, void* ws, void* output, hipStream_t stream, char* 
                                : size_t(p->B) * size_t(p->O);
   if (out_count == 0) return 0;
   if (a.nchunks > 0) {
-    // Two workgroups per CU (64 KiB of dynamic LDS each): 8 wavefronts of
-    // 2^10-amplitude buffers, or one 2^13-amplitude workgroup state.
+    // Dynamic LDS: 8 wavefronts of 2^10-amplitude buffers (64 KiB, two
+    // workgroups per CU), or the widest workgroup state when that is larger
+    // (a 14-qubit state takes 128 KiB, one workgroup per CU; workgroups that
+    // find no room start when others end and then find the queues empty).
     const int grid = std::max(1, std::min(512, a.nchunks));
-    const size_t lds = size_t(1) << (kNoisyMaxQubits + 3);
+    const int lds_log2 = std::max(kWaveMaxQubits + 3, std::min(p->max_nq, kNoisyMaxQubits));
+    const size_t lds = (size_t(1) << lds_log2) * sizeof(float2);
     switch (p->mode) {
       case kModeNoisyExpectation:
         hipLaunchKernelGGL(noisy_kernel<kModeNoisyExpectation>, dim3(grid), dim3(kThreads), lds, stream, a);

@@ -168,6 +168,11 @@ void NoisyLayout(NoisyPlan* p) {
   // leaves them so.
   p->off_nacc = off; off = Align(off + std::max<size_t>(1, p->ident.size()) * sizeof(double));
   p->image = off;
+  if (p->max_nq > kNoisyLdsMaxQubits) {
+    p->state_log2 = p->max_nq;
+    p->off_states = off;
+    off = Align(off + size_t(kNoisyGrid) * (size_t(1) << p->state_log2) * sizeof(cfloat));
+  }
   p->total = off;
   p->staging.assign(p->image, 0);
   auto put = [&](size_t o, const void* src, size_t n) {

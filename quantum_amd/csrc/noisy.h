@@ -10,8 +10,11 @@
 //     engine holds 2^(kWaveMaxQubits - n) trajectories per wavefront, one per
 //     group of 2^(n - kWaveRegBits) lanes (circuits of fewer than
 //     kWaveRegBits qubits get idle qubits added above their own);
-//   * circuits of kWaveMaxQubits+1 .. kNoisyMaxQubits qubits run one
-//     trajectory per workgroup with the state in LDS.
+//   * circuits of kWaveMaxQubits+1 .. kNoisyLdsMaxQubits qubits run one
+//     trajectory per workgroup with the state in LDS;
+//   * circuits of kNoisyLdsMaxQubits+1 .. kNoisyMaxQubits qubits run one
+//     trajectory per workgroup with the state in a per-workgroup buffer of
+//     the plan's workspace (kNoisyGrid buffers of 2^max_nq amplitudes).
 //
 // Channel c of trajectory t of output row i draws the uniform variate
 // philox(seed, shot = t, stream = (i, c)); results depend on the seed only,
@@ -30,9 +33,14 @@
 
 namespace tfqx {
 
-// Widest noisy program: a workgroup trajectory keeps its state in LDS
+// Widest noisy program whose workgroup trajectories keep the state in LDS
 // (2^14 complex64 amplitudes = 128 KiB of gfx950's 160 KiB).
-constexpr int kNoisyMaxQubits = 14;
+constexpr int kNoisyLdsMaxQubits = 14;
+// Widest noisy program: 20 qubits, 8 MiB per workgroup state, 4 GiB of
+// workspace for the kNoisyGrid workgroups.
+constexpr int kNoisyMaxQubits = 20;
+// Workgroups of the noisy kernel's launch (at most).
+constexpr int kNoisyGrid = 512;
 // Random stream of the terminal measurement (samples) and of the one-shot
 // term measurements (sampled expectation, ORed with the term's index).
 constexpr uint32_t kStreamSample = 0xFFFFFFFFu;
@@ -57,6 +65,10 @@ struct NoisyPlan {
   // Device image layout (byte offsets).
   size_t off_counters = 0, off_circuits = 0, off_ops = 0, off_mats = 0, off_terms = 0,
          off_chunks = 0, off_ident = 0, off_nsamp = 0, off_empty = 0, off_nacc = 0, total = 0;
+  // Workgroup state buffers (programs wider than kNoisyLdsMaxQubits only):
+  // kNoisyGrid buffers of 2^state_log2 amplitudes at off_states, else 0.
+  size_t off_states = 0;
+  int state_log2 = 0;
   size_t image = 0;
   std::vector<uint8_t> staging;
 };

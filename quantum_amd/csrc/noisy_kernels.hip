@@ -33,6 +33,8 @@ struct NoisyArgs {
   const float* ident;
   const int32_t* row_empty;
   double* nacc;
+  float2* states;  // per-workgroup states of programs wider than LDS (or null)
+  int state_log2;
   signed char* samples_out;
   float* exp_out;
   unsigned long long seed;
@@ -304,7 +306,7 @@ __device__ __forceinline__ void wave_chunk(const NoisyArgs& a, const DevChunk& c
   esync<true>();
 }
 
-static_assert(kLdsMaxQubitsOneBuffer >= kNoisyMaxQubits, "workgroup trajectories keep the state in LDS");
+static_assert(kLdsMaxQubitsOneBuffer >= kNoisyLdsMaxQubits, "workgroup trajectories keep the state in LDS");
 static_assert(kLdsMaxQubitsOneBuffer >= kWaveMaxQubits + 3,
               "wave trajectories need one 2^kWaveMaxQubits buffer per wavefront (8 wavefronts)");
 
@@ -322,7 +324,13 @@ __global__ __launch_bounds__(kThreads, 4) void noisy_kernel(NoisyArgs a) {
     const int ci = uniform(s_ci);
     __syncthreads();
     if (ci >= a.nblock) break;
-    wg_trajectory<MODE>(a, a.chunks[ci], lds, mbuf, red);
+    const DevChunk ch = a.chunks[ci];
+    // Programs wider than LDS use this workgroup's own buffer in the
+    // workspace; barriers order its global accesses as they do LDS ones.
+    float2* psi = uniform(ch.nq) > kNoisyLdsMaxQubits
+                      ? a.states + (size_t(blockIdx.x) << a.state_log2)
+                      : lds;
+    wg_trajectory<MODE>(a, ch, psi, mbuf, red);
   }
   // Wave trajectories: each wavefront pulls chunks of its own.
   if (a.nblock < a.nchunks) {
@@ -402,6 +410,8 @@ int NoisyLaunch(NoisyPlan* p, void* ws, void* output, hipStream_t stream, char* 
   a.num_samples = reinterpret_cast<const int32_t*>(base + p->off_nsamp);
   a.row_empty = reinterpret_cast<const int32_t*>(base + p->off_empty);
   a.nacc = reinterpret_cast<double*>(base + p->off_nacc);
+  a.states = p->state_log2 ? reinterpret_cast<float2*>(base + p->off_states) : nullptr;
+  a.state_log2 = p->state_log2;
   // Three queue counters, 64 bytes apart, inside the 256-byte counter block.
   int* counters = reinterpret_cast<int*>(base + p->off_counters);
   a.queue = counters;
@@ -425,8 +435,8 @@ int NoisyLaunch(NoisyPlan* p, void* ws, void* output, hipStream_t stream, char* 
     // workgroups per CU), or the widest workgroup state when that is larger
     // (a 14-qubit state takes 128 KiB, one workgroup per CU; workgroups that
     // find no room start when others end and then find the queues empty).
-    const int grid = std::max(1, std::min(512, a.nchunks));
-    const int lds_log2 = std::max(kWaveMaxQubits + 3, std::min(p->max_nq, kNoisyMaxQubits));
+    const int grid = std::max(1, std::min(kNoisyGrid, a.nchunks));
+    const int lds_log2 = std::max(kWaveMaxQubits + 3, std::min(p->max_nq, kNoisyLdsMaxQubits));
     const size_t lds = (size_t(1) << lds_log2) * sizeof(float2);
     switch (p->mode) {
       case kModeNoisyExpectation:

@@ -18,7 +18,7 @@ operator per channel at random; results average over trajectories:
 The reference draws a fresh random seed per call (``tensorflow::random::New64()``);
 here ``seed`` (default: a fresh one) fixes the counter-based Philox stream,
 so a call is reproducible.  All trajectories of a batch run in one persistent
-GPU launch (circuits of at most 14 qubits).
+GPU launch (circuits of at most 20 qubits).
 """
 
 import numpy as np

@@ -2,8 +2,9 @@
 
 * Replica parity: with the same seed the device's trajectories make the same
   Kraus choices as the oracle's replica (``noise_oracle.trajectory_*``), so
-  averages agree to float rounding; circuits of 2-14 qubits cover wave mode
-  (with idle qubits added below 4 qubits) and workgroup trajectories, mixed
+  averages agree to float rounding; circuits of 2-16 qubits cover wave mode
+  (with idle qubits added below 4 qubits) and workgroup trajectories (state
+  in LDS up to 14 qubits, in a workspace buffer beyond), mixed
   in one batch with an empty program and per-slot trajectory counts.
 * Statistical parity with the exact density matrix, as the reference's own
   tests check (``noisy_expectation_op_test.py:246-312``, atol 5e-2): random
@@ -57,7 +58,8 @@ def _batch(seed, widths, depth_per_qubit=3, p_channel=0.5, n_terms=2):
     return S.convert_to_tensor(circuits), names, vals, S.convert_to_tensor(sums)
 
 
-@pytest.mark.parametrize("widths", [(2, 3, 0), (4, 6), (5, 8, 0, 10), (11, 12), (14, 3)])
+@pytest.mark.parametrize("widths", [(2, 3, 0), (4, 6), (5, 8, 0, 10), (11, 12), (14, 3),
+                                    (16, 0, 9)])
 def test_noisy_expectation_matches_replica(widths):
     from quantum_amd import noise
     progs, names, vals, ps = _batch(100 + sum(widths), widths)
@@ -73,7 +75,7 @@ def test_noisy_expectation_matches_replica(widths):
             np.testing.assert_array_equal(out[i], -2.0)
 
 
-@pytest.mark.parametrize("widths", [(3, 6), (9, 0, 11)])
+@pytest.mark.parametrize("widths", [(3, 6), (9, 0, 11), (15, 5)])
 def test_noisy_sampled_expectation_matches_replica(widths):
     from quantum_amd import noise
     progs, names, vals, ps = _batch(200 + sum(widths), widths)
@@ -86,7 +88,7 @@ def test_noisy_sampled_expectation_matches_replica(widths):
     np.testing.assert_allclose(out, ref, atol=2e-4)
 
 
-@pytest.mark.parametrize("widths", [(2, 5, 0), (7, 10), (12, 4), (14,)])
+@pytest.mark.parametrize("widths", [(2, 5, 0), (7, 10), (12, 4), (14,), (15, 3)])
 def test_noisy_samples_match_replica(widths):
     from quantum_amd import noise
     progs, names, vals, _ = _batch(300 + sum(widths), widths)
@@ -131,7 +133,7 @@ def test_single_channel_statistics(chan):
     np.testing.assert_allclose(out, exact, atol=5e-2, rtol=5e-2)
 
 
-@pytest.mark.parametrize("n,batch", [(13, 1), (14, 2), (6, 25)])
+@pytest.mark.parametrize("n,batch", [(13, 1), (14, 2), (17, 1), (6, 25)])
 def test_channel_free_programs_equal_exact_expectation(n, batch):
     from quantum_amd import noise
     rng = np.random.default_rng(n * 7 + batch)
@@ -209,8 +211,8 @@ def test_noisy_empty_inputs_and_errors():
     from quantum_amd import tfq_simulate_ops as T
     with pytest.raises(InvalidArgumentError, match="Could not parse gate id: DP"):
         T.tfq_simulate_expectation(prog, [], np.zeros((1, 0)), ps)
-    wide = S.convert_to_tensor([C.Circuit([C.bit_flip(0.1).on(x) for x in C.LineQubit.range(15)])])
-    with pytest.raises(NotImplementedError, match="at most 14 qubits"):
+    wide = S.convert_to_tensor([C.Circuit([C.bit_flip(0.1).on(x) for x in C.LineQubit.range(21)])])
+    with pytest.raises(NotImplementedError, match="at most 20 qubits"):
         noise.samples(wide, [], np.zeros((1, 0)), [4])
 
 

@@ -220,6 +220,18 @@ int tfqx_debug_fuse(const uint8_t* program, int64_t program_len, const uint8_t* 
                     int64_t gates_cap, int64_t* n_ops, int64_t* n_gates, char* err,
                     int64_t err_len);
 
+/* The sampler's FP64 uniforms for shots [start, start + count) of stream
+ * (stream_lo, stream_hi) under `seed`, written to the device buffer `out`
+ * (count doubles) on `stream`: exactly the draws the samples and sampled-
+ * expectation kernels make (Philox4x32-10, counter = (shot, stream), key =
+ * seed, u = top 53 bits / 2^53).  No reference counterpart: TFQ seeds
+ * tensorflow::random::PhiloxRandom nondeterministically
+ * (core/ops/tfq_simulate_samples_op.cc:149-152); this exposes the fixed
+ * stream so a host replica can check it bit for bit. */
+int tfqx_debug_philox_uniform(uint64_t seed, uint64_t start, int64_t count, uint32_t stream_lo,
+                              uint32_t stream_hi, double* out, void* stream, char* err,
+                              int64_t err_len);
+
 /* Parameter-shift program rewriting (host only).  String results come back
  * in a tfqx_strings handle: count, bytes + offsets[count+1], and the result
  * shape; free it with tfqx_strings_destroy. */

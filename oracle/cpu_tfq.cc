@@ -13,8 +13,14 @@
 //                 psi <- G^dag psi; per symbol: scratch = dG psi (control-
 //                 satisfied amplitudes only), grad += 2 Re<scratch|lambda>;
 //                 lambda <- G^dag lam (core/ops/tfq_adj_grad_op.cc:214-276).
-// Circuits run in parallel over std::threads (the ComputeSmall schedule of
-// util_qsim.h:46-131, one circuit per worker).
+// Two schedules, as TFQ's ops choose them: ComputeSmall runs circuits in
+// parallel over std::threads, one circuit per worker (util_qsim.h:46-131);
+// ComputeLarge runs circuits one after another with every thread on the one
+// state (OpenMP loops, the analogue of qsim's parallel For), which TFQ picks
+// when the widest circuit has >= 26 qubits (>= 25 for the adjoint) or the
+// batch holds one circuit (tfq_simulate_expectation_op.cc:120-126,
+// tfq_adj_grad_op.cc:161-169).  Samples follow qsim's sampler: FP64 norm,
+// sorted uniforms times the norm, one linear scan of the cumulative mass.
 //
 // Gate conventions: bits[0] is the matrix-index MSB; matrices are row-major
 // complex64.  The Python side (oracle/cpu_tfq.py) lowers programs with the
@@ -106,6 +112,140 @@ double real_inner(const c64* a, const c64* b, uint64_t N) {
   double acc = 0;
   for (uint64_t i = 0; i < N; ++i) acc += double(a[i].real()) * b[i].real() + double(a[i].imag()) * b[i].imag();
   return acc;
+}
+
+
+// ------------------------------------------------------------------ state-parallel (ComputeLarge)
+void apply_par(c64* s, int n, const Gate& g, const c64* U, bool zero_others, int nt) {
+  const int64_t half = int64_t(1) << (n - g.nt);
+  if (g.nt == 1) {
+    const uint64_t bit = uint64_t(1) << g.b0;
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (int64_t i = 0; i < half; ++i) {
+      const uint64_t x0 = ins0(uint64_t(i), g.b0), x1 = x0 | bit;
+      if ((x0 & g.cmask) != g.cval) {
+        if (zero_others) s[x0] = s[x1] = c64(0, 0);
+        continue;
+      }
+      const c64 a0 = s[x0], a1 = s[x1];
+      s[x0] = mulc(U[0], a0) + mulc(U[1], a1);
+      s[x1] = mulc(U[2], a0) + mulc(U[3], a1);
+    }
+    return;
+  }
+  const int lo = g.b0 < g.b1 ? g.b0 : g.b1, hi = g.b0 < g.b1 ? g.b1 : g.b0;
+  const uint64_t m0 = uint64_t(1) << g.b0, m1 = uint64_t(1) << g.b1;
+#pragma omp parallel for num_threads(nt) schedule(static)
+  for (int64_t i = 0; i < half; ++i) {
+    const uint64_t base = ins0(ins0(uint64_t(i), lo), hi);
+    uint64_t x[4];
+    for (int m = 0; m < 4; ++m) x[m] = base | ((m >> 1) ? m0 : 0) | ((m & 1) ? m1 : 0);
+    if ((base & g.cmask) != g.cval) {
+      if (zero_others)
+        for (int m = 0; m < 4; ++m) s[x[m]] = c64(0, 0);
+      continue;
+    }
+    c64 a[4];
+    for (int m = 0; m < 4; ++m) a[m] = s[x[m]];
+    for (int r = 0; r < 4; ++r) {
+      c64 acc(0, 0);
+      for (int c = 0; c < 4; ++c) acc += mulc(U[r * 4 + c], a[c]);
+      s[x[r]] = acc;
+    }
+  }
+}
+
+void apply_paulis_par(c64* s, int n, const int32_t* paulis, int np, int nt) {
+  static const c64 X[4] = {c64(0, 0), c64(1, 0), c64(1, 0), c64(0, 0)};
+  static const c64 Y[4] = {c64(0, 0), c64(0, -1), c64(0, 1), c64(0, 0)};
+  static const c64 Z[4] = {c64(1, 0), c64(0, 0), c64(0, 0), c64(-1, 0)};
+  for (int k = 0; k < np; ++k) {
+    Gate g{};
+    g.nt = 1;
+    g.b0 = paulis[2 * k];
+    const int t = paulis[2 * k + 1];
+    apply_par(s, n, g, t == 0 ? X : (t == 1 ? Y : Z), false, nt);
+  }
+}
+
+double real_inner_par(const c64* a, const c64* b, int64_t N, int nt) {
+  double acc = 0;
+#pragma omp parallel for num_threads(nt) schedule(static) reduction(+ : acc)
+  for (int64_t i = 0; i < N; ++i)
+    acc += double(a[i].real()) * b[i].real() + double(a[i].imag()) * b[i].imag();
+  return acc;
+}
+
+void copy_par(c64* dst, const c64* src, int64_t N, int nt) {
+#pragma omp parallel for num_threads(nt) schedule(static)
+  for (int64_t i = 0; i < N; ++i) dst[i] = src[i];
+}
+
+void zero_state_par(c64* s, int64_t N, int nt) {
+#pragma omp parallel for num_threads(nt) schedule(static)
+  for (int64_t i = 0; i < N; ++i) s[i] = c64(0, 0);
+  s[0] = c64(1, 0);
+}
+
+void axpy_par(c64* y, float w, const c64* x, int64_t N, int nt) {
+#pragma omp parallel for num_threads(nt) schedule(static)
+  for (int64_t i = 0; i < N; ++i) y[i] += w * x[i];
+}
+
+double philox_u(unsigned long long seed, unsigned long long shot, uint32_t s_lo, uint32_t s_hi) {
+  uint32_t c0 = uint32_t(shot), c1 = uint32_t(shot >> 32), c2 = s_lo, c3 = s_hi;
+  uint32_t k0 = uint32_t(seed), k1 = uint32_t(seed >> 32);
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = uint64_t(0xD2511F53u) * c0, p1 = uint64_t(0xCD9E8D57u) * c2;
+    const uint32_t n0 = uint32_t(p1 >> 32) ^ c1 ^ k0, n2 = uint32_t(p0 >> 32) ^ c3 ^ k1;
+    c0 = n0; c1 = uint32_t(p1); c2 = n2; c3 = uint32_t(p0);
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  const unsigned long long bits = (uint64_t(c0) << 32) | c1;
+  return double(bits >> 11) * (1.0 / 9007199254740992.0);
+}
+
+
+// qsim-style sampler: FP64 norm, sorted uniforms u_k * norm, one scan of the
+// cumulative mass (parallel over blocks of the state).  Indices come back in
+// ascending order, as qsim returns them.
+void sample_par(const c64* s, int64_t N, int64_t count, uint64_t seed, uint32_t s_lo, uint32_t s_hi,
+                int nt, uint64_t* out) {
+  const int nb = std::max(1, nt);
+  std::vector<double> bsum(size_t(nb), 0.0);
+  const int64_t chunk = (N + nb - 1) / nb;
+#pragma omp parallel for num_threads(nt) schedule(static)
+  for (int b = 0; b < nb; ++b) {
+    double acc = 0;
+    const int64_t e = std::min(N, (b + 1) * chunk);
+    for (int64_t i = b * chunk; i < e; ++i) acc += std::norm(s[i]);
+    bsum[size_t(b)] = acc;
+  }
+  double norm = 0;
+  std::vector<double> bstart(size_t(nb) + 1, 0.0);
+  for (int b = 0; b < nb; ++b) bstart[size_t(b) + 1] = bstart[size_t(b)] + bsum[size_t(b)];
+  norm = bstart[size_t(nb)];
+  std::vector<double> rs(static_cast<size_t>(count));
+  for (int64_t k = 0; k < count; ++k) rs[size_t(k)] = philox_u(seed, uint64_t(k), s_lo, s_hi) * norm;
+  std::sort(rs.begin(), rs.end());
+  // Draws falling in block b are [first[b], first[b+1]).
+  std::vector<int64_t> first(size_t(nb) + 1, count);
+  for (int b = 0; b < nb; ++b)
+    first[size_t(b)] = int64_t(std::lower_bound(rs.begin(), rs.end(), bstart[size_t(b)]) - rs.begin());
+  first[size_t(nb)] = count;
+#pragma omp parallel for num_threads(nt) schedule(static)
+  for (int b = 0; b < nb; ++b) {
+    int64_t k = first[size_t(b)];
+    const int64_t kend = first[size_t(b) + 1];
+    double cum = bstart[size_t(b)];
+    const int64_t e = std::min(N, (b + 1) * chunk);
+    for (int64_t i = b * chunk; i < e && k < kend; ++i) {
+      cum += std::norm(s[i]);
+      while (k < kend && rs[size_t(k)] < cum) out[k++] = uint64_t(i);
+    }
+    for (; k < kend; ++k) out[k] = uint64_t(e - 1);
+  }
 }
 
 }  // namespace
@@ -203,6 +343,101 @@ int cpu_tfq_run(int mode, int64_t B, const int32_t* circ, const int32_t* gates_i
   return 0;
 }
 
+// The ComputeLarge schedule: circuits one after another, every loop over the
+// state parallel over `threads` (arrays as in cpu_tfq_run).  mode 0:
+// expectation (fused gates); mode 1: adjoint (unfused gates); mode 2: simulate
+// the fused gates, then draw `nsamples` samples per circuit from Philox
+// stream (row, 0xFFFFFFFF) into samples_out[B][nsamples] (qsim's ascending
+// order).  gate_limit >= 0 bounds the work for timing a sample of a large
+// circuit: modes 0 and 2 apply only the first gate_limit fused gates; mode 1
+// runs the full forward pass and lambda, then only the last gate_limit gates
+// of the reverse sweep.  Returns 0.
+int cpu_tfq_run_large(int mode, int64_t B, const int32_t* circ, const int32_t* gates_i,
+                      const uint64_t* gates_m, const float* mats, const int32_t* terms_i,
+                      const float* terms_c, const int32_t* paulis, int64_t O, int64_t S,
+                      const float* downstream, int threads, int64_t gate_limit, int64_t nsamples,
+                      uint64_t seed, float* exp_out, float* grad_out, uint64_t* samples_out) {
+  const c64* M = reinterpret_cast<const c64*>(mats);
+  const int nt = std::max(1, threads);
+  auto gate_at = [&](int64_t k) {
+    Gate g;
+    const int32_t* a = gates_i + 9 * k;
+    g.nt = a[0]; g.b0 = a[1]; g.b1 = a[2]; g.mat = a[3]; g.nderiv = a[4];
+    g.dmat0 = a[5]; g.dmat1 = a[6]; g.col0 = a[7]; g.col1 = a[8];
+    g.cmask = gates_m[2 * k]; g.cval = gates_m[2 * k + 1];
+    return g;
+  };
+  std::vector<c64> psi, lam, scratch;
+  for (int64_t i = 0; i < B; ++i) {
+    const int32_t* c = circ + 8 * i;
+    const int n = c[0];
+    const int64_t row = c[7];
+    const int64_t N = int64_t(1) << n;
+    psi.resize(size_t(N));
+    scratch.resize(size_t(N));
+    zero_state_par(psi.data(), N, nt);
+    int gb = mode == 1 ? c[1] : c[3], ge = mode == 1 ? c[2] : c[4];
+    const int fe = (mode != 1 && gate_limit >= 0) ? std::min<int64_t>(ge, gb + gate_limit) : ge;
+    c64 U[16];
+    for (int k = gb; k < fe; ++k) {
+      const Gate g = gate_at(k);
+      load_matrix(M + g.mat, 1 << g.nt, false, U);
+      apply_par(psi.data(), n, g, U, false, nt);
+    }
+    if (mode == 2) {
+      if (nsamples > 0)
+        sample_par(psi.data(), N, nsamples, seed, uint32_t(row), 0xFFFFFFFFu, nt,
+                   samples_out + row * nsamples);
+      continue;
+    }
+    if (mode == 0) {
+      std::vector<double> acc(size_t(O), 0.0);
+      for (int t = c[5]; t < c[6]; ++t) {
+        const int32_t* ti = terms_i + 3 * t;
+        const int np = ti[2] - ti[1];
+        if (np == 0) {
+          acc[size_t(ti[0])] += terms_c[t];
+          continue;
+        }
+        copy_par(scratch.data(), psi.data(), N, nt);
+        apply_paulis_par(scratch.data(), n, paulis + 2 * ti[1], np, nt);
+        acc[size_t(ti[0])] += double(terms_c[t]) * real_inner_par(psi.data(), scratch.data(), N, nt);
+      }
+      for (int64_t j = 0; j < O; ++j) exp_out[row * O + j] = float(acc[size_t(j)]);
+      continue;
+    }
+    lam.resize(size_t(N));
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (int64_t x = 0; x < N; ++x) lam[size_t(x)] = c64(0, 0);
+    for (int t = c[5]; t < c[6]; ++t) {
+      const int32_t* ti = terms_i + 3 * t;
+      const float w = downstream[row * O + ti[0]] * terms_c[t];
+      if (std::fabs(w) < 1e-5f) continue;
+      copy_par(scratch.data(), psi.data(), N, nt);
+      apply_paulis_par(scratch.data(), n, paulis + 2 * ti[1], ti[2] - ti[1], nt);
+      axpy_par(lam.data(), w, scratch.data(), N, nt);
+    }
+    std::vector<double> grad(size_t(S), 0.0);
+    const int rb = gate_limit >= 0 ? std::max<int64_t>(gb, ge - gate_limit) : gb;
+    for (int k = ge - 1; k >= rb; --k) {
+      const Gate g = gate_at(k);
+      load_matrix(M + g.mat, 1 << g.nt, true, U);
+      apply_par(psi.data(), n, g, U, false, nt);
+      for (int d = 0; d < g.nderiv; ++d) {
+        copy_par(scratch.data(), psi.data(), N, nt);
+        c64 dU[16];
+        load_matrix(M + (d == 0 ? g.dmat0 : g.dmat1), 1 << g.nt, false, dU);
+        apply_par(scratch.data(), n, g, dU, true, nt);
+        grad[size_t(d == 0 ? g.col0 : g.col1)] +=
+            2.0 * real_inner_par(scratch.data(), lam.data(), N, nt);
+      }
+      apply_par(lam.data(), n, g, U, false, nt);
+    }
+    for (int64_t s2 = 0; s2 < S; ++s2) grad_out[row * S + s2] = float(grad[size_t(s2)]);
+  }
+  return 0;
+}
+
 // Noisy trajectories (TFQ's ComputeSmall schedule of
 // core/ops/noise/tfq_noisy_expectation.cc:284-398: trajectories spread over
 // the worker threads, one float32 state each).  A gate record with nt == -1 is
@@ -212,20 +447,6 @@ int cpu_tfq_run(int mode, int64_t B, const int32_t* circ, const int32_t* gates_i
 // i draws Philox uniform (seed, t, (i, c)); operator k is the first with
 // u < p_0 + ... + p_k.  ns[B][O]: trajectories per (row, slot).
 // exp_out[B][O] = mean over trajectories + identity terms.
-static double philox_u(unsigned long long seed, unsigned long long shot, uint32_t s_lo, uint32_t s_hi) {
-  uint32_t c0 = uint32_t(shot), c1 = uint32_t(shot >> 32), c2 = s_lo, c3 = s_hi;
-  uint32_t k0 = uint32_t(seed), k1 = uint32_t(seed >> 32);
-  for (int r = 0; r < 10; ++r) {
-    const uint64_t p0 = uint64_t(0xD2511F53u) * c0, p1 = uint64_t(0xCD9E8D57u) * c2;
-    const uint32_t n0 = uint32_t(p1 >> 32) ^ c1 ^ k0, n2 = uint32_t(p0 >> 32) ^ c3 ^ k1;
-    c0 = n0; c1 = uint32_t(p1); c2 = n2; c3 = uint32_t(p0);
-    k0 += 0x9E3779B9u;
-    k1 += 0xBB67AE85u;
-  }
-  const unsigned long long bits = (uint64_t(c0) << 32) | c1;
-  return double(bits >> 11) * (1.0 / 9007199254740992.0);
-}
-
 int cpu_tfq_noisy(int64_t B, const int32_t* circ, const int32_t* gates_i, const uint64_t* gates_m,
                   const float* mats, const int32_t* terms_i, const float* terms_c,
                   const int32_t* paulis, int64_t O, const int32_t* ns, uint64_t seed, int threads,

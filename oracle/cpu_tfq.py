@@ -28,7 +28,8 @@ _PT = {"X": 0, "Y": 1, "Z": 2}
 
 def build():
     """Compile the restatement (g++, x86-64-v3, -O3)."""
-    cmd = ["g++", "-O3", "-march=x86-64-v3", "-std=c++17", "-fPIC", "-shared", "-pthread", SRC,
+    cmd = ["g++", "-O3", "-march=x86-64-v3", "-std=c++17", "-fPIC", "-shared", "-pthread",
+           "-fopenmp", SRC,
            "-o", LIB + ".tmp"]
     subprocess.run(cmd, check=True, capture_output=True, text=True)
     os.replace(LIB + ".tmp", LIB)
@@ -45,6 +46,10 @@ def lib():
         L.cpu_tfq_run.argtypes = [ctypes.c_int, ctypes.c_int64, P, P, P, P, P, P, P,
                                   ctypes.c_int64, ctypes.c_int64, P, ctypes.c_int, P, P]
         L.cpu_tfq_run.restype = ctypes.c_int
+        L.cpu_tfq_run_large.argtypes = [ctypes.c_int, ctypes.c_int64, P, P, P, P, P, P, P,
+                                        ctypes.c_int64, ctypes.c_int64, P, ctypes.c_int,
+                                        ctypes.c_int64, ctypes.c_int64, ctypes.c_uint64, P, P, P]
+        L.cpu_tfq_run_large.restype = ctypes.c_int
         L.cpu_tfq_noisy.argtypes = [ctypes.c_int64, P, P, P, P, P, P, P, ctypes.c_int64, P,
                                     ctypes.c_uint64, ctypes.c_int, P]
         L.cpu_tfq_noisy.restype = ctypes.c_int
@@ -146,8 +151,53 @@ def _p(a):
     return ctypes.c_void_p(a.ctypes.data if a.size else 0)
 
 
+def large_schedule(low, mode):
+    """TFQ's choice of ComputeLarge: widest circuit >= 26 qubits (>= 25 for the
+    adjoint) or a batch of one (tfq_simulate_expectation_op.cc:120-126,
+    tfq_adj_grad_op.cc:161-169)."""
+    max_nq = int(low.circ[:, 0].max()) if low.B else 0
+    return max_nq >= (25 if mode == 1 else 26) or low.B == 1
+
+
+def run_large(low, mode, downstream=None, threads=1, gate_limit=-1):
+    """The ComputeLarge schedule (every thread on one state).  gate_limit >= 0
+    bounds the work: mode 0 applies only the first gate_limit fused gates;
+    mode 1 runs forward and lambda, then the last gate_limit reverse gates."""
+    exp = np.zeros((low.B, max(low.O, 1)), np.float32)
+    grad = np.zeros((low.B, max(low.S, 1)), np.float32)
+    ds = np.ascontiguousarray(downstream if downstream is not None else np.zeros((low.B, low.O)),
+                              dtype=np.float32)
+    mats = np.ascontiguousarray(low.mats.view(np.float32))
+    rc = lib().cpu_tfq_run_large(mode, low.B, _p(low.circ), _p(low.gi), _p(low.gm), _p(mats),
+                                 _p(low.ti), _p(low.tc), _p(low.pl), low.O, low.S, _p(ds),
+                                 int(threads), int(gate_limit), 0, 0, _p(exp), _p(grad), None)
+    if rc != 0:
+        raise RuntimeError("cpu_tfq_run_large failed")
+    exp = exp[:, :low.O]
+    exp[low.empty] = -2.0
+    return exp if mode == 0 else grad[:, :low.S]
+
+
+def sample_large(low, num_samples, seed, threads=1, gate_limit=-1):
+    """ComputeLarge simulation of the fused gates (the first gate_limit when
+    >= 0), then qsim's sampler: indices [B, num_samples] in ascending order,
+    drawn from the Philox stream (row, 0xFFFFFFFF)."""
+    out = np.zeros((low.B, max(1, int(num_samples))), np.uint64)
+    mats = np.ascontiguousarray(low.mats.view(np.float32))
+    ds = np.zeros((low.B, max(low.O, 1)), np.float32)
+    exp = np.zeros((low.B, max(low.O, 1)), np.float32)
+    rc = lib().cpu_tfq_run_large(2, low.B, _p(low.circ), _p(low.gi), _p(low.gm), _p(mats),
+                                 _p(low.ti), _p(low.tc), _p(low.pl), low.O, low.S, _p(ds),
+                                 int(threads), int(gate_limit), int(num_samples),
+                                 int(seed) & (2 ** 64 - 1), _p(exp), None, _p(out))
+    if rc != 0:
+        raise RuntimeError("cpu_tfq_run_large failed")
+    return out[:, :int(num_samples)]
+
+
 def run(low, mode, downstream=None, threads=1):
-    """mode 0: expectations [B, O]; mode 1: adjoint gradients [B, S]."""
+    """mode 0: expectations [B, O]; mode 1: adjoint gradients [B, S]
+    (ComputeSmall schedule: one circuit per thread)."""
     exp = np.zeros((low.B, max(low.O, 1)), np.float32)
     grad = np.zeros((low.B, max(low.S, 1)), np.float32)
     ds = np.ascontiguousarray(downstream if downstream is not None else np.zeros((low.B, low.O)),
@@ -161,6 +211,38 @@ def run(low, mode, downstream=None, threads=1):
     exp = exp[:, :low.O]
     exp[low.empty] = -2.0
     return exp if mode == 0 else grad[:, :low.S]
+
+
+def _lower_chunk(args):
+    return Lowered(*args)
+
+
+def lower_parallel(programs, symbol_names, symbol_values, pauli_sums, workers=8, chunks=None):
+    """``Lowered`` over contiguous row chunks of a batch, built in ``workers``
+    fresh (spawned) interpreters: the Python proto decode dominates the cost
+    of lowering thousands of circuits.  Returns the chunks in row order."""
+    import multiprocessing as mp
+    B = len(programs)
+    k = max(1, chunks or 2 * workers)
+    edges = [B * c // k for c in range(k + 1)]
+    vals = np.asarray(symbol_values, dtype=np.float32).reshape(B, -1)
+    ps = np.asarray(pauli_sums, dtype=object)
+    jobs = [(list(programs[a:b]), list(symbol_names), vals[a:b], ps[a:b])
+            for a, b in zip(edges[:-1], edges[1:]) if b > a]
+    if workers <= 1 or len(jobs) == 1:
+        return [_lower_chunk(j) for j in jobs]
+    with mp.get_context("spawn").Pool(min(workers, len(jobs))) as pool:
+        return pool.map(_lower_chunk, jobs)
+
+
+def run_chunks(chunks, mode, downstream=None, threads=1):
+    """``run`` over the chunks of ``lower_parallel``; rows concatenated."""
+    out, r0 = [], 0
+    for low in chunks:
+        ds = None if downstream is None else np.asarray(downstream)[r0:r0 + low.B]
+        out.append(run(low, mode, ds, threads))
+        r0 += low.B
+    return np.concatenate(out, axis=0)
 
 
 class LoweredNoisy:

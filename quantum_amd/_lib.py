@@ -71,6 +71,9 @@ def _declare(lib):
     lib.tfqx_debug_fuse.argtypes = [P, i64, P, P, i64, P, i32, P, i64, P, i64, P, P,
                                     c.c_char_p, i64]
     lib.tfqx_debug_fuse.restype = c.c_int
+    lib.tfqx_debug_philox_uniform.argtypes = [u64, u64, i64, c.c_uint32, c.c_uint32, P, P,
+                                              c.c_char_p, i64]
+    lib.tfqx_debug_philox_uniform.restype = c.c_int
     lib.tfqx_plan_rebind.argtypes = [P, P, c.c_char_p, i64]
     lib.tfqx_plan_rebind.restype = c.c_int
     lib.tfqx_plan_stats.argtypes = [P, P, i64]
@@ -136,7 +139,7 @@ EXPORTED_SYMBOLS = (
     "tfqx_strings_data", "tfqx_strings_offsets", "tfqx_strings_dim", "tfqx_strings_destroy",
     "tfqx_plan_inner_product",
     "tfqx_plan_inner_product_grad", "tfqx_plan_adjoint_gradient", "tfqx_plan_expectation_and_gradient", "tfqx_plan_rebind",
-    "tfqx_debug_fuse",
+    "tfqx_debug_fuse", "tfqx_debug_philox_uniform",
     "tfqx_plan_stats",
     "tfqx_plan_max_num_qubits", "tfqx_plan_workspace_bytes",
     "tfqx_plan_output_bytes", "tfqx_plan_upload", "tfqx_plan_launch", "tfqx_plan_launch_plain",
@@ -283,6 +286,22 @@ def fusion_decisions(program, symbol_names=(), symbol_values=(), width=2):
     for r in ops[:nops.value]:
         bits = tuple(int(b) for b in r[1:1 + r[0]])
         out.append((bits, bool(r[5]), tuple(int(g) for g in gates[r[6]:r[7]])))
+    return out
+
+
+def philox_uniforms(seed, count, stream_lo, stream_hi, start=0):
+    """The device sampler's FP64 uniforms for shots [start, start + count) of
+    one stream (a torch float64 tensor on the current device)."""
+    import torch
+    dev = torch.device("cuda", torch.cuda.current_device())
+    out = torch.empty(int(count), dtype=torch.float64, device=dev)
+    err = ctypes.create_string_buffer(4096)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    check(lib().tfqx_debug_philox_uniform(int(seed) & (2 ** 64 - 1), int(start), int(count),
+                                          int(stream_lo) & 0xFFFFFFFF,
+                                          int(stream_hi) & 0xFFFFFFFF,
+                                          ctypes.c_void_p(out.data_ptr() if count else 0),
+                                          ctypes.c_void_p(stream), err, 4096), err)
     return out
 
 

@@ -722,6 +722,15 @@ int tfqx_debug_fuse(const uint8_t* program, int64_t program_len, const uint8_t* 
   return TFQX_OK;
 }
 
+int tfqx_debug_philox_uniform(uint64_t seed, uint64_t start, int64_t count, uint32_t stream_lo,
+                              uint32_t stream_hi, double* out, void* stream, char* err,
+                              int64_t len) {
+  if (count < 0 || (count > 0 && !out)) return Fail(err, len, TFQX_INVALID_ARGUMENT, "invalid argument");
+  const int rc = LaunchPhiloxUniforms(seed, start, count, stream_lo, stream_hi, out,
+                                      static_cast<hipStream_t>(stream), err, len);
+  return rc ? TFQX_DEVICE_ERROR : TFQX_OK;
+}
+
 int tfqx_plan_rebind(tfqx_plan* p, const float* values, char* err, int64_t len) {
   if (!p || (!values && p->S > 0)) return Fail(err, len, TFQX_INVALID_ARGUMENT, "null argument");
   if (!p->shared)

@@ -79,5 +79,7 @@ void SchedulePasses(const std::vector<FusedOp>& ops, int n, int L, std::vector<D
 size_t LargeWorkspaceBytes(LargePlan* p);
 int LargeUpload(LargePlan* p, void* ws, hipStream_t stream, char* err, int64_t len);
 int LargeLaunch(LargePlan* p, void* ws, void* output, hipStream_t stream, char* err, int64_t len);
+int LaunchPhiloxUniforms(uint64_t seed, uint64_t start, int64_t count, uint32_t s_lo,
+                         uint32_t s_hi, double* out, hipStream_t stream, char* err, int64_t len);
 
 }  // namespace tfqx

@@ -416,6 +416,16 @@ __global__ __launch_bounds__(kThreads) void shots_parity_kernel(
   if (threadIdx.x == 0) partials[blockIdx.x] = tot;
 }
 
+// The sampler's uniforms for shots [start, start + count) of one stream, as
+// the sampling kernels draw them (exposed for a bit-exact check of the RNG).
+__global__ __launch_bounds__(kThreads) void philox_uniform_kernel(
+    unsigned long long seed, unsigned long long start, int64_t count, uint32_t s_lo,
+    uint32_t s_hi, double* __restrict__ out) {
+  for (int64_t k = int64_t(blockIdx.x) * kThreads + threadIdx.x; k < count;
+       k += int64_t(gridDim.x) * kThreads)
+    out[k] = philox_uniform(seed, start + (unsigned long long)k, s_lo, s_hi);
+}
+
 size_t AlignUp(size_t x) { return (x + 255) & ~size_t(255); }
 
 int Err(char* err, int64_t len, const std::string& msg) {
@@ -765,6 +775,17 @@ int LargeLaunch(LargePlan* p, void* ws, void* output, hipStream_t stream, char* 
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return Err(err, len, std::string("large-state kernels: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int LaunchPhiloxUniforms(uint64_t seed, uint64_t start, int64_t count, uint32_t s_lo,
+                         uint32_t s_hi, double* out, hipStream_t stream, char* err, int64_t len) {
+  if (count <= 0) return 0;
+  const int64_t blocks = std::min<int64_t>((count + kThreads - 1) / kThreads, 4096);
+  hipLaunchKernelGGL(philox_uniform_kernel, dim3(unsigned(blocks)), dim3(kThreads), 0, stream,
+                     (unsigned long long)seed, (unsigned long long)start, count, s_lo, s_hi, out);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return Err(err, len, std::string("philox kernel: ") + hipGetErrorString(e));
   return 0;
 }
 

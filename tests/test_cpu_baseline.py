@@ -38,6 +38,56 @@ def test_cpu_baseline_matches_oracle():
                                    atol=1e-4)
 
 
+def test_compute_large_schedule_matches_oracle():
+    """The state-parallel (ComputeLarge) schedule gives the same results as the
+    oracle, including the empty program's -2, at 1 and 4 threads."""
+    progs, names, vals, ps, grads = _batch(4)
+    low = cpu_tfq.Lowered(list(progs), names, vals, ps)
+    want_e = O.simulate_expectation(list(progs), names, vals, ps)
+    want_g = O.adjoint_gradient(list(progs), names, vals, ps, grads)
+    for threads in (1, 4):
+        np.testing.assert_allclose(cpu_tfq.run_large(low, 0, threads=threads), want_e, atol=1e-4)
+        np.testing.assert_allclose(cpu_tfq.run_large(low, 1, downstream=grads, threads=threads),
+                                   want_g, atol=1e-4)
+
+
+def test_compute_large_bounded_reverse_sweep_covers_the_last_gates():
+    """gate_limit = G runs the whole reverse sweep; a bound touching only the
+    last gates leaves the symbols of earlier gates at zero."""
+    q = C.LineQubit.range(3)
+    circ = C.Circuit.from_moments([[C.rx(C.Symbol("a")).on(q[0])], [C.CNOT(q[0], q[1])],
+                                   [C.ry(C.Symbol("b")).on(q[2])]])
+    progs = S.convert_to_tensor([circ])
+    ps = S.convert_to_tensor([[C.PauliSum([C.PauliString({q[0]: "Z", q[2]: "X"})])]])
+    vals = np.array([[0.3, -0.7]], np.float32)
+    low = cpu_tfq.Lowered(list(progs), ["a", "b"], vals, ps)
+    ones = np.ones((1, 1), np.float32)
+    full = cpu_tfq.run_large(low, 1, downstream=ones, threads=2)
+    np.testing.assert_allclose(cpu_tfq.run_large(low, 1, downstream=ones, threads=2, gate_limit=3),
+                               full)
+    last = cpu_tfq.run_large(low, 1, downstream=ones, threads=2, gate_limit=1)
+    assert last[0, 0] == 0.0 and abs(last[0, 1] - full[0, 1]) < 1e-6 and full[0, 0] != 0.0
+
+
+def test_compute_large_sampler_matches_inverse_cdf_draws():
+    """qsim's sampler (sorted uniforms, one scan) returns, in ascending order,
+    the multiset of the oracle's inverse-CDF draws from the same Philox
+    stream; only draws within float32 rounding of a CDF boundary may move."""
+    rng = np.random.default_rng(9)
+    q = C.LineQubit.range(8)
+    circs = [W.random_circuit(rng, q, 30) for _ in range(3)]
+    progs = S.convert_to_tensor(circs)
+    ps = S.convert_to_tensor([[C.PauliSum([])]] * 3)
+    low = cpu_tfq.Lowered(list(progs), [], np.zeros((3, 0), np.float32), ps)
+    shots, seed = 20000, 31
+    got = cpu_tfq.sample_large(low, shots, seed, threads=4)
+    for i in range(3):
+        probs = np.abs(O.simulate(O.Program(progs[i], {}))) ** 2
+        want = np.sort(O.sample_indices(probs, seed, i, 0xFFFFFFFF, shots))
+        assert np.all(np.diff(got[i].astype(np.int64)) >= 0)
+        assert np.mean(got[i].astype(np.int64) == want) > 0.999
+
+
 def test_basic_fusion_preserves_the_state():
     rng = np.random.default_rng(5)
     q = C.LineQubit.range(4)

@@ -273,6 +273,62 @@ def test_unitarity_all_families():
         np.testing.assert_allclose(U @ U.conj().T, np.eye(len(U)), atol=1e-12)
 
 
+def _two_qubit_unitary(gate):
+    q = C.LineQubit.range(2)
+    return O.Program(_ser(C.Circuit(gate.on(q[0], q[1])))).gates[0].unitary()
+
+
+@pytest.mark.parametrize("t,gs", [(1.0, 0.0), (0.5, 0.0), (0.3125, -0.5), (-0.71, 0.25)])
+def test_two_qubit_eigen_gates_closed_forms(t, gs):
+    """Closed forms independent of the eigen-projector sums the oracle uses:
+    ZZ^t = diag(1, w, w, 1), CZ^t = diag(1, 1, 1, w), and on the {|01>, |10>}
+    block SWAP^t = [[a, b], [b, a]] with a = (1 + w)/2, b = (1 - w)/2 and
+    ISWAP^t = [[c, i s], [i s, c]] with c = cos(pi t/2), s = sin(pi t/2);
+    XX^t = a I + b X(x)X and YY^t = a I + b Y(x)Y (P^2 = I for a Pauli
+    product P); CNOT^t = I (+) X^t with X^t = a I + b X; all times the global
+    phase e^{i pi t gs} (w = e^{i pi t})."""
+    w = np.exp(1j * np.pi * t)
+    ph = np.exp(1j * np.pi * t * gs)
+    a, b = (1 + w) / 2, (1 - w) / 2
+    I4 = np.eye(4)
+    X = np.array([[0, 1], [1, 0]])
+    Y = np.array([[0, -1j], [1j, 0]])
+    cases = {
+        "ZZ": (C.ZZPowGate(exponent=t, global_shift=gs), np.diag([1, w, w, 1])),
+        "CZ": (C.CZPowGate(exponent=t, global_shift=gs), np.diag([1, 1, 1, w])),
+        "SWAP": (C.SwapPowGate(exponent=t, global_shift=gs),
+                 np.array([[1, 0, 0, 0], [0, a, b, 0], [0, b, a, 0], [0, 0, 0, 1]])),
+        "ISWAP": (C.ISwapPowGate(exponent=t, global_shift=gs),
+                  np.array([[1, 0, 0, 0],
+                            [0, np.cos(np.pi * t / 2), 1j * np.sin(np.pi * t / 2), 0],
+                            [0, 1j * np.sin(np.pi * t / 2), np.cos(np.pi * t / 2), 0],
+                            [0, 0, 0, 1]])),
+        "XX": (C.XXPowGate(exponent=t, global_shift=gs), a * I4 + b * np.kron(X, X)),
+        "YY": (C.YYPowGate(exponent=t, global_shift=gs), a * I4 + b * np.kron(Y, Y)),
+        "CNOT": (C.CNotPowGate(exponent=t, global_shift=gs),
+                 np.block([[np.eye(2), np.zeros((2, 2))],
+                           [np.zeros((2, 2)), a * np.eye(2) + b * X]])),
+    }
+    for name, (gate, U) in cases.items():
+        got = _two_qubit_unitary(gate)
+        np.testing.assert_allclose(got, ph * U, atol=1e-6, err_msg=name)
+
+
+def test_swap_and_iswap_special_points():
+    """SWAP^1 is the permutation |01> <-> |10>; SWAP^0.5 is Cirq's sqrt-SWAP
+    with entries (1 +- i)/2; ISWAP^1 maps |01> -> i|10>; ZZ^1 = Z (x) Z."""
+    perm = np.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]])
+    np.testing.assert_allclose(_two_qubit_unitary(C.SwapPowGate(exponent=1.0)), perm, atol=1e-7)
+    h = np.array([[1, 0, 0, 0], [0, (1 + 1j) / 2, (1 - 1j) / 2, 0],
+                  [0, (1 - 1j) / 2, (1 + 1j) / 2, 0], [0, 0, 0, 1]])
+    np.testing.assert_allclose(_two_qubit_unitary(C.SwapPowGate(exponent=0.5)), h, atol=1e-7)
+    isw = _two_qubit_unitary(C.ISwapPowGate(exponent=1.0))
+    np.testing.assert_allclose(isw @ np.array([0, 1, 0, 0]), [0, 0, 1j, 0], atol=1e-7)
+    Z = np.diag([1, -1])
+    np.testing.assert_allclose(_two_qubit_unitary(C.ZZPowGate(exponent=1.0)), np.kron(Z, Z),
+                               atol=1e-7)
+
+
 def test_adjoint_vs_parameter_shift_and_cd():
     rng = np.random.default_rng(11)
     q = C.LineQubit.range(4)

@@ -40,6 +40,15 @@ FP32_PEAK_TFLOPS = 157.3
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "batch_kernel_uneven4096_traffic.json")
 
 
+def host_threads():
+    """Host threads for the CPU legs: every core this process may run on,
+    capped by OMP_NUM_THREADS when the machine sets it (the GPU boxes allot
+    16 host threads to each GPU)."""
+    cores = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(cores, int(omp))) if omp.isdigit() and int(omp) > 0 else cores
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -323,6 +332,255 @@ def noisy_measure(stream, steps=10, warmup=2, trajectories=2000, cpu_rows=8, cpu
     return result
 
 
+def _time_plan(plan, out, stream, reps):
+    """Average seconds per launch of an uploaded plan (one warm-up launch)."""
+    plan.launch(out, stream)
+    return time_launch(lambda: plan.launch(out, stream), stream, reps=reps)
+
+
+def sampling_measure(stream, forward_ms=None, shots=10 ** 6, seed=4321, cpu=True, cpu_gates=24):
+    """The 30-qubit grid circuit (5x6 GridQubits, 14 cycles of sqrt-X/Y/W +
+    FSim(pi/2, pi/6)): tfq_simulate_samples at 1e6 shots and the 49-term sum
+    Z_i Z_j over grid edges as a sampled expectation with 1e6 shots per term.
+
+    Each launch re-simulates the circuit, so the sampler's own time is the
+    launch minus the forward passes (``forward_ms``, measured by the HBM leg
+    on the same circuit).  The sampler's algorithmic HBM bytes are one read
+    of the 8 GiB state (8 B x 2^30) plus its output."""
+    import torch
+    from quantum_amd import _lib
+    from quantum_amd import serialize as S
+    from quantum_amd import workloads as W
+    from quantum_amd.tfq_simulate_ops import _common, _plan_args
+    circuit, zz = W.config2_grid()
+    progs, names, values = _common(S.convert_to_tensor([circuit]), [], np.zeros((1, 0)))
+    ps = S.convert_to_tensor([[zz]])
+    L = _lib.lib()
+    dev = torch.device("cuda", torch.cuda.current_device())
+    res = {"workload": "30-qubit 5x6 grid circuit, 14 cycles of random sqrt-X/sqrt-Y/sqrt-W + "
+                       "FSim(pi/2, pi/6); samples: 1e6 shots; sampled expectation: 49-term "
+                       "sum Z_i Z_j over grid edges, 1e6 shots per term",
+           "shots": shots}
+    P, N, args = _plan_args(progs, names, values)
+    plan = _lib.Plan(L.tfqx_plan_simulate_samples, *args, shots, seed)
+    out = torch.empty((1, shots, 30), dtype=torch.int8, device=dev)
+    plan.upload(dev, stream)
+    t = _time_plan(plan, out, stream, reps=3)
+    bits = out[0]
+    valid = bool(((bits == 0) | (bits == 1)).all())
+    del plan, out
+    torch.cuda.empty_cache()
+    samp = {"ms_per_launch": round(t * 1e3, 3), "all_bits_valid": valid}
+    if forward_ms:
+        st = t * 1e3 - forward_ms
+        bytes_ = 8.0 * 2 ** 30 + shots * 30
+        samp.update({"sampler_ms": round(st, 3),
+                     "sampler_roofline": {"bound": "hbm", "achieved": round(bytes_ / st / 1e6, 1),
+                                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                          "frac": round(bytes_ / st / 1e6 / HBM_PEAK_GBS, 4),
+                                          "bytes": bytes_}})
+    res["samples"] = samp
+    log(f"C2 samples: {t * 1e3:.2f} ms per launch (1e6 shots), sampler "
+        f"{samp.get('sampler_ms')} ms")
+    P, N, args = _plan_args(progs, names, values)
+    Ps = _lib.StringTensor(ps)
+    ns = np.full((1, 1), shots, np.int32)
+    plan = _lib.Plan(L.tfqx_plan_simulate_sampled_expectation, *args, Ps.ptr, Ps.off, 1,
+                     ns.ctypes.data, seed)
+    out = torch.empty((1, 1), dtype=torch.float32, device=dev)
+    plan.upload(dev, stream)
+    t2 = _time_plan(plan, out, stream, reps=2)
+    est = float(out.cpu()[0, 0])
+    del plan, out
+    P, N, args = _plan_args(progs, names, values)
+    plan = _lib.Plan(L.tfqx_plan_simulate_expectation, *args, Ps.ptr, Ps.off, 1)
+    out = torch.empty((1, 1), dtype=torch.float32, device=dev)
+    plan.upload(dev, stream)
+    plan.launch(out, stream)
+    exact = float(out.cpu()[0, 0])
+    del plan, out
+    torch.cuda.empty_cache()
+    se = {"ms_per_launch": round(t2 * 1e3, 3), "estimate": est, "exact": exact,
+          "abs_err": abs(est - exact), "five_sigma_bound": 5 * 7.0 / 1e3}
+    if forward_ms:
+        st = t2 * 1e3 - forward_ms
+        se["sampler_ms"] = round(st, 3)
+        se["shots_per_s"] = round(49 * shots / st * 1e3, 1)
+    res["sampled_expectation"] = se
+    log(f"C2 sampled expectation: {t2 * 1e3:.2f} ms per launch, estimate {est:.5f} vs exact "
+        f"{exact:.5f}")
+    if cpu:
+        from oracle import cpu_tfq
+        cores = host_threads()
+        low = cpu_tfq.Lowered(list(progs), names, values, ps)
+        fused = int(low.circ[0, 4] - low.circ[0, 3])
+        k = min(cpu_gates, fused)
+        t0 = time.perf_counter()
+        cpu_tfq.sample_large(low, 0, seed, threads=cores, gate_limit=0)
+        t_init = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        cpu_tfq.sample_large(low, 0, seed, threads=cores, gate_limit=k)
+        t_k = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        cpu_tfq.sample_large(low, shots, seed, threads=cores, gate_limit=0)
+        t_s = time.perf_counter() - t0 - t_init
+        per_gate = (t_k - t_init) / max(1, k)
+        full = t_init + per_gate * fused + t_s
+        res["cpu_baseline"] = {
+            "value": round(full * 1e3, 1), "unit": "ms per samples call (extrapolated)",
+            "cores": cores, "kind": "port",
+            "per_gate_ms": round(per_gate * 1e3, 2), "sampler_ms": round(t_s * 1e3, 1),
+            "sample": f"ComputeLarge schedule of oracle/cpu_tfq.cc (all threads on one 2^30 "
+                      f"float32 state): the first {k} of {fused} basic-fused gates timed and "
+                      f"extrapolated per gate, plus qsim's sampler at {shots} shots; "
+                      f"{cores} threads"}
+        log(f"C2 cpu: {per_gate * 1e3:.1f} ms per fused gate, sampler {t_s * 1e3:.0f} ms, "
+            f"full samples call ~{full:.1f} s on {cores} threads")
+    return res
+
+
+def adjoint_large_measure(stream, cpu=True, cpu_layers=1):
+    """C3: 26-qubit hardware-efficient ansatz (8 layers of ry, rz per qubit +
+    CZ ring, 416 symbols), batch 8 parameter sets, transverse-field Ising
+    observable (25 ZZ + 0.7 * 26 X = 51 terms): tfq_adj_grad in one call.
+
+    HBM accounting per parameter set: 16 B x 2^26 per forward pass, 32 B x
+    2^26 per reverse pass (psi and lambda together), and lambda's build as one
+    read + one write of the state (8 + 8 B x 2^26; its per-term gathers are
+    cache reuse)."""
+    import torch
+    from quantum_amd import serialize as S
+    from quantum_amd import tfq_adj_grad_op as A
+    from quantum_amd import workloads as W
+    circuit, names, values, ham = W.config3_hea()
+    B = values.shape[0]
+    progs = S.convert_to_tensor([circuit] * B)
+    ps = S.convert_to_tensor([[ham]] * B)
+    ones = np.ones((B, 1), np.float32)
+    plan, shape = A.plan_adj_grad(progs, names, values, ps, ones)
+    st = plan.stats()
+    dev = torch.device("cuda", torch.cuda.current_device())
+    out = torch.empty(shape, dtype=torch.float32, device=dev)
+    plan.upload(dev, stream)
+    t = _time_plan(plan, out, stream, reps=2)
+    grads = out.cpu().numpy()
+    del plan, out
+    torch.cuda.empty_cache()
+    N = 2 ** 26
+    fwd = st["forward_passes"] // B if st["forward_passes"] >= B else st["forward_passes"]
+    rev = st["adjoint_passes"] // B if st["adjoint_passes"] >= B else st["adjoint_passes"]
+    bytes_set = fwd * 16.0 * N + rev * 32.0 * N + 16.0 * N
+    achieved = bytes_set * B / t / 1e9
+    res = {"workload": "26-qubit HEA, 8 layers ry/rz + CZ ring (416 symbols), batch 8, 51-term "
+                       "TFIM (sum ZZ + 0.7 sum X), TFQ gradient gates",
+           "ms_per_launch": round(t * 1e3, 2), "param_sets": B,
+           "grads_per_s": round(B / t, 2), "forward_passes_per_set": fwd,
+           "reverse_passes_per_set": rev,
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "bytes_per_param_set": bytes_set},
+           "grad_abs_max": float(np.abs(grads).max())}
+    log(f"C3 adjoint: {t * 1e3:.1f} ms per launch of {B} x 416 gradients ({B / t:.2f} grads/s), "
+        f"{fwd} forward + {rev} reverse passes per set, {achieved:.0f} GB/s")
+    if cpu:
+        from oracle import cpu_tfq
+        cores = host_threads()
+        low = cpu_tfq.Lowered(list(progs[:1]), names, values[:1], ps[:1])
+        G = int(low.circ[0, 2] - low.circ[0, 1])
+        k = min(G, 78 * cpu_layers)
+        t0 = time.perf_counter()
+        cpu_tfq.run_large(low, 1, downstream=ones[:1], threads=cores, gate_limit=0, fwd_limit=0)
+        t_lam = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        cpu_tfq.run_large(low, 1, downstream=ones[:1], threads=cores, gate_limit=k, fwd_limit=k)
+        t_k = time.perf_counter() - t0
+        full = t_lam + (t_k - t_lam) * G / k
+        res["cpu_baseline"] = {
+            "value": round(1.0 / full, 4), "unit": "grads/s (one parameter set, extrapolated)",
+            "cores": cores, "kind": "port", "seconds_per_grad": round(full, 2),
+            "sample": f"ComputeLarge schedule of oracle/cpu_tfq.cc (all threads on 2^26 float32 "
+                      f"states): lambda (51 terms) plus the first {k} forward and last {k} "
+                      f"reverse of {G} gates (one ansatz layer), extrapolated to all gates; "
+                      f"{cores} threads"}
+        log(f"C3 cpu: {full:.1f} s per gradient ({cores} threads)")
+    return res
+
+
+def surrogate_measure(stream, train_batch=1024, steps=20):
+    """C5's network: 144 -> 512 -> 512 -> 512 -> 1 SiLU, BF16 MFMA / FP32
+    accumulate: one Sobolev training step at B = train_batch and one
+    inference + input-gradient call at B = 1 and B = 32, against the BF16
+    dense peak."""
+    import torch
+    from quantum_amd import surrogate as SG
+    dev = torch.device("cuda", torch.cuda.current_device())
+    P, H = 144, 512
+    model = SG.MLPSurrogate(P, hidden=H, max_batch=train_batch, seed=0)
+    rng = np.random.default_rng(5)
+    X = torch.tensor(rng.uniform(-1, 1, (train_batch, P)), dtype=torch.float32, device=dev)
+    E = torch.tensor(rng.normal(size=train_batch), dtype=torch.float32, device=dev)
+    G = torch.tensor(rng.normal(size=(train_batch, P)), dtype=torch.float32, device=dev)
+    for _ in range(3):
+        model.train_step(X, E, G, 1.0, 1e-4)
+    t = time_launch(lambda: model.train_step(X, E, G, 1.0, 1e-4), stream, reps=steps)
+    flops = 2.0 * train_batch * (5 * P * H + 12 * H * H)
+    res = {"network": "144-512-512-512-1 SiLU, BF16 weights, FP32 accumulate (600,065 params)",
+           "train_step_ms": round(t * 1e3, 4), "train_batch": train_batch,
+           "roofline": {"bound": "mfma", "achieved": round(flops / t / 1e12, 3), "peak": 2500.0,
+                        "unit": "TFLOP/s", "frac": round(flops / t / 1e12 / 2500.0, 5),
+                        "flops_per_step": flops}}
+    for b in (1, 32):
+        xb = X[:b].contiguous()
+        model.predict(xb)
+        tp = time_launch(lambda: model.predict(xb), stream, reps=steps)
+        res[f"predict_with_grad_ms_b{b}"] = round(tp * 1e3, 4)
+    log(f"C5 training step {t * 1e3:.3f} ms ({flops / t / 1e12:.1f} TFLOP/s); predict "
+        f"b1 {res['predict_with_grad_ms_b1']} ms, b32 {res['predict_with_grad_ms_b32']} ms")
+    return res
+
+
+def vqe_measure(stream, steps=3000, ref_steps=4000, seed=1005):
+    """C5 end to end: 12-qubit TFIM VQE (6 layers ry/rz + CZ ring, 144 symbols,
+    H = sum ZZ + sum X).  Wall-clock until the energy is within 1e-3
+    (relative) of E_ref, the energy exact-gradient Adam converges to in
+    ``ref_steps`` steps, for exact Adam and for the two surrogate-assisted
+    loops (each allowed ``steps`` steps); a loop that does not get there
+    reports null."""
+    import torch
+    from quantum_amd import surrogate as SG
+    from quantum_amd import workloads as W
+    circuit, names, ham = W.config5_vqe()
+    exact = SG.ExactEnergy(circuit, names, ham)
+    rng = np.random.default_rng(seed)
+    theta0 = rng.uniform(-0.1, 0.1, len(names))
+    ref = SG.vqe_exact(exact, theta0, steps=ref_steps)
+    e_ref = ref["energy"]
+    target = e_ref + 1e-3 * abs(e_ref)
+    res = {"workload": "12-qubit TFIM VQE, 6 layers ry/rz + CZ ring (144 symbols), H = sum ZZ + "
+                       "sum X; Adam lr 0.02 from theta0 ~ U(-0.1, 0.1)",
+           "e_ref": e_ref, "e_target": target, "ref_steps": ref_steps}
+    runs = {"exact_adam": lambda: SG.vqe_exact(exact, theta0, steps=steps, e_target=target)}
+    model = SG.MLPSurrogate(len(names), hidden=512, max_batch=1024, seed=0)
+    runs["surrogate_vqe"] = lambda: SG.SurrogateVQE(model, exact).run(theta0, steps=steps,
+                                                                     e_target=target)
+    model2 = SG.MLPSurrogate(len(names), hidden=512, max_batch=1024, seed=1)
+    runs["candidate_vqe"] = lambda: SG.CandidateVQE(model2, exact).run(theta0, steps=steps,
+                                                                      e_target=target)
+    for name, fn in runs.items():
+        r = fn()
+        res[name] = {"reached_step": r["reached"][0] if r["reached"] else None,
+                     "seconds_to_target": round(r["reached"][1], 3) if r["reached"] else None,
+                     "final_energy": r["energy"], "seconds": round(r["seconds"], 3),
+                     "exact_calls": r["exact_calls"]}
+        log(f"C5 {name}: reached {res[name]['reached_step']} in {res[name]['seconds_to_target']} s,"
+            f" final {r['energy']:.5f}")
+    ex = res["exact_adam"]["seconds_to_target"]
+    best = [res[k]["seconds_to_target"] for k in ("surrogate_vqe", "candidate_vqe")
+            if res[k]["seconds_to_target"] is not None]
+    res["surrogate_beats_exact"] = bool(best and ex and min(best) < ex)
+    return res
+
+
 def measured_traffic(circuits, flops):
     """HBM bytes per launch from the committed rocprofv3 counter summary, if it
     was taken on this same workload (same circuit count and flop count)."""
@@ -338,14 +596,14 @@ def measured_traffic(circuits, flops):
 
 def cpu_baseline(progs, names, values, ps, weights, gpu_exp, gpu_grad, sample, check):
     """TFQ's CPU (qsim) schedule restated in float32 C++ (oracle/cpu_tfq.cc) on
-    the host's cores (at most 16, one GPU's share), on the first `sample`
+    the host's cores (``host_threads``), on the first `sample`
     circuits of the batch: expectation (basic fusion) + adjoint gradient.
     Lowering the protos to gate lists (Python) is not timed.  Also returns the
     max |GPU - oracle| over the first `check` circuits (complex128 oracle) and
     the max |GPU - CPU baseline| over the sample."""
     from oracle import cpu_tfq
     from oracle import tfq_oracle as O
-    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    cores = host_threads()
     k = min(sample, len(progs))
     low = cpu_tfq.Lowered(list(progs[:k]), names, values[:k], ps[:k])
     cpu_tfq.run(low, 0, threads=cores)  # warm-up (page-in, thread start)
@@ -375,12 +633,17 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the autotuner (the --only profiling runs never tune: they use "
                          "the planner defaults)")
-    ap.add_argument("--only", choices=["all", "batch", "hbm", "copy", "ipgrad", "noisy", "ops"],
+    ap.add_argument("--only", choices=["all", "batch", "hbm", "copy", "ipgrad", "noisy", "ops",
+                                       "c2", "c3", "c5", "vqe"],
                     default="all",
                     help="profiling runs: 'batch' launches only the batched kernel (graph "
                          "replays, no plain launches, no CPU baseline); 'hbm' only the 30-qubit "
                          "streaming passes; 'copy' a 1 GiB device copy (counter calibration); "
-                         "'ipgrad' the inner-product gradient batch")
+                         "'ipgrad' the inner-product gradient batch; 'c2' the 30-qubit samples "
+                         "and sampled expectation; 'c3' the 26-qubit adjoint; 'c5' the "
+                         "surrogate network's kernels; 'vqe' the surrogate-assisted VQE")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines of the C2/C3 "
+                    "legs")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py measures one GPU: this build has no multi-GPU path (each box has one "
@@ -412,6 +675,22 @@ def main():
         return
     if args.only == "ops":
         print(json.dumps({"ops": ops_measure(stream)}), flush=True)
+        return
+    if args.only == "c2":
+        hbm = hbm_roofline(stream)
+        print(json.dumps({"sampling": sampling_measure(
+            stream, forward_ms=hbm["ms_per_pass"] * hbm["passes"], cpu=not args.no_cpu)}),
+            flush=True)
+        return
+    if args.only == "c3":
+        print(json.dumps({"adjoint_26q": adjoint_large_measure(stream, cpu=not args.no_cpu)}),
+              flush=True)
+        return
+    if args.only == "c5":
+        print(json.dumps({"surrogate": surrogate_measure(stream)}), flush=True)
+        return
+    if args.only == "vqe":
+        print(json.dumps({"vqe": vqe_measure(stream)}), flush=True)
         return
     if args.only == "noisy":
         print(json.dumps({"noisy": noisy_measure(stream, steps=args.steps, warmup=args.warmup)}),
@@ -487,6 +766,11 @@ def main():
     ipg = inner_product_grad_measure(stream)
     noisy = noisy_measure(stream)
     other_ops = ops_measure(stream)
+    fwd_ms = hbm["ms_per_pass"] * hbm["passes"] if hbm else None
+    sampling = sampling_measure(stream, forward_ms=fwd_ms, cpu=not args.no_cpu)
+    adj26 = adjoint_large_measure(stream, cpu=not args.no_cpu)
+    surrogate = surrogate_measure(stream)
+    vqe = vqe_measure(stream)
 
     done, cpu_dt, cores, err, checked, err_cpu = cpu_baseline(
         progs, names, values, ps, weights, gpu_exp, gpu_grad, args.cpu_sample, args.oracle_check)
@@ -523,6 +807,10 @@ def main():
         "inner_product_grad": ipg,
         "noisy": noisy,
         "ops": other_ops,
+        "sampling_30q": sampling,
+        "adjoint_26q": adj26,
+        "surrogate": surrogate,
+        "vqe": vqe,
         "cpu_baseline": {"value": round(done / cpu_dt, 3), "unit": "circuits/s", "cores": cores,
                          "kind": "port",
                          "sample": f"first {done} circuits of the same batch, expectation (basic "

@@ -350,13 +350,15 @@ int cpu_tfq_run(int mode, int64_t B, const int32_t* circ, const int32_t* gates_i
 // stream (row, 0xFFFFFFFF) into samples_out[B][nsamples] (qsim's ascending
 // order).  gate_limit >= 0 bounds the work for timing a sample of a large
 // circuit: modes 0 and 2 apply only the first gate_limit fused gates; mode 1
-// runs the full forward pass and lambda, then only the last gate_limit gates
-// of the reverse sweep.  Returns 0.
+// runs only the last gate_limit gates of the reverse sweep, and fwd_limit >= 0
+// applies only the first fwd_limit gates of its forward pass (lambda is
+// always built in full).  Returns 0.
 int cpu_tfq_run_large(int mode, int64_t B, const int32_t* circ, const int32_t* gates_i,
                       const uint64_t* gates_m, const float* mats, const int32_t* terms_i,
                       const float* terms_c, const int32_t* paulis, int64_t O, int64_t S,
-                      const float* downstream, int threads, int64_t gate_limit, int64_t nsamples,
-                      uint64_t seed, float* exp_out, float* grad_out, uint64_t* samples_out) {
+                      const float* downstream, int threads, int64_t gate_limit, int64_t fwd_limit,
+                      int64_t nsamples, uint64_t seed, float* exp_out, float* grad_out,
+                      uint64_t* samples_out) {
   const c64* M = reinterpret_cast<const c64*>(mats);
   const int nt = std::max(1, threads);
   auto gate_at = [&](int64_t k) {
@@ -374,10 +376,11 @@ int cpu_tfq_run_large(int mode, int64_t B, const int32_t* circ, const int32_t* g
     const int64_t row = c[7];
     const int64_t N = int64_t(1) << n;
     psi.resize(size_t(N));
-    scratch.resize(size_t(N));
+    if (mode != 2) scratch.resize(size_t(N));
     zero_state_par(psi.data(), N, nt);
     int gb = mode == 1 ? c[1] : c[3], ge = mode == 1 ? c[2] : c[4];
-    const int fe = (mode != 1 && gate_limit >= 0) ? std::min<int64_t>(ge, gb + gate_limit) : ge;
+    const int64_t flim = mode == 1 ? fwd_limit : gate_limit;
+    const int fe = flim >= 0 ? int(std::min<int64_t>(ge, gb + flim)) : ge;
     c64 U[16];
     for (int k = gb; k < fe; ++k) {
       const Gate g = gate_at(k);

@@ -48,7 +48,8 @@ def lib():
         L.cpu_tfq_run.restype = ctypes.c_int
         L.cpu_tfq_run_large.argtypes = [ctypes.c_int, ctypes.c_int64, P, P, P, P, P, P, P,
                                         ctypes.c_int64, ctypes.c_int64, P, ctypes.c_int,
-                                        ctypes.c_int64, ctypes.c_int64, ctypes.c_uint64, P, P, P]
+                                        ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                        ctypes.c_uint64, P, P, P]
         L.cpu_tfq_run_large.restype = ctypes.c_int
         L.cpu_tfq_noisy.argtypes = [ctypes.c_int64, P, P, P, P, P, P, P, ctypes.c_int64, P,
                                     ctypes.c_uint64, ctypes.c_int, P]
@@ -159,10 +160,11 @@ def large_schedule(low, mode):
     return max_nq >= (25 if mode == 1 else 26) or low.B == 1
 
 
-def run_large(low, mode, downstream=None, threads=1, gate_limit=-1):
+def run_large(low, mode, downstream=None, threads=1, gate_limit=-1, fwd_limit=-1):
     """The ComputeLarge schedule (every thread on one state).  gate_limit >= 0
     bounds the work: mode 0 applies only the first gate_limit fused gates;
-    mode 1 runs forward and lambda, then the last gate_limit reverse gates."""
+    mode 1 runs the last gate_limit reverse gates (and, with fwd_limit >= 0,
+    only the first fwd_limit forward gates; lambda is always built)."""
     exp = np.zeros((low.B, max(low.O, 1)), np.float32)
     grad = np.zeros((low.B, max(low.S, 1)), np.float32)
     ds = np.ascontiguousarray(downstream if downstream is not None else np.zeros((low.B, low.O)),
@@ -170,7 +172,8 @@ def run_large(low, mode, downstream=None, threads=1, gate_limit=-1):
     mats = np.ascontiguousarray(low.mats.view(np.float32))
     rc = lib().cpu_tfq_run_large(mode, low.B, _p(low.circ), _p(low.gi), _p(low.gm), _p(mats),
                                  _p(low.ti), _p(low.tc), _p(low.pl), low.O, low.S, _p(ds),
-                                 int(threads), int(gate_limit), 0, 0, _p(exp), _p(grad), None)
+                                 int(threads), int(gate_limit), int(fwd_limit), 0, 0, _p(exp),
+                                 _p(grad), None)
     if rc != 0:
         raise RuntimeError("cpu_tfq_run_large failed")
     exp = exp[:, :low.O]
@@ -188,7 +191,7 @@ def sample_large(low, num_samples, seed, threads=1, gate_limit=-1):
     exp = np.zeros((low.B, max(low.O, 1)), np.float32)
     rc = lib().cpu_tfq_run_large(2, low.B, _p(low.circ), _p(low.gi), _p(low.gm), _p(mats),
                                  _p(low.ti), _p(low.tc), _p(low.pl), low.O, low.S, _p(ds),
-                                 int(threads), int(gate_limit), int(num_samples),
+                                 int(threads), int(gate_limit), -1, int(num_samples),
                                  int(seed) & (2 ** 64 - 1), _p(exp), None, _p(out))
     if rc != 0:
         raise RuntimeError("cpu_tfq_run_large failed")

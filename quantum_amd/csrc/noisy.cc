@@ -168,10 +168,15 @@ void NoisyLayout(NoisyPlan* p) {
   // leaves them so.
   p->off_nacc = off; off = Align(off + std::max<size_t>(1, p->ident.size()) * sizeof(double));
   p->image = off;
+  p->grid = std::max(1, std::min(kNoisyGrid, int(p->chunks.size())));
+  int lds_nq = kWaveMaxQubits + 3;
+  for (int c = 0; c < p->nblock; ++c)
+    if (p->chunks[size_t(c)].nq <= kNoisyLdsMaxQubits) lds_nq = std::max(lds_nq, p->chunks[size_t(c)].nq);
+  p->lds_log2 = lds_nq;
   if (p->max_nq > kNoisyLdsMaxQubits) {
     p->state_log2 = p->max_nq;
     p->off_states = off;
-    off = Align(off + size_t(kNoisyGrid) * (size_t(1) << p->state_log2) * sizeof(cfloat));
+    off = Align(off + size_t(p->grid) * (size_t(1) << p->state_log2) * sizeof(cfloat));
   }
   p->total = off;
   p->staging.assign(p->image, 0);

@@ -14,7 +14,8 @@
 //     trajectory per workgroup with the state in LDS;
 //   * circuits of kNoisyLdsMaxQubits+1 .. kNoisyMaxQubits qubits run one
 //     trajectory per workgroup with the state in a per-workgroup buffer of
-//     the plan's workspace (kNoisyGrid buffers of 2^max_nq amplitudes).
+//     the plan's workspace (one buffer of 2^max_nq amplitudes per launched
+//     workgroup: min(kNoisyGrid, work items)).
 //
 // Channel c of trajectory t of output row i draws the uniform variate
 // philox(seed, shot = t, stream = (i, c)); results depend on the seed only,
@@ -65,8 +66,13 @@ struct NoisyPlan {
   // Device image layout (byte offsets).
   size_t off_counters = 0, off_circuits = 0, off_ops = 0, off_mats = 0, off_terms = 0,
          off_chunks = 0, off_ident = 0, off_nsamp = 0, off_empty = 0, off_nacc = 0, total = 0;
+  // Workgroups of the launch: min(kNoisyGrid, chunks), at least 1.
+  int grid = 1;
+  // Dynamic LDS of the launch (log2 amplitudes): the widest workgroup
+  // trajectory that keeps its state in LDS, at least the wave-mode buffers.
+  int lds_log2 = 0;
   // Workgroup state buffers (programs wider than kNoisyLdsMaxQubits only):
-  // kNoisyGrid buffers of 2^state_log2 amplitudes at off_states, else 0.
+  // `grid` buffers of 2^state_log2 amplitudes at off_states, else 0.
   size_t off_states = 0;
   int state_log2 = 0;
   size_t image = 0;

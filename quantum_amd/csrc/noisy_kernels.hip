@@ -1,8 +1,9 @@
 // Noisy trajectory kernel for gfx950 (see noisy.h).
 //
 // One persistent launch pulls work items (DevChunk) from two queues:
-//   1. workgroup chunks: one trajectory of an 11..13-qubit circuit, the state
-//      in LDS, channels through workgroup reductions;
+//   1. workgroup chunks: one trajectory of an 11..20-qubit circuit, channels
+//      through workgroup reductions; the state is in LDS up to 14 qubits and
+//      in the workgroup's own workspace buffer for 15..20 qubits;
 //   2. wave chunks: up to 2^(10 - n) trajectories of an n <= 10 qubit circuit
 //      side by side in one wavefront's registers (register engine with
 //      L = 10: the bits above n index the trajectory), channels through
@@ -432,12 +433,13 @@ int NoisyLaunch(NoisyPlan* p, void* ws, void* output, hipStream_t stream, char* 
   if (out_count == 0) return 0;
   if (a.nchunks > 0) {
     // Dynamic LDS: 8 wavefronts of 2^10-amplitude buffers (64 KiB, two
-    // workgroups per CU), or the widest workgroup state when that is larger
-    // (a 14-qubit state takes 128 KiB, one workgroup per CU; workgroups that
-    // find no room start when others end and then find the queues empty).
-    const int grid = std::max(1, std::min(kNoisyGrid, a.nchunks));
-    const int lds_log2 = std::max(kWaveMaxQubits + 3, std::min(p->max_nq, kNoisyLdsMaxQubits));
-    const size_t lds = (size_t(1) << lds_log2) * sizeof(float2);
+    // workgroups per CU), or the widest workgroup state kept in LDS when that
+    // is larger (a 14-qubit state takes 128 KiB, one workgroup per CU;
+    // workgroups that find no room start when others end and then find the
+    // queues empty).  Workspace trajectories (over 14 qubits) need no LDS
+    // state.  The workspace holds one state buffer per launched workgroup.
+    const int grid = p->grid;
+    const size_t lds = (size_t(1) << p->lds_log2) * sizeof(float2);
     switch (p->mode) {
       case kModeNoisyExpectation:
         hipLaunchKernelGGL(noisy_kernel<kModeNoisyExpectation>, dim3(grid), dim3(kThreads), lds, stream, a);

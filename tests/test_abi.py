@@ -174,3 +174,18 @@ def test_fuser_replica_pinned_by_hand_example():
     want = [((2, 3), False, (0, 1)), ((1,), False, (2,)), ((1, 2), False, (3,)), ((0,), False, (4,))]
     assert fuser.fuse(fuser.program_masks(data), 2) == want
     assert _lib.fusion_decisions(data, width=2) == want
+
+
+@pytest.mark.parametrize("n,T,grid", [(20, 1, 1), (16, 3, 3), (15, 700, 512)])
+def test_noisy_workspace_holds_one_state_per_launched_workgroup(n, T, grid):
+    """Programs over 14 qubits keep each workgroup trajectory's state in the
+    workspace: one 2^n buffer per launched workgroup, min(512, trajectories)."""
+    from quantum_amd import noise
+    q = C.LineQubit.range(n)
+    circ = C.Circuit([C.H(x) for x in q] + [C.depolarize(0.01).on(q[0])])
+    progs = S.convert_to_tensor([circ])
+    ps = S.convert_to_tensor([[C.PauliSum([C.pauli_z(q[0])])]])
+    plan, _ = noise.plan_expectation(progs, [], np.zeros((1, 0)), ps,
+                                     np.full((1, 1), T, np.int32), seed=1)
+    states = grid * 8 * 2 ** n
+    assert states <= plan.workspace_bytes < states + 4 * 2 ** 20

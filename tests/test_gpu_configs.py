@@ -1,5 +1,5 @@
-"""The benchmark configurations other than the uneven batch, at their full
-sizes, as parity cases.
+"""The benchmark configurations at their full sizes, as parity cases (the
+uneven batch has its own module, test_gpu_c4.py).
 
 * C1 (64 circuits, 10 qubits, 600 symbols): every expectation against the
   complex128 oracle (1e-5).
@@ -8,8 +8,9 @@ sizes, as parity cases.
   49-term sum ZZ and the ZZ estimate from 1e6 samples both agree with the exact
   expectation within their statistical error; every sample is a valid 30-bit
   string.
-* C3 (26 qubits, 416 symbols): the adjoint gradient against parameter shift
-  (rotation gates: [E(t + pi/2) - E(t - pi/2)] / 2) and central differences
+* C3 (26 qubits, 416 symbols, all 8 parameter sets): the adjoint gradient
+  against parameter shift on 32 columns (rotation gates: [E(t + pi/2) -
+  E(t - pi/2)] / 2, exact for ry/rz) at 5e-5 and central differences
   (h = 1e-3) computed with the exact expectation op; TFQ's finite-difference
   gradient gates agree with the exact derivative.
 """
@@ -96,30 +97,48 @@ def test_config2_samples_reproduce_the_zz_expectation(grid30):
 @pytest.fixture(scope="module")
 def hea26():
     circuit, names, values, ham = W.config3_hea()
-    return S.convert_to_tensor([circuit] * 2), names, values[:2], S.convert_to_tensor([[ham]] * 2)
+    B = values.shape[0]
+    return circuit, S.convert_to_tensor([circuit] * B), names, values, ham
 
 
 def test_config3_adjoint_matches_parameter_shift_and_central_difference(hea26):
+    """All 8 parameter sets; 32 symbol columns spread over the 8 layers (ry of
+    qubits 0, 13 and 25, rz of qubit 7, in every layer).  For ry/rz the two-term
+    parameter shift [E(t + pi/2) - E(t - pi/2)] / 2 is the exact derivative,
+    so the analytic adjoint must match it to the float32 resolution of two
+    26-qubit expectations of a 51-term observable (|E| <= 43.2): 5e-5.  All
+    shifted sets run as one 512-row batch."""
     from quantum_amd import tfq_adj_grad_op as adj
     from quantum_amd import tfq_simulate_ops as sim
-    progs, names, values, ps = hea26
-    ones = np.ones((2, 1), np.float32)
-    _say("C3 adjoint gradient, 26 qubits, 2 x 416 symbols")
+    circuit, progs, names, values, ham = hea26
+    B = values.shape[0]
+    ps = S.convert_to_tensor([[ham]] * B)
+    ones = np.ones((B, 1), np.float32)
+    _say("C3 adjoint gradient, 26 qubits, 8 x 416 symbols")
     g = adj.tfq_adj_grad(progs, names, values, ps, ones, derivative="analytic").cpu().numpy()
     g_tfq = adj.tfq_adj_grad(progs, names, values, ps, ones).cpu().numpy()
-    assert g.shape == (2, 416)
+    assert g.shape == (B, 416)
     # TFQ's float32 finite-difference gradient gates track the exact derivative.
     np.testing.assert_allclose(g_tfq, g, atol=5e-3)
-    cols = [0, 13, 51, 200, 415]
+    cols = sorted({l * 52 + o for l in range(8) for o in (0, 13, 25, 26 + 7)})
+    assert len(cols) == 32
+    shifted, where = [], []
     for c in cols:
-        vp = values.astype(np.float64).copy()
-        vm = vp.copy()
-        vp[:, c] += math.pi / 2
-        vm[:, c] -= math.pi / 2
-        _say(f"C3 parameter shift, symbol {c}")
-        ep = sim.tfq_simulate_expectation(progs, names, vp.astype(np.float32), ps).cpu().numpy()
-        em = sim.tfq_simulate_expectation(progs, names, vm.astype(np.float32), ps).cpu().numpy()
-        np.testing.assert_allclose(g[:, c], (ep - em)[:, 0] / 2, atol=1e-4)
+        for sign in (1, -1):
+            v = values.astype(np.float64).copy()
+            v[:, c] += sign * math.pi / 2
+            shifted.append(v.astype(np.float32))
+            where.append((c, sign))
+    vals = np.concatenate(shifted)
+    R = vals.shape[0]
+    _say(f"C3 parameter shift: {R} shifted 26-qubit expectations in one call")
+    e = sim.tfq_simulate_expectation(S.convert_to_tensor([circuit] * R), names, vals,
+                                     S.convert_to_tensor([[ham]] * R)).cpu().numpy()[:, 0]
+    e = e.reshape(len(cols), 2, B)
+    shift = (e[:, 0, :] - e[:, 1, :]).T / 2
+    err = float(np.abs(g[:, cols] - shift).max())
+    _say(f"C3 max |adjoint - parameter shift| = {err:.2e}")
+    np.testing.assert_allclose(g[:, cols], shift, atol=5e-5)
     for c in cols[:2]:
         h = 1e-3
         vp = values.astype(np.float64).copy()

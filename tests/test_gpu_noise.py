@@ -152,6 +152,26 @@ def test_channel_free_programs_equal_exact_expectation(n, batch):
     np.testing.assert_allclose(out, ref, atol=5e-4, rtol=5e-4)
 
 
+@pytest.mark.parametrize("n,T", [(20, 1), (18, 600)])
+def test_wide_channel_free_programs_in_workspace(n, T):
+    """Workspace trajectories at the widest sizes: one 20-qubit trajectory
+    (a workspace of one 8 MiB state), and 600 trajectories of an 18-qubit
+    program, more than the 512 workgroups of a launch, each exact."""
+    from quantum_amd import noise
+    rng = np.random.default_rng(n * 13 + T)
+    q = C.LineQubit.range(n)
+    circ = W.random_circuit(rng, q, 2 * n) + C.Circuit([C.I(x) for x in q])
+    progs = S.convert_to_tensor([circ])
+    ps = S.convert_to_tensor([[W.random_pauli_sum(rng, q, 3), C.PauliSum([C.pauli_z(q[0])])]])
+    ns = np.full((1, 2), T, np.int32)
+    plan, _ = noise.plan_expectation(progs, [], np.zeros((1, 0)), ps, ns, seed=5)
+    assert plan.workspace_bytes < (min(T, 512) + 1) * 8 * 2 ** n
+    _say(f"wide channel-free noisy expectation n={n} T={T}")
+    out = noise.expectation(progs, [], np.zeros((1, 0)), ps, ns, seed=5).cpu().numpy()
+    ref = O.simulate_expectation(list(progs), [], np.zeros((1, 0)), ps)
+    np.testing.assert_allclose(out, ref, atol=5e-4, rtol=5e-4)
+
+
 @pytest.mark.parametrize("n,T", [(10, 200), (7, 1000), (12, 64)])
 def test_every_trajectory_counts_once(n, T):
     """Regression: many work items over many workgroups (each trajectory of a

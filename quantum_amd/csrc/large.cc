@@ -190,8 +190,12 @@ int TileQubits(int mode, bool two_buffers) {
   const int want = two_buffers ? t.tile_two : t.tile_single;
   if (want > 0) return std::min(want, two_buffers ? 13 : kMaxTile);
   // Measured on the 30-qubit grid circuit: 13-qubit tiles need 20 passes
-  // instead of 28 at 12 qubits and finish the circuit 27% sooner.
-  return two_buffers ? 11 : 13;
+  // instead of 28 at 12 qubits and finish the circuit 27% sooner.  Two-state
+  // (adjoint) passes also take 13 qubits: the register engine needs tiles of
+  // 12-13 qubits, and on the 26-qubit, 416-symbol adjoint 13-qubit tiles take
+  // 48.7 ms per parameter set against 61.3 ms at 12 and 140 ms at 11 (LDS
+  // path, one barrier-synchronised reduction per derivative per tile).
+  return 13;
 }
 
 void AddPasses(LargePlan* lp, const std::vector<FusedOp>& ops, int n, int L, PassPlan* pp) {

@@ -10,9 +10,13 @@
 //   expect_kernel    XOR-mask Pauli reduction over the state, FP64 partials.
 //   lambda_kernel    lambda = sum_t w_t P_t psi (one streaming pass).
 //   ip_kernel        <psi|phi> with FP64 partials.
-//   cdf_* / shots_*  FP64 prefix of |psi|^2 in 4096-amplitude blocks plus a
-//                    scan of block sums; each shot draws a Philox uniform and
-//                    binary-searches blocks, then its block.
+//   prob_blocks / shots_*  sampling without a full CDF: one coalesced read of
+//                    the state gives, per 4096-amplitude block, the FP64
+//                    inclusive prefix of its 128 sub-block masses (32
+//                    amplitudes each) and the block total; a scan of the block
+//                    totals follows.  Each shot draws a Philox uniform,
+//                    binary-searches the blocks, then the block's sub-blocks,
+//                    and scans the 32 amplitudes of one sub-block.
 #include <hip/hip_runtime.h>
 
 #include <string.h>
@@ -29,6 +33,18 @@ namespace {
 
 constexpr int kMaxPassDerivs = 64;
 constexpr int kCdfBlockLog = 12;
+constexpr int kSubLog = 5;
+constexpr int kMaxSub = 1 << (kCdfBlockLog - kSubLog);
+// Terms of one measurement basis sampled by one launch (blockIdx.y).
+constexpr int kParityJobs = 64;
+
+// One term of a sampled expectation: its RNG stream, shot count and parity
+// mask (X, Y and Z qubits after the basis change).
+struct ParityJob {
+  uint32_t s_lo, s_hi;
+  int32_t nshots, pad;
+  uint64_t pmask;
+};
 
 struct PassArgs {
   const float2* src;
@@ -319,31 +335,51 @@ __global__ void rot1q_kernel(const float2* __restrict__ src, float2* __restrict_
   }
 }
 
-// Block-local inclusive FP64 prefix of |s|^2 and block totals.
-__global__ __launch_bounds__(kThreads) void cdf_block_kernel(const float2* __restrict__ s, int n,
-                                                             double* __restrict__ cdf,
-                                                             double* __restrict__ bsum) {
-  __shared__ double red[kWaves];
-  const int blog = min(n, kCdfBlockLog);
+__device__ __forceinline__ double prob(const float2 v) {
+  return double(v.x) * double(v.x) + double(v.y) * double(v.y);
+}
+
+__device__ __forceinline__ int sampler_block_log(int n) { return min(n, kCdfBlockLog); }
+__device__ __forceinline__ int sampler_sub_log(int n) { return min(sampler_block_log(n), kSubLog); }
+
+// Per block of 2^blog amplitudes (blog = min(n, 12)): the inclusive FP64
+// prefix of its sub-block masses (sub-blocks of 2^slog amplitudes, slog =
+// min(blog, 5)) in sincl[block * nsub + j], and the block total in bsum.
+// Threads read consecutive amplitudes (coalesced); a sub-block's lanes are
+// adjacent in one wavefront and sum by shuffles.
+__global__ __launch_bounds__(kThreads) void prob_blocks_kernel(const float2* __restrict__ s, int n,
+                                                               double* __restrict__ sincl,
+                                                               double* __restrict__ bsum) {
+  __shared__ double sub[kMaxSub];
+  __shared__ double wtot;
+  const int blog = sampler_block_log(n), slog = sampler_sub_log(n);
   const uint32_t BLK = 1u << blog;
+  const int nsub = 1 << (blog - slog);
+  const uint32_t width = 1u << slog;
   const uint64_t nb = uint64_t(1) << (n - blog);
-  const uint32_t chunk = (BLK + kThreads - 1) / kThreads;
+  const int tid = threadIdx.x, lane = tid & 63;
   for (uint64_t b = blockIdx.x; b < nb; b += gridDim.x) {
-    const uint64_t off = b * BLK;
-    const uint32_t lo = threadIdx.x * chunk, hi = min(BLK, lo + chunk);
-    double run = 0.0;
-    for (uint32_t i = lo; i < hi; ++i) {
-      const float2 v = s[off + i];
-      run += double(v.x) * double(v.x) + double(v.y) * double(v.y);
+    const float2* blk = s + b * BLK;
+    for (uint32_t x0 = 0; x0 < BLK; x0 += kThreads) {
+      const uint32_t x = x0 + uint32_t(tid);
+      double p = x < BLK ? prob(blk[x]) : 0.0;
+      for (uint32_t off = 1; off < width; off <<= 1) p += __shfl_xor(p, int(off), 64);
+      if (x < BLK && (uint32_t(lane) & (width - 1)) == 0) sub[x >> slog] = p;
     }
-    double total;
-    double pre = block_exclusive_scan(run, red, &total);
-    for (uint32_t i = lo; i < hi; ++i) {
-      const float2 v = s[off + i];
-      pre += double(v.x) * double(v.x) + double(v.y) * double(v.y);
-      cdf[off + i] = pre;
+    __syncthreads();
+    // Inclusive scan of the nsub <= 128 sub-block masses by the first two waves.
+    double v = tid < nsub ? sub[tid] : 0.0;
+    if (tid < 128) {
+      for (int off = 1; off < 64; off <<= 1) {
+        const double o = __shfl_up(v, off, 64);
+        if (lane >= off) v += o;
+      }
+      if (tid == 63) wtot = v;
     }
-    if (threadIdx.x == 0) bsum[b] = total;
+    __syncthreads();
+    if (tid >= 64 && tid < 128) v += wtot;
+    if (tid < nsub) sincl[b * uint64_t(nsub) + tid] = v;
+    if (tid == nsub - 1) bsum[b] = v;
     __syncthreads();
   }
 }
@@ -364,13 +400,17 @@ __global__ __launch_bounds__(kThreads) void cdf_scan_kernel(const double* __rest
   }
 }
 
-__device__ __forceinline__ uint64_t sample_index(const double* __restrict__ cdf,
+// Smallest x whose inclusive prefix mass exceeds u * total: block by binary
+// search of the block prefix, sub-block by binary search of the block's
+// sub-block prefix, amplitude by a scan of the sub-block.
+__device__ __forceinline__ uint64_t sample_index(const float2* __restrict__ s,
+                                                 const double* __restrict__ sincl,
                                                  const double* __restrict__ bincl, int n,
                                                  double u) {
-  const int blog = min(n, kCdfBlockLog);
+  const int blog = sampler_block_log(n), slog = sampler_sub_log(n);
   const uint64_t nb = uint64_t(1) << (n - blog);
-  const double total = bincl[nb - 1];
-  const double t = u * total;
+  const int nsub = 1 << (blog - slog);
+  const double t = u * bincl[nb - 1];
   uint64_t lo = 0, hi = nb - 1;
   while (lo < hi) {
     const uint64_t mid = (lo + hi) >> 1;
@@ -379,41 +419,52 @@ __device__ __forceinline__ uint64_t sample_index(const double* __restrict__ cdf,
   }
   const uint64_t b = lo;
   const double t2 = t - (b > 0 ? bincl[b - 1] : 0.0);
-  const double* c = cdf + (b << blog);
-  uint32_t l2 = 0, h2 = (1u << blog) - 1;
+  const double* c = sincl + b * uint64_t(nsub);
+  int l2 = 0, h2 = nsub - 1;
   while (l2 < h2) {
-    const uint32_t mid = (l2 + h2) >> 1;
+    const int mid = (l2 + h2) >> 1;
     if (c[mid] > t2) h2 = mid;
     else l2 = mid + 1;
   }
-  return (b << blog) + l2;
+  double acc = l2 > 0 ? c[l2 - 1] : 0.0;
+  const uint64_t x0 = (b << blog) + (uint64_t(l2) << slog);
+  const uint32_t width = 1u << slog;
+  for (uint32_t i = 0; i < width; ++i) {
+    acc += prob(s[x0 + i]);
+    if (acc > t2) return x0 + i;
+  }
+  return x0 + width - 1;
 }
 
 __global__ __launch_bounds__(kThreads) void shots_samples_kernel(
-    const double* __restrict__ cdf, const double* __restrict__ bincl, int n,
-    unsigned long long seed, uint32_t row, int nshots, int max_nq, signed char* __restrict__ out) {
+    const float2* __restrict__ psi, const double* __restrict__ sincl,
+    const double* __restrict__ bincl, int n, unsigned long long seed, uint32_t row, int nshots,
+    int max_nq, signed char* __restrict__ out) {
   for (int k = blockIdx.x * kThreads + threadIdx.x; k < nshots; k += gridDim.x * kThreads) {
     const double u = philox_uniform(seed, (unsigned long long)k, row, 0xFFFFFFFFu);
-    const uint64_t x = sample_index(cdf, bincl, n, u);
+    const uint64_t x = sample_index(psi, sincl, bincl, n, u);
     signed char* o = out + size_t(k) * max_nq;
     for (int q = 0; q < max_nq; ++q)
       o[max_nq - 1 - q] = q < n ? (signed char)((x >> q) & 1u) : (signed char)-2;
   }
 }
 
+// Terms jobs[blockIdx.y] of one measurement basis: partials[y * gridDim.x + x]
+// is workgroup x's sum of the +-1 parities of its share of the term's shots.
 __global__ __launch_bounds__(kThreads) void shots_parity_kernel(
-    const double* __restrict__ cdf, const double* __restrict__ bincl, int n,
-    unsigned long long seed, uint32_t s_lo, uint32_t s_hi, int nshots, uint64_t pmask,
-    double* partials) {
+    const float2* __restrict__ psi, const double* __restrict__ sincl,
+    const double* __restrict__ bincl, int n, unsigned long long seed,
+    const ParityJob* __restrict__ jobs, double* partials) {
   __shared__ double red[kWaves];
+  const ParityJob J = jobs[blockIdx.y];
   int par = 0;
-  for (int k = blockIdx.x * kThreads + threadIdx.x; k < nshots; k += gridDim.x * kThreads) {
-    const double u = philox_uniform(seed, (unsigned long long)k, s_lo, s_hi);
-    const uint64_t x = sample_index(cdf, bincl, n, u);
-    par += (__popcll(x & pmask) & 1) ? -1 : 1;
+  for (int k = blockIdx.x * kThreads + threadIdx.x; k < J.nshots; k += gridDim.x * kThreads) {
+    const double u = philox_uniform(seed, (unsigned long long)k, J.s_lo, J.s_hi);
+    const uint64_t x = sample_index(psi, sincl, bincl, n, u);
+    par += (__popcll(x & J.pmask) & 1) ? -1 : 1;
   }
   const double tot = block_sum(double(par), red);
-  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+  if (threadIdx.x == 0) partials[size_t(blockIdx.y) * gridDim.x + blockIdx.x] = tot;
 }
 
 // The sampler's uniforms for shots [start, start + count) of one stream, as
@@ -447,7 +498,7 @@ int StreamGrid() { return 2048; }
 // Workspace layout: the image (ops, mats, terms, param columns, identity
 // constants, per-term coefficients/shots) followed by scratch and state buffers.
 static void LargeLayout(LargePlan* p, size_t* off_pcols, size_t* off_ident, size_t* off_coefs,
-                        size_t* off_shots, size_t* off_red) {
+                        size_t* off_shots, size_t* off_jobs, size_t* off_red) {
   const int grid = StreamGrid();
   p->grid = grid;
   size_t off = 0;
@@ -459,6 +510,7 @@ static void LargeLayout(LargePlan* p, size_t* off_pcols, size_t* off_ident, size
   *off_ident = off; off = AlignUp(off + size_t(p->B) * nslots * sizeof(float));
   *off_coefs = off; off = AlignUp(off + p->terms.size() * sizeof(float));
   *off_shots = off; off = AlignUp(off + p->terms.size() * sizeof(int32_t));
+  *off_jobs = off; off = AlignUp(off + p->terms.size() * sizeof(ParityJob));
   p->image_bytes = off;
   const size_t slots = std::max<size_t>({p->param_cols.size(), size_t(2), size_t(64)});
   p->partial_slots = slots;
@@ -472,9 +524,11 @@ static void LargeLayout(LargePlan* p, size_t* off_pcols, size_t* off_ident, size
                    p->mode == kModeSampledExpectation || p->mode == kModeInnerProductGrad;
   p->off_buf2 = off;
   if (two) off = AlignUp(off + amps * sizeof(float2));
+  // Sampling: the within-block prefix of sub-block masses (one FP64 per 32
+  // amplitudes), then block totals and their prefix.
   p->off_cdf = off;
   const bool sampling = p->mode == kModeSamples || p->mode == kModeSampledExpectation;
-  if (sampling) off = AlignUp(off + amps * sizeof(double));
+  if (sampling) off = AlignUp(off + std::max<size_t>(1, amps >> kSubLog) * sizeof(double));
   p->off_blocks = off;
   if (sampling) {
     const size_t nb = std::max<size_t>(1, amps >> kCdfBlockLog);
@@ -484,12 +538,12 @@ static void LargeLayout(LargePlan* p, size_t* off_pcols, size_t* off_ident, size
 }
 
 struct LargeOffsets {
-  size_t pcols, ident, coefs, shots, red;
+  size_t pcols, ident, coefs, shots, jobs, red;
 };
 
 static LargeOffsets Offsets(LargePlan* p) {
   LargeOffsets o{};
-  LargeLayout(p, &o.pcols, &o.ident, &o.coefs, &o.shots, &o.red);
+  LargeLayout(p, &o.pcols, &o.ident, &o.coefs, &o.shots, &o.jobs, &o.red);
   return o;
 }
 
@@ -511,17 +565,23 @@ int LargeUpload(LargePlan* p, void* ws, hipStream_t stream, char* err, int64_t l
   const size_t nslots = size_t(p->mode == kModeInnerProduct ? p->K : p->O);
   std::vector<float> ident(size_t(p->B) * nslots, 0.0f), coefs(p->terms.size(), 0.0f);
   std::vector<int32_t> shots(p->terms.size(), 1);
+  std::vector<ParityJob> jobs(p->terms.size());
   for (const auto& lc : p->circuits) {
     for (size_t j = 0; j < lc.ident.size(); ++j) ident[size_t(lc.row) * nslots + j] = lc.ident[j];
     for (int t = lc.term_begin; t < lc.term_end; ++t) {
-      coefs[t] = p->terms[t].re;
+      const DevTermWide& T = p->terms[t];
+      coefs[t] = T.re;
       if (!p->num_samples.empty())
-        shots[t] = p->num_samples[size_t(lc.row) * p->O + p->terms[t].slot];
+        shots[t] = p->num_samples[size_t(lc.row) * p->O + T.slot];
+      // Stream (row, slot << 16 | term index), as the LDS-resident kernel's.
+      jobs[t] = ParityJob{uint32_t(lc.row), (uint32_t(T.slot) << 16) | uint32_t(T.pad), shots[t], 0,
+                          T.fmask | T.zmask};
     }
   }
   put(o.ident, ident.data(), ident.size() * sizeof(float));
   put(o.coefs, coefs.data(), coefs.size() * sizeof(float));
   put(o.shots, shots.data(), shots.size() * sizeof(int32_t));
+  put(o.jobs, jobs.data(), jobs.size() * sizeof(ParityJob));
   p->staging.swap(img);
   hipError_t e = hipMemcpyAsync(ws, p->staging.data(), p->staging.size(), hipMemcpyHostToDevice, stream);
   if (e != hipSuccess) return Err(err, len, std::string("hipMemcpyAsync: ") + hipGetErrorString(e));
@@ -542,7 +602,8 @@ int LargeLaunch(LargePlan* p, void* ws, void* output, hipStream_t stream, char* 
   double* gacc = reinterpret_cast<double*>(base + p->off_gacc);
   float2* psi = reinterpret_cast<float2*>(base + p->off_psi);
   float2* buf2 = reinterpret_cast<float2*>(base + p->off_buf2);
-  double* cdf = reinterpret_cast<double*>(base + p->off_cdf);
+  double* sincl = reinterpret_cast<double*>(base + p->off_cdf);
+  const ParityJob* jobs = reinterpret_cast<const ParityJob*>(base + o.jobs);
   const size_t amps_max = size_t(1) << p->max_nq;
   double* bsum = reinterpret_cast<double*>(base + p->off_blocks);
   double* bincl = bsum + std::max<size_t>(1, amps_max >> kCdfBlockLog);
@@ -571,6 +632,15 @@ int LargeLaunch(LargePlan* p, void* ws, void* output, hipStream_t stream, char* 
                          dim3(256), 0, stream, dst, int64_t(1) << n, make_float2(1.f, 0.f),
                          make_float2(0.f, 0.f));
     }
+  };
+
+  // The sampler's block sums of |s|^2 for an n-qubit state and their scan.
+  auto prob_blocks = [&](const float2* s, int n) {
+    const int blog = std::min(n, kCdfBlockLog);
+    const uint64_t nb = uint64_t(1) << (n - blog);
+    hipLaunchKernelGGL(prob_blocks_kernel, dim3(int(std::min<uint64_t>(nb, uint64_t(grid)))),
+                       dim3(kThreads), 0, stream, s, n, sincl, bsum);
+    hipLaunchKernelGGL(cdf_scan_kernel, dim3(1), dim3(kThreads), 0, stream, bsum, int64_t(nb), bincl);
   };
 
   for (const LargeCircuit& lc : p->circuits) {
@@ -700,29 +770,28 @@ int LargeLaunch(LargePlan* p, void* ws, void* output, hipStream_t stream, char* 
       hipLaunchKernelGGL(copy_state, dim3(grid), dim3(256), 0, stream, psi, n,
                          static_cast<float2*>(output) + size_t(row) * amps_max, int64_t(amps_max));
     } else if (p->mode == kModeSamples) {
-      const uint64_t nb = uint64_t(1) << (n - std::min(n, kCdfBlockLog));
-      hipLaunchKernelGGL(cdf_block_kernel, dim3(int(std::min<uint64_t>(nb, uint64_t(grid)))),
-                         dim3(kThreads), 0, stream, psi, n, cdf, bsum);
-      hipLaunchKernelGGL(cdf_scan_kernel, dim3(1), dim3(kThreads), 0, stream, bsum, int64_t(nb), bincl);
+      prob_blocks(psi, n);
       if (p->nshots > 0)
         hipLaunchKernelGGL(shots_samples_kernel,
                            dim3(std::max(1, std::min(grid, (p->nshots + kThreads - 1) / kThreads))),
-                           dim3(kThreads), 0, stream, cdf, bincl, n, (unsigned long long)p->seed,
-                           uint32_t(row), p->nshots, p->max_nq,
+                           dim3(kThreads), 0, stream, psi, sincl, bincl, n,
+                           (unsigned long long)p->seed, uint32_t(row), p->nshots, p->max_nq,
                            static_cast<signed char*>(output) + size_t(row) * p->nshots * p->max_nq);
     } else if (p->mode == kModeSampledExpectation) {
-      // Terms are ordered by slot, then by measurement basis: a CDF is rebuilt
-      // only when the basis changes.  Each term draws its own shots.
+      // Terms are ordered by slot, then by measurement basis: the sampler's
+      // block sums are rebuilt only when the basis changes, and the terms of
+      // one basis (up to kParityJobs) draw their shots in one launch, each
+      // from its own stream.
       uint64_t cur_f = ~uint64_t(0), cur_y = ~uint64_t(0);
-      int cur_slot = -1;
-      for (int t = lc.term_begin; t < lc.term_end; ++t) {
-        const DevTermWide& T = p->terms[t];
+      const float2* src = psi;
+      int t0 = lc.term_begin;
+      while (t0 < lc.term_end) {
+        const DevTermWide& T = p->terms[t0];
         const uint64_t yb = T.fmask & T.zmask;
-        if (T.fmask != cur_f || yb != cur_y || T.slot != cur_slot) {
+        if (T.fmask != cur_f || yb != cur_y) {
           cur_f = T.fmask;
           cur_y = yb;
-          cur_slot = T.slot;
-          const float2* src = psi;
+          src = psi;
           if (T.fmask) {
             bool first = true;
             for (int q = 0; q < n; ++q) {
@@ -738,20 +807,22 @@ int LargeLaunch(LargePlan* p, void* ws, void* output, hipStream_t stream, char* 
             }
             src = buf2;
           }
-          const uint64_t nb = uint64_t(1) << (n - std::min(n, kCdfBlockLog));
-          hipLaunchKernelGGL(cdf_block_kernel, dim3(int(std::min<uint64_t>(nb, uint64_t(grid)))),
-                             dim3(kThreads), 0, stream, src, n, cdf, bsum);
-          hipLaunchKernelGGL(cdf_scan_kernel, dim3(1), dim3(kThreads), 0, stream, bsum, int64_t(nb),
-                             bincl);
+          prob_blocks(src, n);
         }
-        const int ns = p->num_samples[size_t(row) * p->O + T.slot];
-        const int g = std::max(1, std::min(grid, (ns + kThreads - 1) / kThreads));
-        hipLaunchKernelGGL(shots_parity_kernel, dim3(g), dim3(kThreads), 0, stream, cdf, bincl, n,
-                           (unsigned long long)p->seed, uint32_t(row),
-                           (uint32_t(T.slot) << 16) | uint32_t(T.pad), ns, T.fmask | T.zmask,
-                           partials);
-        hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kThreads), 0, stream, partials, g, 1, 0,
-                           redbuf + (t - lc.term_begin));
+        // Terms [t0, te): the same basis, at most kParityJobs of them.
+        int te = t0 + 1;
+        int max_ns = p->num_samples[size_t(row) * p->O + T.slot];
+        while (te < lc.term_end && te - t0 < kParityJobs && p->terms[te].fmask == cur_f &&
+               (p->terms[te].fmask & p->terms[te].zmask) == cur_y) {
+          max_ns = std::max(max_ns, p->num_samples[size_t(row) * p->O + p->terms[te].slot]);
+          ++te;
+        }
+        const int g = std::max(1, std::min(grid, (max_ns + kThreads - 1) / kThreads));
+        hipLaunchKernelGGL(shots_parity_kernel, dim3(g, te - t0), dim3(kThreads), 0, stream, src,
+                           sincl, bincl, n, (unsigned long long)p->seed, jobs + t0, partials);
+        hipLaunchKernelGGL(reduce_kernel, dim3(te - t0), dim3(kThreads), 0, stream, partials, g, 1, g,
+                           redbuf + (t0 - lc.term_begin));
+        t0 = te;
       }
       int t = lc.term_begin;
       for (int64_t j = 0; j < nslots; ++j) {

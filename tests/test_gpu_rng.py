@@ -78,3 +78,43 @@ def test_samples_equal_oracle_except_at_rounding_boundaries():
         np.testing.assert_array_equal(got[safe], want[safe], err_msg=f"row {i}")
         decided += int(safe.sum())
     assert decided > 0.99 * batch * shots
+
+
+def test_streaming_sampler_matches_oracle_draws():
+    """The HBM-streaming sampler (block sums, sub-block prefixes, one
+    sub-block scan per shot) on a batch of 19-, 3- and 0-qubit programs:
+    samples equal the oracle's inverse-CDF draws from the same Philox stream
+    (a draw may move only if it lies within float32 rounding of a CDF
+    boundary, about 1e-6 chance per shot here), and the sampled expectation
+    of X/Y/Z terms (per-term basis change, per-term streams) equals the
+    oracle's to two shots' worth (2 * 2 / shots)."""
+    from quantum_amd import tfq_simulate_ops as sim
+    rng = np.random.default_rng(19)
+    names = ["s0", "s1"]
+    circuits, sums = [], []
+    for n in (19, 3, 0, 19):
+        if n == 0:
+            circuits.append(C.Circuit())
+            sums.append([C.PauliSum([]), C.PauliSum([])])
+            continue
+        q = C.LineQubit.range(n)
+        circuits.append(W.random_circuit(rng, q, 2 * n, symbols=names, p_control=0.1) +
+                        C.Circuit([C.I(x) for x in q]))
+        sums.append([W.random_pauli_sum(rng, q, 3, max_weight=3),
+                     W.random_pauli_sum(rng, q, 2, max_weight=2)])
+    vals = rng.uniform(-2, 2, size=(4, 2)).astype(np.float32)
+    progs, ps = S.convert_to_tensor(circuits), S.convert_to_tensor(sums)
+    shots, seed = 20000, 4242
+    _say("streaming samples n=19 vs oracle")
+    got = sim.tfq_simulate_samples(progs, names, vals, [shots], seed=seed).cpu().numpy()
+    want = O.simulate_samples(list(progs), names, vals, shots, seed)
+    assert got.shape == want.shape == (4, shots, 19)
+    rows_differ = np.any(got != want, axis=2).sum()
+    assert rows_differ <= 2, rows_differ
+    np.testing.assert_array_equal(got[2], -2)
+    ns = np.full((4, 2), shots, np.int32)
+    ns[:, 1] = 7000
+    _say("streaming sampled expectation n=19 vs oracle")
+    got_e = sim.tfq_simulate_sampled_expectation(progs, names, vals, ps, ns, seed=seed).cpu().numpy()
+    want_e = O.simulate_sampled_expectation(list(progs), names, vals, ps, ns, seed)
+    np.testing.assert_allclose(got_e, want_e, atol=4.0 / 7000)

@@ -293,6 +293,17 @@ int tfqx_plan_launch(tfqx_plan* plan, void* workspace, void* output, void* strea
 /* Same results, one kernel launch per circuit (the un-batched GPU path). */
 int tfqx_plan_launch_plain(tfqx_plan* plan, void* workspace, void* output, void* stream,
                            char* err, int64_t err_len);
+/* tfqx_plan_upload + tfqx_plan_launch with the workspace in the library's own
+ * static device array (32 MiB per GPU, held until the process ends) instead
+ * of a caller buffer: allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Plans
+ * whose workspace exceeds 32 MiB, or that take the HBM-streaming path, are
+ * rejected with TFQX_INVALID_ARGUMENT.  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (different streams or unordered
+ * host threads) race. */
+int tfqx_plan_launch_noalloc(tfqx_plan* plan, void* output, void* stream, char* err,
+                             int64_t err_len);
 /* Capture tfqx_plan_launch into a hipGraph and replay it. */
 int tfqx_graph_capture(tfqx_plan* plan, void* workspace, void* output, void* stream,
                        tfqx_graph** graph, char* err, int64_t err_len);

@@ -92,6 +92,8 @@ def _declare(lib):
     lib.tfqx_plan_upload.argtypes = [P, P, P, c.c_char_p, i64]
     lib.tfqx_plan_launch.argtypes = [P, P, P, P, c.c_char_p, i64]
     lib.tfqx_plan_launch_plain.argtypes = [P, P, P, P, c.c_char_p, i64]
+    lib.tfqx_plan_launch_noalloc.argtypes = [P, P, P, c.c_char_p, i64]
+    lib.tfqx_plan_launch_noalloc.restype = c.c_int
     lib.tfqx_graph_capture.argtypes = [P, P, P, P, P, c.c_char_p, i64]
     lib.tfqx_graph_launch.argtypes = [P, P, c.c_char_p, i64]
     lib.tfqx_graph_destroy.argtypes = [P]
@@ -143,6 +145,7 @@ EXPORTED_SYMBOLS = (
     "tfqx_plan_stats",
     "tfqx_plan_max_num_qubits", "tfqx_plan_workspace_bytes",
     "tfqx_plan_output_bytes", "tfqx_plan_upload", "tfqx_plan_launch", "tfqx_plan_launch_plain",
+    "tfqx_plan_launch_noalloc",
     "tfqx_graph_capture", "tfqx_graph_launch", "tfqx_graph_destroy", "tfqx_plan_destroy",
     "tfqx_set_fusion_width", "tfqx_set_wave_min_circuits", "tfqx_set_tuning", "tfqx_version", "tfqx_mlp_layout", "tfqx_mlp_set_params",
     "tfqx_mlp_predict", "tfqx_mlp_loss_grad", "tfqx_mlp_train_step", "tfqx_gemm_bf16")
@@ -229,6 +232,14 @@ class Plan:
         check(fn(self.handle, ctypes.c_void_p(self.workspace.data_ptr()),
                  ctypes.c_void_p(out.data_ptr() if out.numel() else 0),
                  ctypes.c_void_p(stream), err, 4096), err)
+
+    def launch_noalloc(self, out, stream):
+        """Upload and launch with the library's static device scratch (no
+        workspace; see include/tfqx.h: tfqx_plan_launch_noalloc)."""
+        err = ctypes.create_string_buffer(4096)
+        check(self._lib.tfqx_plan_launch_noalloc(
+            self.handle, ctypes.c_void_p(out.data_ptr() if out.numel() else 0),
+            ctypes.c_void_p(stream), err, 4096), err)
 
     def capture_graph(self, out, stream):
         err = ctypes.create_string_buffer(4096)

@@ -355,7 +355,13 @@ void tfqx_set_tuning(int32_t register_engine, int32_t tile_single, int32_t tile_
  * No reference counterpart. */
 void tfqx_set_wave_min_circuits(int32_t count);
 
-/* Fusion width (qubits per fused block) the planner uses; 0 = autotuned. */
+/* Fusion width (qubits per fused block) the planner uses; 0 = default (2).
+ * The batched kernel fuses at most 4 qubits.  On the HBM-streaming path, 5 or
+ * 6 fuses blocks of that many qubits in the forward passes and applies each
+ * such block as a dense (2^k x 2^k) . (2^k x 2^(n-k)) complex contraction on
+ * FP32 MFMA (one pass per block); the gates between them run in ordinary tile
+ * passes.  No reference counterpart: qsim's basic fuser is fixed at 2 qubits
+ * (core/src/circuit_parser_qsim.cc:857-859). */
 void tfqx_set_fusion_width(int32_t width);
 const char* tfqx_version(void);
 

@@ -2,7 +2,9 @@
 
 The planner has process-wide knobs (``include/tfqx.h``):
 
-* ``tfqx_set_fusion_width``: widest fused block (1..4; 2 is TFQ's basic fuser).
+* ``tfqx_set_fusion_width``: widest fused block (1..4 for the batched kernel;
+  2 is TFQ's basic fuser).  On the streaming path 5 or 6 also fuses blocks of
+  that many qubits and applies them as dense MFMA contractions.
 * ``tfqx_set_tuning``: register engine level (0 off, 1 batched kernel, 2
   batched and streaming passes) and the streaming path's tile qubits for one-
   and two-state passes.
@@ -38,6 +40,8 @@ STREAM_CANDIDATES = (
     {"fusion_width": 2, "register_engine": 1, "tile_single": 13},
     {"fusion_width": 2, "register_engine": 2, "tile_single": 12},
     {"fusion_width": 2, "register_engine": 1, "tile_single": 14},
+    {"fusion_width": 5, "register_engine": 2, "tile_single": 13},
+    {"fusion_width": 6, "register_engine": 2, "tile_single": 13},
 )
 
 DEFAULT = {"fusion_width": 0, "register_engine": 2, "tile_single": 0, "tile_two": 0}

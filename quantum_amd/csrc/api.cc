@@ -300,7 +300,7 @@ int BuildPlan(int mode, Inputs& in, int64_t B, int64_t S, int64_t O, int64_t K,
   if (p->max_nq > resident && !tiled_ok) {
     p->large = true;
     Status s = BuildLargePlan(mode, in.circuits, in.sums, downstream, in.others, B, S, O, K,
-                              tfq_fd, with_expectation, &p->large_plan);
+                              tfq_fd, with_expectation, &p->large_plan, g_fusion_width.load());
     if (!s.ok()) return Fail(err, len, s.code, s.msg);
     p->total = p->large_plan->workspace_bytes;
     *out = p.release();
@@ -763,7 +763,12 @@ int tfqx_plan_stats(const tfqx_plan* p, int64_t* out, int64_t n) {
       for (const auto& ps : c.fwd.passes) v[1] += real_ops(ps);
       for (const auto& ps : c.rev.passes) v[4] += real_ops(ps);
       const double A = std::ldexp(1.0, c.n);
-      for (const auto& ps : c.fwd.passes) mf += MfmaFlopsOfRange(lp.ops, ps.op_begin, ps.op_end, A, false);
+      for (const auto& ps : c.fwd.passes) {
+        mf += MfmaFlopsOfRange(lp.ops, ps.op_begin, ps.op_end, A, false);
+        // A dense pass: 8 * 2^k real flops per amplitude (the 2^(k+1)-square
+        // real form of the complex block times every column), all on MFMA.
+        if (ps.dense) mf += 8.0 * A * double(1 << lp.dense[size_t(ps.dense - 1)].k);
+      }
       for (const auto& ps : c.rev.passes) mf += MfmaFlopsOfRange(lp.ops, ps.op_begin, ps.op_end, A, true);
     }
     v[9] = int64_t(mf);

@@ -198,6 +198,39 @@ int TileQubits(int mode, bool two_buffers) {
   return 13;
 }
 
+// A dense pass for one fused block of 5-6 qubits: the tile holds its targets
+// and the lowest other qubits (kDenseTile in all, so rows stay contiguous).
+void AddDensePass(LargePlan* lp, const FusedOp& f, int n, PassPlan* pp) {
+  const int L = std::min(n, kDenseTile);
+  uint64_t T = 0;
+  for (int j = 0; j < f.k; ++j) T |= uint64_t(1) << f.bits[j];
+  for (int b = 0; b < n && Popc(T) < L; ++b) T |= uint64_t(1) << b;
+  DevPass p{};
+  p.L = L;
+  p.tilemask = T;
+  int local[64], j = 0;
+  for (int b = 0; b < 64; ++b) {
+    local[b] = -1;
+    if (T >> b & 1) {
+      p.tbit[j] = uint8_t(b);
+      local[b] = j++;
+    }
+  }
+  p.nlow = 0;
+  while (p.nlow < L && p.tbit[p.nlow] == p.nlow) p.nlow++;
+  p.op_begin = p.op_end = int(lp->ops.size());
+  DenseOp d;
+  d.k = f.k;
+  for (int i = 0; i < f.k; ++i) {
+    d.bits[i] = f.bits[i];
+    d.pos[i] = local[f.bits[i]];
+  }
+  d.mat = PushM(&lp->mats, f.M, false, 1 << f.k);
+  lp->dense.push_back(d);
+  p.dense = int(lp->dense.size());
+  pp->passes.push_back(p);
+}
+
 void AddPasses(LargePlan* lp, const std::vector<FusedOp>& ops, int n, int L, PassPlan* pp) {
   for (const auto& f : ops)
     if (Popc(OpMask(f)) > std::min(L, n)) lp->too_wide = true;
@@ -206,7 +239,39 @@ void AddPasses(LargePlan* lp, const std::vector<FusedOp>& ops, int n, int L, Pas
   // The streaming passes keep the LDS path unless tuning level 2 asks for the
   // register engine there too.
   SchedulePasses(ops, n, L, &passes, &lp->ops, &lp->mats, GetTuning().reg_engine >= 2);
-  pp->passes = passes;
+  pp->passes.insert(pp->passes.end(), passes.begin(), passes.end());
+}
+
+// Forward passes with wide fusion (`width` 5 or 6): blocks of 5-6 qubits
+// become dense MFMA passes; the gates of the smaller blocks between them are
+// fused again at width 2 and run as ordinary tile passes.  Blocks keep the
+// fuser's order, which is a valid application order.
+void AddPassesWide(LargePlan* lp, const Circuit& c, int width, int n, int L, PassPlan* pp) {
+  FuseOptions fw;
+  fw.max_fuse = width;
+  std::vector<FusedOp> blocks;
+  FuseGates(c, 0, int(c.gates.size()), fw, &blocks);
+  FuseOptions f2;
+  f2.max_fuse = 2;
+  Circuit run;
+  run.num_qubits = c.num_qubits;
+  auto flush = [&]() {
+    if (run.gates.empty()) return;
+    std::vector<FusedOp> small;
+    FuseGates(run, 0, int(run.gates.size()), f2, &small);
+    AddPasses(lp, small, n, L, pp);
+    run.gates.clear();
+  };
+  for (const FusedOp& b : blocks) {
+    const bool dense = b.k >= 5 && b.cmask == 0 && n >= kDenseTile && !b.diag;
+    if (!dense) {
+      for (int gi : b.gates) run.gates.push_back(c.gates[gi]);
+      continue;
+    }
+    flush();
+    AddDensePass(lp, b, n, pp);
+  }
+  flush();
 }
 
 }  // namespace
@@ -215,7 +280,8 @@ Status BuildLargePlan(int mode, const std::vector<Circuit>& circuits,
                       const std::vector<std::vector<std::vector<Pauli>>>& sums,
                       const float* downstream, const std::vector<std::vector<Circuit>>& others,
                       int64_t B, int64_t S, int64_t O, int64_t K, bool tfq_fd,
-                      bool with_expectation, std::unique_ptr<LargePlan>* out) {
+                      bool with_expectation, std::unique_ptr<LargePlan>* out,
+                      int wide_fusion) {
   auto lp = std::make_unique<LargePlan>();
   lp->mode = mode;
   lp->with_expectation = with_expectation;
@@ -237,9 +303,14 @@ Status BuildLargePlan(int mode, const std::vector<Circuit>& circuits,
     if (lc.empty) continue;
     const bool grad = mode == kModeAdjoint || mode == kModeInnerProductGrad;
     lc.L = std::min(lc.n, TileQubits(mode, grad));
-    std::vector<FusedOp> fused;
-    FuseGates(c, 0, int(c.gates.size()), fo, &fused);
-    AddPasses(lp.get(), fused, lc.n, std::min(lc.n, TileQubits(mode, false)), &lc.fwd);
+    if (wide_fusion >= 5) {
+      AddPassesWide(lp.get(), c, std::min(wide_fusion, kMaxFuseWide), lc.n,
+                    std::min(lc.n, TileQubits(mode, false)), &lc.fwd);
+    } else {
+      std::vector<FusedOp> fused;
+      FuseGates(c, 0, int(c.gates.size()), fo, &fused);
+      AddPasses(lp.get(), fused, lc.n, std::min(lc.n, TileQubits(mode, false)), &lc.fwd);
+    }
     // Terms.
     lc.term_begin = int(lp->terms.size());
     const bool sampled = mode == kModeSampledExpectation;

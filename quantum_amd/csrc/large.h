@@ -40,6 +40,16 @@ struct LargeCircuit {
   std::vector<float> ident;           // O identity constants
 };
 
+// A fused block of 5-6 qubits applied by one dense MFMA pass: matrix (2^k x
+// 2^k complex64, row-major, index bit i <-> global bit bits[i]) at `mat` in the
+// plan's matrix array; pos[i] is bits[i]'s position in the pass's tile.
+struct DenseOp {
+  int k = 0;
+  int bits[kMaxFuseWide];
+  int pos[kMaxFuseWide];
+  uint32_t mat = 0;
+};
+
 struct LargePlan {
   int mode = 0;
   int64_t B = 0, S = 0, O = 0, K = 0;
@@ -53,6 +63,7 @@ struct LargePlan {
   std::vector<DevPass> passes;
   std::vector<DevTermWide> terms;
   std::vector<int32_t> param_cols;    // symbol column of every (param op, derivative)
+  std::vector<DenseOp> dense;         // dense passes (DevPass::dense - 1 indexes this)
   // Workspace layout.
   size_t off_ops = 0, off_mats = 0, off_passes = 0, off_terms = 0, off_partials = 0,
          off_gacc = 0, off_psi = 0, off_buf2 = 0, off_cdf = 0, off_blocks = 0,
@@ -69,7 +80,12 @@ Status BuildLargePlan(int mode, const std::vector<Circuit>& circuits,
                       const std::vector<std::vector<std::vector<Pauli>>>& sums,
                       const float* downstream, const std::vector<std::vector<Circuit>>& others,
                       int64_t B, int64_t S, int64_t O, int64_t K, bool tfq_fd,
-                      bool with_expectation, std::unique_ptr<LargePlan>* out);
+                      bool with_expectation, std::unique_ptr<LargePlan>* out,
+                      int wide_fusion = 0);
+
+// Dense passes use tiles of kDenseTile qubits: 2^(kDenseTile - k) columns of
+// the fused block's 2^k target amplitudes per tile.
+constexpr int kDenseTile = 12;
 
 // Group fused ops into tile passes (exposed for tests of the scheduler).
 void SchedulePasses(const std::vector<FusedOp>& ops, int n, int L, std::vector<DevPass>* passes,

@@ -180,6 +180,161 @@ __global__ __launch_bounds__(kThreads, TWO ? 2 : 4) void pass_kernel(PassArgs a)
   }
 }
 
+// ------------------------------------------------------------------ dense passes
+// One fused block of K = 5 or 6 qubits applied to the whole state as a dense
+// contraction on FP32 MFMA.  In real form the block is
+//   [Yr; Yi] = [[Gr, -Gi]; [Gi, Gr]] [Xr; Xi],
+// a (2D x 2D) . (2D x C) product per tile (D = 2^K target amplitudes, C =
+// 2^(kDenseTile - K) columns: the other tile qubits), run as 16x16 output
+// tiles of v_mfma_f32_16x16x4f32 (A: lane l holds row l%16, k-row l/16; B:
+// k-row l/16, column l%16; D: rows 4(l/16)..+3 of column l%16).  There are 32
+// output tiles; each wavefront owns four with one row tile, whose slice of the
+// real matrix (2D/4 values per lane) stays in registers for the launch.  A
+// tile moves global -> LDS -> MFMA operands -> LDS -> global; the LDS index of
+// (target row r, column c) is rowoff[r] | coloff[c].  Each thread's share of
+// the tile two ahead is loaded into registers while the current tile's MFMAs
+// run, and its tile offsets are computed once per launch.  K = 5 fits 128
+// VGPRs, so two workgroups share a CU (four tiles' loads in flight per CU);
+// K = 6 holds a 32-value matrix slice and runs one workgroup per CU.
+struct DenseArgs {
+  float2* psi;
+  const float2* G;  // D x D complex, row-major, index bit i <-> pos[i]
+  DevPass pass;
+  int n;
+  int pos[kMaxFuseWide];
+};
+
+template <int K>
+__global__ __launch_bounds__(kThreads, K == 5 ? 4 : 2) void dense_kernel(DenseArgs a) {
+  constexpr int D = 1 << K, D2 = 2 * D, L = kDenseTile, C = 1 << (L - K);
+  constexpr int RT = D2 / 16, CT = C / 16, KS = D2 / 4, TPW = 4;
+  constexpr int PER = (1 << L) / kThreads;  // amplitudes each thread moves per tile
+  static_assert(RT * CT == TPW * kWaves, "32 output tiles, four per wavefront");
+  extern __shared__ float2 lds[];
+  __shared__ uint32_t rowoff[D], coloff[C];
+  __shared__ uint64_t tab[kTileTable];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int rt = (w * TPW) / CT, ct0 = (w * TPW) % CT;
+  if (tid < D) {
+    uint32_t o = 0;
+#pragma unroll
+    for (int i = 0; i < K; ++i) o |= uint32_t((tid >> i) & 1) << a.pos[i];
+    rowoff[tid] = o;
+  }
+  if (tid < C) {
+    uint32_t tmask = 0;
+#pragma unroll
+    for (int i = 0; i < K; ++i) tmask |= 1u << a.pos[i];
+    uint32_t o = 0;
+    int src = 0;
+    for (int b = 0; b < L; ++b) {
+      if ((tmask >> b) & 1u) continue;
+      o |= uint32_t((tid >> src) & 1) << b;
+      ++src;
+    }
+    coloff[tid] = o;
+  }
+  const DevPass ps = a.pass;
+  build_tile_table(ps, tab);
+  // This lane's A operands: real-form rows rt*16 + lane%16, k-rows 4s + lane/16.
+  float areg[KS];
+  {
+    const int R = rt * 16 + (lane & 15), r = R & (D - 1);
+    const bool rim = R >= D;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const int kc = 4 * s + (lane >> 4), c = kc & (D - 1);
+      const bool kim = kc >= D;
+      const float2 g = a.G[r * D + c];
+      areg[s] = !rim ? (!kim ? g.x : -g.y) : (!kim ? g.y : g.x);
+    }
+  }
+  __syncthreads();
+  // Tile-invariant part of the global index of this thread's amplitudes
+  // j = tid + kThreads * i (tile index bits deposited onto the tile's global
+  // bits); a tile adds its base, whose bits are all outside the tile.
+  const bool use_tab = ps.L - ps.nlow <= 8;
+  uint32_t off[PER];  // within-state offsets: states of up to 32 qubits
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const uint32_t j = uint32_t(tid + kThreads * i);
+    off[i] = uint32_t(use_tab ? tile_index_tab(0, j, ps.nlow, tab) : tile_index(0, j, ps));
+  }
+  const uint64_t ntiles = uint64_t(1) << (a.n - L);
+  float* Lf = reinterpret_cast<float*>(lds);
+  const int cl = lane & 15, kq = lane >> 4;
+  uint32_t co[TPW];
+#pragma unroll
+  for (int j = 0; j < TPW; ++j) co[j] = coloff[(ct0 + j) * 16 + cl];
+  // Row offsets of this lane's k-rows 4s + kq (s < D/4): the real part of a
+  // loaded amplitude is the operand of k-step s, its imaginary part that of
+  // k-step s + D/4.
+  uint32_t ro[D / 4];
+#pragma unroll
+  for (int s = 0; s < D / 4; ++s) ro[s] = rowoff[4 * s + kq];
+  // Tile t's body: its amplitudes (loaded earlier into `pre`) go to LDS, the
+  // loads of tile `tn` (two tiles ahead) are issued into `pre`, the MFMAs run,
+  // and the result goes back to global memory.
+  auto body = [&](uint64_t t, uint64_t tn, float2 (&pre)[PER]) {
+    const uint64_t base = tile_base(t, ps.tilemask, a.n);
+#pragma unroll
+    for (int i = 0; i < PER; ++i) lds[tid + kThreads * i] = pre[i];
+    __syncthreads();
+    if (tn < ntiles) {
+      const uint64_t nb = tile_base(tn, ps.tilemask, a.n);
+#pragma unroll
+      for (int i = 0; i < PER; ++i) pre[i] = a.psi[nb | off[i]];
+    }
+    mf4 acc[TPW];
+#pragma unroll
+    for (int j = 0; j < TPW; ++j) acc[j] = mf4{0.f, 0.f, 0.f, 0.f};
+    // One column tile at a time: its operands are loaded before the MFMAs
+    // that use them.
+#pragma unroll
+    for (int j = 0; j < TPW; ++j) {
+      float2 bv[D / 4];
+#pragma unroll
+      for (int s = 0; s < D / 4; ++s) bv[s] = lds[ro[s] | co[j]];
+#pragma unroll
+      for (int s = 0; s < D / 4; ++s)
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(areg[s], bv[s].x, acc[j], 0, 0, 0);
+#pragma unroll
+      for (int s = 0; s < D / 4; ++s)
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(areg[s + D / 4], bv[s].y, acc[j], 0, 0, 0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int R = rt * 16 + 4 * kq + i;
+      const uint32_t rw = rowoff[R & (D - 1)];
+      const uint32_t part = R >= D ? 1u : 0u;
+#pragma unroll
+      for (int j = 0; j < TPW; ++j) Lf[2u * (rw | co[j]) + part] = acc[j][i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PER; ++i) a.psi[base | off[i]] = lds[tid + kThreads * i];
+    // Every thread has read its LDS values before the next tile overwrites them.
+    __syncthreads();
+  };
+  // Two tiles' loads in flight per workgroup: this workgroup's tiles alternate
+  // between the register sets preA and preB.
+  const uint64_t G = gridDim.x;
+  float2 preA[PER], preB[PER];
+  for (int h = 0; h < 2; ++h) {
+    const uint64_t t = blockIdx.x + h * G;
+    if (t < ntiles) {
+      const uint64_t base = tile_base(t, ps.tilemask, a.n);
+#pragma unroll
+      for (int i = 0; i < PER; ++i) (h ? preB : preA)[i] = a.psi[base | off[i]];
+    }
+  }
+  for (uint64_t t = blockIdx.x; t < ntiles; t += 2 * G) {
+    body(t, t + 2 * G, preA);
+    if (t + G < ntiles) body(t + G, t + 3 * G, preB);
+  }
+}
+
 __global__ __launch_bounds__(kThreads) void expect_kernel(const float2* __restrict__ psi, int n,
                                                           const DevTermWide* __restrict__ terms,
                                                           int tb, int te, double* partials) {
@@ -613,6 +768,27 @@ int LargeLaunch(LargePlan* p, void* ws, void* output, hipStream_t stream, char* 
   auto run_passes = [&](const PassPlan& pp, int n, float2* dst, bool init_first) {
     bool first = init_first;
     for (const DevPass& ps : pp.passes) {
+      if (ps.dense) {
+        const DenseOp& d = p->dense[size_t(ps.dense - 1)];
+        if (first)
+          hipLaunchKernelGGL(fill_row_c64,
+                             dim3(std::min<int64_t>(grid, ((int64_t(1) << n) + 255) / 256)),
+                             dim3(256), 0, stream, dst, int64_t(1) << n, make_float2(1.f, 0.f),
+                             make_float2(0.f, 0.f));
+        DenseArgs da{};
+        da.psi = dst;
+        da.G = mats + d.mat;
+        da.pass = ps;
+        da.n = n;
+        for (int i = 0; i < d.k; ++i) da.pos[i] = d.pos[i];
+        const uint64_t ntiles = uint64_t(1) << (n - ps.L);
+        const int g = int(std::min<uint64_t>(ntiles, uint64_t(grid)));
+        const size_t lds = (size_t(1) << kDenseTile) * sizeof(float2);
+        if (d.k == 5) hipLaunchKernelGGL(dense_kernel<5>, dim3(g), dim3(kThreads), lds, stream, da);
+        else hipLaunchKernelGGL(dense_kernel<6>, dim3(g), dim3(kThreads), lds, stream, da);
+        first = false;
+        continue;
+      }
       PassArgs a{};
       a.src = dst;
       a.dst = dst;

@@ -143,7 +143,13 @@ void ApplyLeft(const Gate& g, const int* bits, int k, double* acc /* re,im inter
 void BlockMatrix(const std::vector<Gate>& gs, const int* gates, int ngates, const int* bits, int k,
                  cdouble* out) {
   const int dim = 1 << k;
-  double acc[2 * 256];
+  double small[2 * 256];
+  std::vector<double> big;
+  double* acc = small;
+  if (dim > 16) {
+    big.resize(size_t(2) * dim * dim);
+    acc = big.data();
+  }
   for (int i = 0; i < 2 * dim * dim; ++i) acc[i] = 0;
   for (int i = 0; i < dim; ++i) acc[2 * (i * dim + i)] = 1.0;
   for (int t = 0; t < ngates; ++t) ApplyLeft(gs[gates[t]], bits, k, acc);

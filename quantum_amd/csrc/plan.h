@@ -23,7 +23,7 @@ using cfloat = std::complex<float>;
 // A fused block on global bit positions (ascending); matrix index bit j <-> bits[j].
 struct FusedOp {
   int k = 0;
-  int bits[kMaxFuse];
+  int bits[kMaxFuseWide];
   uint64_t cmask = 0, cval = 0;   // controls kept outside the matrix
   std::vector<cdouble> M;
   bool diag = false;

@@ -13,6 +13,9 @@ namespace tfqx {
 
 // Maximum target qubits of one fused op applied in LDS.
 constexpr int kMaxFuse = 4;
+// Widest fused block: the streaming path applies blocks of 5-6 qubits as dense
+// MFMA contractions (every other kernel takes at most kMaxFuse).
+constexpr int kMaxFuseWide = 6;
 // Maximum derivative matrices attached to one parameterised gate (FSIM theta+phi,
 // PhasedX / PhasedISwap phase+exponent).
 constexpr int kMaxDeriv = 2;
@@ -101,6 +104,10 @@ struct alignas(8) DevPass {
   int32_t nlow;       // tile bits 0..nlow-1 are global bits 0..nlow-1
   uint8_t tbit[kMaxTile + 2];
   uint64_t tilemask;  // global bits covered by the tile
+  // Streaming path: 1 + index of the dense (k >= 5 qubit) fused gate this pass
+  // applies as an MFMA contraction instead of ops [op_begin, op_end); 0 if none.
+  int32_t dense;
+  int32_t pad;
 };
 
 // One circuit of a batched launch (LDS-resident: n <= L of the launch).

@@ -26,6 +26,27 @@ def _say(msg):
     print(f"[gpu-case] {msg}", file=sys.stderr, flush=True)
 
 
+def assert_draws_match(bits, program, names, vals, seed, row, min_gap=1e-6):
+    """Samples [shots, max_nq] of one row equal the oracle's inverse-CDF draws
+    from the Philox stream (row, 0xFFFFFFFF), except draws whose uniform lies
+    within `min_gap` of a CDF boundary (float32 state rounding decides those);
+    returns the number of draws checked."""
+    p = O.Program(program, O._symbol_maps(names, vals)[0])
+    probs = np.abs(O.simulate(p)) ** 2
+    cdf = np.cumsum(probs)
+    shots = bits.shape[0]
+    u = philox.uniform_doubles(seed, shots, row, 0xFFFFFFFF) * cdf[-1]
+    want = np.minimum(np.searchsorted(cdf, u, side="right"), len(cdf) - 1)
+    pos = np.searchsorted(cdf, u)
+    gap = np.minimum(np.abs(u - cdf[np.minimum(pos, len(cdf) - 1)]),
+                     np.abs(u - cdf[np.maximum(pos - 1, 0)]))
+    safe = gap > min_gap
+    got = (bits[:, ::-1][:, :p.n].astype(np.int64) << np.arange(p.n)).sum(axis=1)
+    np.testing.assert_array_equal(got[safe], want[safe])
+    assert np.all(bits[:, :bits.shape[1] - p.n] == -2)
+    return int(safe.sum())
+
+
 @pytest.fixture(scope="module", autouse=True)
 def _need_gpu():
     import torch
@@ -84,8 +105,8 @@ def test_streaming_sampler_matches_oracle_draws():
     """The HBM-streaming sampler (block sums, sub-block prefixes, one
     sub-block scan per shot) on a batch of 19-, 3- and 0-qubit programs:
     samples equal the oracle's inverse-CDF draws from the same Philox stream
-    (a draw may move only if it lies within float32 rounding of a CDF
-    boundary, about 1e-6 chance per shot here), and the sampled expectation
+    (a draw may move only if its uniform lies within 1e-6 of a CDF boundary,
+    where float32 rounding of the state decides), and the sampled expectation
     of X/Y/Z terms (per-term basis change, per-term streams) equals the
     oracle's to two shots' worth (2 * 2 / shots)."""
     from quantum_amd import tfq_simulate_ops as sim
@@ -107,10 +128,10 @@ def test_streaming_sampler_matches_oracle_draws():
     shots, seed = 20000, 4242
     _say("streaming samples n=19 vs oracle")
     got = sim.tfq_simulate_samples(progs, names, vals, [shots], seed=seed).cpu().numpy()
-    want = O.simulate_samples(list(progs), names, vals, shots, seed)
-    assert got.shape == want.shape == (4, shots, 19)
-    rows_differ = np.any(got != want, axis=2).sum()
-    assert rows_differ <= 2, rows_differ
+    assert got.shape == (4, shots, 19)
+    checked = sum(assert_draws_match(got[i], progs[i], names, vals[i:i + 1], seed, i)
+                  for i in (0, 1, 3))
+    assert checked > 0.9 * 3 * shots
     np.testing.assert_array_equal(got[2], -2)
     ns = np.full((4, 2), shots, np.int32)
     ns[:, 1] = 7000

@@ -104,7 +104,8 @@ int tfqx_plan_simulate_samples(
  * trajectory t of row i draws Philox4x32-10 uniform (seed, shot t, stream
  * (i, c)); the reference seeds from tensorflow::random::New64().  Circuits of
  * at most 20 qubits (TFQX_UNIMPLEMENTED beyond; programs of 15-20 qubits add
- * 512 x 2^max_qubits x 8 bytes of workspace). */
+ * one 2^max_qubits x 8-byte state per launched workgroup, min(512, work
+ * items), to the workspace). */
 int tfqx_plan_noisy_expectation(
     const uint8_t* programs, const int64_t* program_offsets, int64_t batch,
     const uint8_t* symbol_names, const int64_t* symbol_offsets, int64_t num_symbols,

@@ -581,6 +581,43 @@ def vqe_measure(stream, steps=3000, ref_steps=4000, seed=1005):
     return res
 
 
+def op_call_rate(progs, names, values, ps, weights, calls=5):
+    """End-to-end rate of the op a training loop calls on the uneven batch:
+    ``expectation_and_gradient`` (host plan + upload + launch, until the
+    results are on the device).  The first call plans; later calls with new
+    symbol values and downstream gradients reuse the plan (rebind + upload +
+    launch)."""
+    import torch
+    from quantum_amd import plan_cache
+    from quantum_amd import tfq_adj_grad_op as adj
+    plan_cache.clear()
+    rng = np.random.default_rng(7)
+    B = len(progs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    adj.expectation_and_gradient(progs, names, values, ps, weights)
+    torch.cuda.synchronize()
+    cold = time.perf_counter() - t0
+    times = []
+    for _ in range(calls):
+        v = (values + rng.normal(scale=0.01, size=values.shape)).astype(np.float32)
+        g = (weights * rng.uniform(0.5, 1.5, size=weights.shape)).astype(np.float32)
+        t0 = time.perf_counter()
+        adj.expectation_and_gradient(progs, names, v, ps, g)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    warm = float(np.median(times))
+    plan_cache.clear()
+    log(f"op calls: first {cold * 1e3:.0f} ms ({B / cold:.0f} circuits/s), reused plan "
+        f"{warm * 1e3:.1f} ms ({B / warm:.0f} circuits/s)")
+    return {"op": "expectation_and_gradient (tfq_adj_grad with the forward pass)",
+            "first_call_ms": round(cold * 1e3, 1), "first_call_circuits_per_s": round(B / cold, 1),
+            "reused_plan_call_ms": round(warm * 1e3, 2),
+            "reused_plan_circuits_per_s": round(B / warm, 1), "calls_timed": calls,
+            "note": "wall clock per call until the outputs are on the device; reused-plan calls "
+                    "change every symbol value and downstream gradient"}
+
+
 def measured_traffic(circuits, flops):
     """HBM bytes per launch from the committed rocprofv3 counter summary, if it
     was taken on this same workload (same circuit count and flop count)."""
@@ -634,14 +671,15 @@ def main():
                     help="skip the autotuner (the --only profiling runs never tune: they use "
                          "the planner defaults)")
     ap.add_argument("--only", choices=["all", "batch", "hbm", "copy", "ipgrad", "noisy", "ops",
-                                       "c2", "c3", "c5", "vqe"],
+                                       "c2", "c3", "c5", "vqe", "opcalls"],
                     default="all",
                     help="profiling runs: 'batch' launches only the batched kernel (graph "
                          "replays, no plain launches, no CPU baseline); 'hbm' only the 30-qubit "
                          "streaming passes; 'copy' a 1 GiB device copy (counter calibration); "
                          "'ipgrad' the inner-product gradient batch; 'c2' the 30-qubit samples "
                          "and sampled expectation; 'c3' the 26-qubit adjoint; 'c5' the "
-                         "surrogate network's kernels; 'vqe' the surrogate-assisted VQE")
+                         "surrogate network's kernels; 'vqe' the surrogate-assisted VQE; "
+                         "'opcalls' the end-to-end op call rate on the uneven batch")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines of the C2/C3 "
                     "legs")
     args = ap.parse_args()
@@ -697,6 +735,10 @@ def main():
               flush=True)
         return
     progs, names, values, ps, weights = build_uneven(args.circuits)
+    if args.only == "opcalls":
+        print(json.dumps({"op_calls": op_call_rate(progs, names, values, ps, weights)}),
+              flush=True)
+        return
     tuned = None
     if args.only == "all" and not args.no_autotune:
         from quantum_amd import autotune as AT
@@ -763,6 +805,7 @@ def main():
         if batch_setting:
             from quantum_amd import autotune as AT
             AT.apply(batch_setting)
+    op_calls = op_call_rate(progs, names, values, ps, weights)
     ipg = inner_product_grad_measure(stream)
     noisy = noisy_measure(stream)
     other_ops = ops_measure(stream)
@@ -802,6 +845,7 @@ def main():
         "max_abs_err_vs_oracle": err,
         "oracle_rows_checked": checked,
         "max_abs_err_vs_cpu_baseline": err_cpu,
+        "op_calls": op_calls,
         "roofline": roof,
         "hbm_pass": hbm,
         "inner_product_grad": ipg,

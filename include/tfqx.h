@@ -198,13 +198,22 @@ int tfqx_plan_expectation_and_gradient(
     tfqx_plan** plan, char* err, int64_t err_len);
 
 /* Re-evaluate a plan for new symbol values [batch, num_symbols] without
- * re-parsing or re-fusing; call tfqx_plan_upload again before launching.  Only
- * for plans whose programs are one byte-identical LDS-resident program tiled
- * over the batch (TFQ layers tile one circuit over the batch,
- * python/layers/circuit_executors/expectation.py:231-390); other plans return
- * TFQX_INVALID_ARGUMENT.  No reference counterpart: the reference re-parses
- * every call (core/ops/parse_context.cc:60-92). */
+ * re-parsing, re-resolving or re-fusing: only the matrices of ops that contain
+ * symbolic gates are rebuilt.  Call tfqx_plan_upload again before launching.
+ * Works for every plan of the batch kernel (LDS-resident, wave-mode and tiled
+ * circuits, one program tiled over the batch or a different program per row);
+ * streaming and noisy plans return TFQX_INVALID_ARGUMENT.  No reference
+ * counterpart: the reference re-parses every call
+ * (core/ops/parse_context.cc:60-92). */
 int tfqx_plan_rebind(tfqx_plan* plan, const float* symbol_values, char* err, int64_t err_len);
+
+/* New downstream gradients [batch, num_ops] for an adjoint plan of the batch
+ * kernel (tfqx_plan_adjoint_gradient / tfqx_plan_expectation_and_gradient):
+ * rewrites the lambda weights g_ij * c_t, zero where |g_ij * c_t| < 1e-5 as
+ * AccumulateOperators drops them (core/src/util_qsim.h:379-383).  Call
+ * tfqx_plan_upload again before launching.  No reference counterpart. */
+int tfqx_plan_rebind_downstream(tfqx_plan* plan, const float* downstream_grads, char* err,
+                                int64_t err_len);
 
 /* Fusion decisions of the planner for one program (host only): the fused ops
  * of the forward program at the given width (1..4).  ops_out[i][8] = {k,

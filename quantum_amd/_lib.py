@@ -76,6 +76,8 @@ def _declare(lib):
     lib.tfqx_debug_philox_uniform.restype = c.c_int
     lib.tfqx_plan_rebind.argtypes = [P, P, c.c_char_p, i64]
     lib.tfqx_plan_rebind.restype = c.c_int
+    lib.tfqx_plan_rebind_downstream.argtypes = [P, P, c.c_char_p, i64]
+    lib.tfqx_plan_rebind_downstream.restype = c.c_int
     lib.tfqx_plan_stats.argtypes = [P, P, i64]
     lib.tfqx_plan_stats.restype = c.c_int
     for name in ("tfqx_plan_simulate_expectation", "tfqx_plan_simulate_sampled_expectation",
@@ -141,6 +143,7 @@ EXPORTED_SYMBOLS = (
     "tfqx_strings_data", "tfqx_strings_offsets", "tfqx_strings_dim", "tfqx_strings_destroy",
     "tfqx_plan_inner_product",
     "tfqx_plan_inner_product_grad", "tfqx_plan_adjoint_gradient", "tfqx_plan_expectation_and_gradient", "tfqx_plan_rebind",
+    "tfqx_plan_rebind_downstream",
     "tfqx_debug_fuse", "tfqx_debug_philox_uniform",
     "tfqx_plan_stats",
     "tfqx_plan_max_num_qubits", "tfqx_plan_workspace_bytes",
@@ -206,12 +209,20 @@ class Plan:
         return dict(zip(keys, [int(x) for x in out]))
 
     def rebind(self, values):
-        """New symbol values (float32 [batch, n_symbols]) for a plan built from one
-        program tiled over the batch; upload again before launching."""
+        """New symbol values (float32 [batch, n_symbols]) for a batch-kernel plan
+        (not streaming or noisy); upload again before launching."""
         v = np.ascontiguousarray(values, dtype=np.float32)
         err = ctypes.create_string_buffer(4096)
         check(self._lib.tfqx_plan_rebind(self.handle, v.ctypes.data if v.size else None, err, 4096),
               err)
+
+    def rebind_downstream(self, grads):
+        """New downstream gradients (float32 [batch, n_ops]) for an adjoint plan of
+        the batch kernel; upload again before launching."""
+        g = np.ascontiguousarray(grads, dtype=np.float32)
+        err = ctypes.create_string_buffer(4096)
+        check(self._lib.tfqx_plan_rebind_downstream(self.handle, g.ctypes.data if g.size else None,
+                                                    err, 4096), err)
 
     @property
     def workspace_bytes(self):
@@ -320,13 +331,21 @@ def set_wave_min_circuits(count=-1):
     """Smallest count of 4-10 qubit circuits run one per wavefront (negative:
     the default 1024; 0: always)."""
     lib().tfqx_set_wave_min_circuits(int(count))
+    _settings_changed()
 
 
 def set_tuning(register_engine=2, tile_single=0, tile_two=0):
     """Planner tuning for later plans (see include/tfqx.h: tfqx_set_tuning).
     register_engine: 0 off, 1 batched kernel, 2 batched and streaming (default)."""
     lib().tfqx_set_tuning(int(register_engine), int(tile_single), int(tile_two))
+    _settings_changed()
 
 
 def set_fusion_width(width):
     lib().tfqx_set_fusion_width(int(width))
+    _settings_changed()
+
+
+def _settings_changed():
+    from . import plan_cache
+    plan_cache.bump_generation()

@@ -104,6 +104,18 @@ struct tfqx_plan {
   bool shared = false;
   Circuit tmpl;
   SharedRecipe recipe;
+  // Any other batch-kernel plan: per-row recipes of the ops with symbolic
+  // gates, so tfqx_plan_rebind re-evaluates them for new values.
+  bool rebindable = false;
+  std::vector<RowRebind> rows_rebind;
+  // Adjoint plans: every row's lambda terms in emission order (slot, Pauli
+  // coefficient, number of Y factors), for tfqx_plan_rebind_downstream.
+  struct LamTerm {
+    int slot;
+    float coef;
+    int ny;
+  };
+  std::vector<std::vector<LamTerm>> lam_meta;
   // States wider than one LDS tile run on the HBM-streaming path.
   bool large = false;
   std::unique_ptr<LargePlan> large_plan;
@@ -316,13 +328,34 @@ int BuildPlan(int mode, Inputs& in, int64_t B, int64_t S, int64_t O, int64_t K,
     jobs[i].row = int(i);
   }
   const int slots = (mode == kModeInnerProduct || mode == kModeInnerProductGrad) ? int(K) : int(O);
+  // Plans can be rebound to new symbol values (and, for the adjoint, new
+  // downstream weights): op lists decide diagonal storage from gate kinds and
+  // keep every lambda term.
+  opt.fuse.structural_diag = true;
+  opt.keep_all_lambda = mode == kModeAdjoint;
   if (in.shared) {
     p->shared = true;
     p->tmpl = in.circuits[0];
     BuildSharedBatchProgram(p->tmpl, jobs, in.values, int(S), slots, opt, HostThreads(), &p->prog,
                             &p->recipe);
   } else {
-    BuildBatchProgram(jobs, slots, int(S), opt, HostThreads(), &p->prog);
+    std::vector<SharedRecipe> recipes;
+    BuildBatchProgram(jobs, slots, int(S), opt, HostThreads(), &p->prog, &recipes);
+    p->rows_rebind.resize(size_t(B));
+    ParallelFor(int(B), [&](int i) {
+      p->rows_rebind[size_t(i)] = CompactRecipe(in.circuits[size_t(i)], recipes[size_t(i)]);
+    });
+    p->rebindable = true;
+  }
+  if (mode == kModeAdjoint) {
+    p->lam_meta.resize(size_t(B));
+    for (int64_t i = 0; i < B; ++i) {
+      const Circuit& c = in.circuits[in.shared ? 0 : size_t(i)];
+      if (c.gates.empty() || in.sums[size_t(i)].empty()) continue;
+      for (int64_t j = 0; j < O; ++j)
+        for (const auto& pt : in.sums[size_t(i)][size_t(j)])
+          p->lam_meta[size_t(i)].push_back({int(j), pt.coef, pt.ny});
+    }
   }
   p->lds_log2 = std::max(1, std::min(p->max_nq, resident));
   // Wave mode carves the full LDS allocation into one pair of 2^kWaveMaxQubits
@@ -733,14 +766,48 @@ int tfqx_debug_philox_uniform(uint64_t seed, uint64_t start, int64_t count, uint
 
 int tfqx_plan_rebind(tfqx_plan* p, const float* values, char* err, int64_t len) {
   if (!p || (!values && p->S > 0)) return Fail(err, len, TFQX_INVALID_ARGUMENT, "null argument");
-  if (!p->shared)
+  if (p->shared) {
+    RebindSharedBatchProgram(p->tmpl, p->recipe, values, int(p->S), p->tfq_fd, HostThreads(),
+                             &p->prog);
+  } else if (p->rebindable) {
+    RebindBatchProgram(p->rows_rebind, values, int(p->S), p->tfq_fd, HostThreads(), &p->prog);
+  } else {
     return Fail(err, len, TFQX_INVALID_ARGUMENT,
-                "Plan cannot be rebound: its programs are not one LDS-resident program tiled over "
-                "the batch.");
-  RebindSharedBatchProgram(p->tmpl, p->recipe, values, int(p->S), p->tfq_fd, HostThreads(),
-                           &p->prog);
+                "Plan cannot be rebound: it runs on the HBM-streaming path or is a noisy plan.");
+  }
   if (!p->prog.mats.empty())
     memcpy(p->staging.data() + p->off_mats, p->prog.mats.data(), p->prog.mats.size() * sizeof(cfloat));
+  return TFQX_OK;
+}
+
+int tfqx_plan_rebind_downstream(tfqx_plan* p, const float* downstream, char* err, int64_t len) {
+  if (!p || (!downstream && p->B * p->O > 0))
+    return Fail(err, len, TFQX_INVALID_ARGUMENT, "null argument");
+  if (p->mode != kModeAdjoint || p->large || p->noisy || p->lam_meta.size() != size_t(p->B))
+    return Fail(err, len, TFQX_INVALID_ARGUMENT,
+                "Plan cannot be rebound to new downstream gradients: it is not an adjoint plan "
+                "of the batch kernel.");
+  for (const DevCircuit& dc : p->prog.circuits) {
+    const auto& meta = p->lam_meta[size_t(dc.row)];
+    if (size_t(dc.lam_end - dc.lam_begin) != meta.size())
+      return Fail(err, len, TFQX_INTERNAL, "lambda term count mismatch");
+    for (int k = dc.lam_begin; k < dc.lam_end; ++k) {
+      const auto& m = meta[size_t(k - dc.lam_begin)];
+      float w = downstream[size_t(dc.row) * p->O + m.slot] * m.coef;
+      // AccumulateOperators (util_qsim.h:379-383) skips |g c| < 1e-5.
+      if (std::fabs(w) < 1e-5f) w = 0.0f;
+      float re = w, im = 0.0f;
+      for (int y = 0; y < (m.ny & 3); ++y) {
+        const float nr = -im, ni = re;
+        re = nr;
+        im = ni;
+      }
+      DevTerm& t = p->prog.terms[size_t(k)];
+      t.re = re;
+      t.im = im;
+      memcpy(p->staging.data() + p->off_terms + size_t(k) * sizeof(DevTerm), &t, sizeof(DevTerm));
+    }
+  }
   return TFQX_OK;
 }
 

@@ -50,7 +50,7 @@ DevTermWide WideTerm(const Pauli& p, float w, int slot, bool phase_y, int index)
 
 void SchedulePasses(const std::vector<FusedOp>& ops, int n, int L, std::vector<DevPass>* passes,
                     std::vector<DevOp>* dev_ops, std::vector<cfloat>* mats,
-                    bool register_engine) {
+                    bool register_engine, const Circuit* c, SharedRecipe* rec) {
   L = std::min(L, n);
   const int low_run = std::min(kMinLowRun, L);
   const uint64_t lowmask = (uint64_t(1) << low_run) - 1;
@@ -173,6 +173,7 @@ void SchedulePasses(const std::vector<FusedOp>& ops, int n, int L, std::vector<D
         d.type = f.diag ? kOpDiag : kOpUnitary;
         d.mat = PushM(mats, f.M, f.diag, dim);
       }
+      if (rec && c) AppendRecipe(f, *c, d, rec);
       dev_ops->push_back(d);
     }
     if (register_engine) InsertRemaps(dev_ops, size_t(p.op_begin), L);

@@ -88,9 +88,12 @@ Status BuildLargePlan(int mode, const std::vector<Circuit>& circuits,
 constexpr int kDenseTile = 12;
 
 // Group fused ops into tile passes (exposed for tests of the scheduler).
+// With `rec` (and the circuit `c` the ops come from), each op's matrix recipe
+// is recorded for rebinding.
 void SchedulePasses(const std::vector<FusedOp>& ops, int n, int L, std::vector<DevPass>* passes,
                     std::vector<DevOp>* dev_ops, std::vector<cfloat>* mats,
-                    bool register_engine);
+                    bool register_engine, const Circuit* c = nullptr,
+                    SharedRecipe* rec = nullptr);
 
 size_t LargeWorkspaceBytes(LargePlan* p);
 int LargeUpload(LargePlan* p, void* ws, hipStream_t stream, char* err, int64_t len);

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <numeric>
 #include <thread>
+#include <unordered_map>
 
 namespace tfqx {
 namespace {
@@ -377,6 +378,24 @@ void FuseGates(const Circuit& c, int begin, int end, const FuseOptions& opt,
   for (size_t bi = 0; bi < blocks.size(); ++bi) close(int(bi));
 }
 
+void AppendRecipe(const FusedOp& f, const Circuit& c, const DevOp& d, SharedRecipe* rec) {
+  OpRecipe r;
+  r.kind = f.param ? 2 : (f.cmask ? 1 : 0);
+  r.k = uint8_t(f.k);
+  r.diag = !f.param && f.diag;
+  for (int j = 0; j < f.k; ++j) r.bits[j] = f.bits[j];
+  r.gate_begin = int(rec->gates.size());
+  for (int gi : f.gates) {
+    rec->gates.push_back(gi);
+    if (c.gates[gi].nsym > 0) r.symbolic = true;
+  }
+  r.gate_end = int(rec->gates.size());
+  r.mat = d.mat;
+  r.nderiv = f.param ? f.nderiv : 0;
+  for (int i = 0; i < r.nderiv; ++i) r.dmat[i] = d.dmat[i];
+  rec->ops.push_back(r);
+}
+
 namespace {
 
 // One circuit's program, with offsets relative to its own arrays.
@@ -433,23 +452,7 @@ DevOp MakeDevOp(const FusedOp& f, std::vector<cfloat>* mats, const Circuit* c = 
     d.type = f.diag ? kOpDiag : kOpUnitary;
     d.mat = PushMatrix(mats, f.M, f.diag, dim);
   }
-  if (rec) {
-    OpRecipe r;
-    r.kind = f.param ? 2 : (f.cmask ? 1 : 0);
-    r.k = uint8_t(f.k);
-    r.diag = !f.param && f.diag;
-    for (int j = 0; j < f.k; ++j) r.bits[j] = f.bits[j];
-    r.gate_begin = int(rec->gates.size());
-    for (int gi : f.gates) {
-      rec->gates.push_back(gi);
-      if (c->gates[gi].nsym > 0) r.symbolic = true;
-    }
-    r.gate_end = int(rec->gates.size());
-    r.mat = d.mat;
-    r.nderiv = f.param ? f.nderiv : 0;
-    for (int i = 0; i < r.nderiv; ++i) r.dmat[i] = d.dmat[i];
-    rec->ops.push_back(r);
-  }
+  if (rec) AppendRecipe(f, *c, d, rec);
   return d;
 }
 
@@ -506,7 +509,10 @@ double AddTerms(const CircuitJob& job, int num_slots, const PlanOptions& opt, Pi
       const float gj = job.downstream[j];
       for (const auto& p : (*job.sums)[j]) {
         const float w = gj * p.coef;
-        if (std::fabs(w) < 1e-5f) continue;
+        if (std::fabs(w) < 1e-5f) {
+          if (opt.keep_all_lambda) pc->terms.push_back(MakeTerm(p, 0.0f, 0));
+          continue;
+        }
         pc->terms.push_back(MakeTerm(p, w, 0));
       }
     }
@@ -545,7 +551,7 @@ void BuildPiece(const CircuitJob& job, int num_slots, const PlanOptions& opt, Pi
   dc.fwd_begin = 0;
   dc.fwd_pass_begin = 0;
   if (tiled) {
-    SchedulePasses(fused, dc.n, opt.tile_log2, &pc->passes, &pc->ops, &pc->mats, true);
+    SchedulePasses(fused, dc.n, opt.tile_log2, &pc->passes, &pc->ops, &pc->mats, true, rc, rec);
   } else {
     for (const auto& f : fused) pc->ops.push_back(MakeDevOp(f, &pc->mats, rc, rec));
     remaps(size_t(dc.fwd_begin));
@@ -591,7 +597,8 @@ void BuildPiece(const CircuitJob& job, int num_slots, const PlanOptions& opt, Pi
       seg_end = pg[p - 1];
     }
     if (tiled) {
-      SchedulePasses(rev_ops, dc.n, opt.tile_log2, &pc->passes, &pc->ops, &pc->mats, true);
+      SchedulePasses(rev_ops, dc.n, opt.tile_log2, &pc->passes, &pc->ops, &pc->mats, true, rc,
+                     rec);
     } else {
       for (const auto& f : rev_ops) pc->ops.push_back(MakeDevOp(f, &pc->mats, rc, rec));
       remaps(size_t(dc.rev_begin));
@@ -769,11 +776,81 @@ double CircuitCost(const DevCircuit& c, const BatchProgram& p) {
 }
 
 void BuildBatchProgram(const std::vector<CircuitJob>& jobs, int num_slots, int num_symbols,
-                       const PlanOptions& opt, int threads, BatchProgram* out) {
+                       const PlanOptions& opt, int threads, BatchProgram* out,
+                       std::vector<SharedRecipe>* recipes) {
   const int B = int(jobs.size());
   std::vector<Piece> pieces(B);
-  RunParallel(B, threads, [&](int i) { BuildPiece(jobs[i], num_slots, opt, &pieces[i]); });
+  if (recipes) recipes->assign(size_t(B), SharedRecipe());
+  RunParallel(B, threads, [&](int i) {
+    BuildPiece(jobs[i], num_slots, opt, &pieces[i],
+               recipes ? &(*recipes)[size_t(jobs[i].row)] : nullptr);
+  });
   Concatenate(pieces, num_slots, num_symbols, opt, out);
+}
+
+RowRebind CompactRecipe(const Circuit& c, const SharedRecipe& rec) {
+  RowRebind out;
+  std::unordered_map<int, int> where;
+  for (const OpRecipe& r : rec.ops) {
+    if (!r.symbolic) continue;
+    OpRecipe q = r;
+    q.gate_begin = int(out.recipe.gates.size());
+    for (int i = r.gate_begin; i < r.gate_end; ++i) {
+      const int gi = rec.gates[size_t(i)];
+      auto it = where.find(gi);
+      if (it == where.end()) {
+        const Gate& g = c.gates[size_t(gi)];
+        RebindGate rg{};
+        rg.kind = g.kind;
+        rg.nt = int8_t(g.nt);
+        rg.nsym = int8_t(g.nsym);
+        for (int j = 0; j < 2; ++j) {
+          rg.bits[j] = g.bits[j];
+          rg.sym_pidx[j] = int8_t(j < g.nsym ? g.sym_pidx[j] : 0);
+          rg.sym_col[j] = j < g.nsym ? g.sym_col[j] : 0;
+        }
+        rg.cmask = g.cmask;
+        rg.cval = g.cval;
+        for (int j = 0; j < 5; ++j) rg.params[j] = g.params[j];
+        it = where.emplace(gi, int(out.gates.size())).first;
+        out.gates.push_back(rg);
+      }
+      out.recipe.gates.push_back(it->second);
+    }
+    q.gate_end = int(out.recipe.gates.size());
+    out.recipe.ops.push_back(q);
+  }
+  return out;
+}
+
+void RebindBatchProgram(const std::vector<RowRebind>& rows, const float* values, int num_symbols,
+                        bool tfq_fd, int threads, BatchProgram* prog) {
+  const int B = prog->num_rows;
+  RunParallel(B, threads, [&](int row) {
+    const RowRebind& rr = rows[size_t(row)];
+    if (rr.recipe.ops.empty()) return;
+    const float* v = values + size_t(row) * num_symbols;
+    std::vector<Gate> gs(rr.gates.size());
+    for (size_t i = 0; i < gs.size(); ++i) {
+      const RebindGate& r = rr.gates[i];
+      Gate& g = gs[i];
+      g.kind = r.kind;
+      g.nt = r.nt;
+      g.nsym = r.nsym;
+      g.bits[0] = r.bits[0];
+      g.bits[1] = r.bits[1];
+      g.cmask = r.cmask;
+      g.cval = r.cval;
+      for (int j = 0; j < 5; ++j) g.params[j] = r.params[j];
+      for (int j = 0; j < 2; ++j) {
+        g.sym_pidx[j] = r.sym_pidx[j];
+        g.sym_col[j] = r.sym_col[j];
+      }
+      for (int j = 0; j < g.nsym; ++j) g.params[g.sym_pidx[j]] = v[g.sym_col[j]];
+      GateUnitary(g.kind, g.params, g.U);
+    }
+    RebindRow(gs, rr.recipe, tfq_fd, prog->mats.data() + prog->row_mat_base[size_t(row)]);
+  });
 }
 
 void BuildSharedBatchProgram(const Circuit& tmpl, const std::vector<CircuitJob>& jobs,

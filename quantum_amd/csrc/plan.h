@@ -63,6 +63,10 @@ struct SharedRecipe {
   size_t mats_per_row = 0;
 };
 
+// Record how to rebuild the matrices of device op `d` (made from fused block
+// `f` of circuit `c`, matrices at d.mat / d.dmat in the row's array).
+void AppendRecipe(const FusedOp& f, const Circuit& c, const DevOp& d, SharedRecipe* rec);
+
 // Fuse gates [begin, end) of a circuit into blocks (deterministic greedy
 // frontier fuser: gates join the open block holding their qubits while the
 // union stays within max_fuse qubits; disjoint open blocks merge when a gate
@@ -129,11 +133,40 @@ struct PlanOptions {
   bool tfq_fd = true;     // adjoint derivative: TFQ's float32 FD (true) or exact
   int tile_log2 = 30;     // circuits wider than this run tiled (state in global scratch)
   bool wave = false;      // small circuits (kWaveRegBits..kWaveMaxQubits qubits) in wave mode
+  // Keep every adjoint lambda term, with weight 0 where TFQ drops it (|g c| <
+  // 1e-5), so new downstream weights can be written into the same term list.
+  bool keep_all_lambda = false;
 };
 
-// Build the per-circuit program pieces and concatenate them.
+// Build the per-circuit program pieces and concatenate them.  With `recipes`,
+// row r's matrix recipe goes to (*recipes)[r] (relative to the row's first
+// matrix, BatchProgram::row_mat_base), so RebindBatchProgram can re-evaluate
+// the matrices for new symbol values; the caller then sets
+// opt.fuse.structural_diag so the op list does not depend on the values.
 void BuildBatchProgram(const std::vector<CircuitJob>& jobs, int num_slots, int num_symbols,
-                       const PlanOptions& opt, int threads, BatchProgram* out);
+                       const PlanOptions& opt, int threads, BatchProgram* out,
+                       std::vector<SharedRecipe>* recipes = nullptr);
+// What re-evaluating one row's matrices needs: the recipes of its ops that
+// contain symbolic gates, and those ops' gates in compact form (no matrix;
+// recipe gate indices point into `gates`).
+struct RebindGate {
+  GateKind kind;
+  int8_t nt, nsym;
+  int8_t sym_pidx[2];
+  int bits[2];
+  uint64_t cmask, cval;
+  float params[5];
+  int sym_col[2];
+};
+struct RowRebind {
+  std::vector<RebindGate> gates;
+  SharedRecipe recipe;
+};
+// Keep the symbolic ops of `rec` (built from circuit `c`) with their gates.
+RowRebind CompactRecipe(const Circuit& c, const SharedRecipe& rec);
+// Re-evaluate every row's symbolic matrices for new values [rows, num_symbols].
+void RebindBatchProgram(const std::vector<RowRebind>& rows, const float* values, int num_symbols,
+                        bool tfq_fd, int threads, BatchProgram* prog);
 
 // Rows whose programs are byte-identical: `tmpl` is the program resolved with
 // row 0's values; the op list is built once and each row re-evaluates only the

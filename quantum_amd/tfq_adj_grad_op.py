@@ -45,13 +45,44 @@ def plan_adj_grad(programs, symbol_names, symbol_values, pauli_sums, prev_grad,
     return plan, (programs.shape[0], len(names))
 
 
+def _cached(programs, symbol_names, symbol_values, pauli_sums, prev_grad, derivative,
+            with_expectation):
+    """Launch an adjoint plan, reusing a recent call's plan when the programs,
+    names and Pauli sums are the same (new values and downstream weights are
+    rebound; see ``plan_cache``)."""
+    from . import plan_cache
+    from .tfq_simulate_ops import _device_and_stream
+    programs, names, values = _common(programs, symbol_names, symbol_values)
+    pauli_sums = as_string_tensor(pauli_sums, 2, "pauli_sums")
+    grads = as_values(prev_grad, "downstream_grads")
+    _, stream = _device_and_stream()
+    key = ("adjoint", plan_cache.digest(programs, pauli_sums), tuple(names), programs.shape,
+           pauli_sums.shape, grads.shape, derivative, with_expectation, stream)
+    box = {}
+
+    def build():
+        plan, box["shape"] = plan_adj_grad(programs, names, values, pauli_sums, grads,
+                                           derivative, with_expectation)
+        return plan
+
+    def rebind(plan):
+        plan.rebind(values)
+        plan.rebind_downstream(grads)
+
+    def launch(plan):
+        if "shape" not in box:
+            B, O, S = programs.shape[0], pauli_sums.shape[1], len(names)
+            box["shape"] = (B * (O + S),) if with_expectation else (B, S)
+        return _run(plan, box["shape"], _torch().float32)
+
+    return plan_cache.run(key, build, rebind, launch)
+
+
 def expectation_and_gradient(programs, symbol_names, symbol_values, pauli_sums, prev_grad,
                              derivative="tfq"):
     """Expectations ``[batch, n_ops]`` and adjoint gradients ``[batch, n_symbols]``
     from one persistent launch (the forward state is computed once)."""
-    plan, shape = plan_adj_grad(programs, symbol_names, symbol_values, pauli_sums, prev_grad,
-                                derivative, with_expectation=True)
-    flat = _run(plan, shape, _torch().float32)
+    flat = _cached(programs, symbol_names, symbol_values, pauli_sums, prev_grad, derivative, True)
     B = len(programs)
     O = flat.numel() // max(B, 1) - len(list(symbol_names)) if B else 0
     exp = flat[:B * O].reshape(B, O)
@@ -61,6 +92,4 @@ def expectation_and_gradient(programs, symbol_names, symbol_values, pauli_sums, 
 
 def tfq_adj_grad(programs, symbol_names, symbol_values, pauli_sums, prev_grad,
                  derivative="tfq"):
-    plan, shape = plan_adj_grad(programs, symbol_names, symbol_values, pauli_sums, prev_grad,
-                                derivative)
-    return _run(plan, shape, _torch().float32)
+    return _cached(programs, symbol_names, symbol_values, pauli_sums, prev_grad, derivative, False)

@@ -140,9 +140,23 @@ def plan_expectation(programs, symbol_names, symbol_values, pauli_sums):
 
 
 def tfq_simulate_expectation(programs, symbol_names, symbol_values, pauli_sums):
-    """Expectation values ``[batch, n_ops]`` (float32); -2 for empty programs."""
-    plan, shape = plan_expectation(programs, symbol_names, symbol_values, pauli_sums)
-    return _run(plan, shape, _torch().float32)
+    """Expectation values ``[batch, n_ops]`` (float32); -2 for empty programs.
+
+    A call with the same programs, names and Pauli sums as a recent one reuses
+    its plan with the new symbol values (``plan_cache``)."""
+    from . import plan_cache
+    programs, names, values = _common(programs, symbol_names, symbol_values)
+    pauli_sums = as_string_tensor(pauli_sums, 2, "pauli_sums")
+    _, stream = _device_and_stream()
+    key = ("expectation", plan_cache.digest(programs, pauli_sums), tuple(names),
+           programs.shape, pauli_sums.shape, stream)
+    shape = (programs.shape[0], pauli_sums.shape[1])
+
+    def build():
+        return plan_expectation(programs, names, values, pauli_sums)[0]
+
+    return plan_cache.run(key, build, lambda plan: plan.rebind(values),
+                          lambda plan: _run(plan, shape, _torch().float32))
 
 
 def tfq_simulate_state(programs, symbol_names, symbol_values):

@@ -126,7 +126,10 @@ def test_channel_gate_id_rejected():
              Ps.ptr, Ps.off, 1)
 
 
-def test_rebind_only_for_one_program_tiled_over_the_batch():
+def test_rebind_for_batch_kernel_plans_but_not_streaming_plans():
+    """Rebinding keeps the plan's structure (op and pass counts) for one
+    program tiled over the batch, for a batch of different programs and for
+    tiled circuits; streaming plans (19+ qubits) are refused."""
     circuit, names, q = W.hea_ring(6, 2)
     ham = W.tfim(q, 0.7)
     rng = np.random.default_rng(0)
@@ -137,14 +140,20 @@ def test_rebind_only_for_one_program_tiled_over_the_batch():
     assert shared.stats() == stats
     other = W.hea_ring(6, 1)[0]
     mixed = _plan_expectation([circuit, other] * 2, names, vals[:4], [[ham]] * 4)
-    with pytest.raises(InvalidArgumentError, match="cannot be rebound"):
-        mixed.rebind(vals[:4])
-    # Same program but wider than one workgroup's LDS: planned per row, not rebindable.
+    stats = mixed.stats()
+    mixed.rebind(vals[:4] + 1.0)
+    assert mixed.stats() == stats
     wide, wnames, wq = W.hea_ring(16, 1)
-    big = _plan_expectation([wide] * 2, wnames, rng.uniform(-3, 3, (2, len(wnames))),
-                            [[W.tfim(wq, 1.0)]] * 2)
+    tiled = _plan_expectation([wide] * 2, wnames, rng.uniform(-3, 3, (2, len(wnames))),
+                              [[W.tfim(wq, 1.0)]] * 2)
+    stats = tiled.stats()
+    tiled.rebind(np.zeros((2, len(wnames))))
+    assert tiled.stats() == stats
+    huge, hnames, hq = W.hea_ring(19, 1)
+    big = _plan_expectation([huge], hnames, rng.uniform(-3, 3, (1, len(hnames))),
+                            [[W.tfim(hq, 1.0)]])
     with pytest.raises(InvalidArgumentError, match="cannot be rebound"):
-        big.rebind(np.zeros((2, len(wnames))))
+        big.rebind(np.zeros((1, len(hnames))))
 
 
 @pytest.mark.parametrize("width", [1, 2, 3, 4])

@@ -618,6 +618,71 @@ def op_call_rate(progs, names, values, ps, weights, calls=5):
                     "change every symbol value and downstream gradient"}
 
 
+def plain_best_measure(progs, names, values, ps, weights, stream, stride=4, chunk=256):
+    """Unbatched comparator: circuits of the uneven batch each in a plan of
+    their own (what a caller passing one circuit gets), launched alone on the
+    faster of their single-circuit paths -- one workgroup of the batched kernel
+    (LDS-resident to 13-14 qubits, tiled through scratch above), or, for 14-16
+    qubits, the HBM-streaming passes, each spread over 2^(n-13) workgroups.
+    Device time per launch from HIP events around it; host planning is not
+    counted.  Every ``stride``-th circuit is timed (widths and depths are drawn
+    independently per circuit, so the sample has the batch's mix) and the sum
+    is scaled to the whole batch."""
+    import torch
+    from quantum_amd import _lib
+    from quantum_amd import tfq_adj_grad_op as adj
+    dev = torch.device("cuda", 0)
+    s = torch.cuda.current_stream()
+    B = len(progs)
+    widths = {}
+
+    def timed(rows):
+        res = {}
+        for lo in range(0, len(rows), chunk):
+            pend = []
+            for i in rows[lo:lo + chunk]:
+                plan, shp = adj.plan_adj_grad(progs[i:i + 1], names, values[i:i + 1],
+                                              ps[i:i + 1], weights[i:i + 1],
+                                              with_expectation=True)
+                widths[i] = plan.stats()["max_qubits"]
+                out = torch.empty(shp, dtype=torch.float32, device=dev)
+                plan.upload(dev, stream)
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record(s)
+                plan.launch(out, stream)
+                b.record(s)
+                pend.append((i, a, b, plan, out))
+            torch.cuda.synchronize()
+            for i, a, b, _, _ in pend:
+                res[i] = a.elapsed_time(b)
+        return res
+
+    rows = list(range(0, B, stride))
+    timed(rows[:32])  # load and warm the kernels of the batched path
+    one_wg = timed(rows)
+    wide_rows = [i for i in rows if widths[i] >= 14]
+    _lib.set_streaming_min_qubits(14)
+    try:
+        timed(wide_rows[:8])
+        wide = timed(wide_rows)
+    finally:
+        _lib.set_streaming_min_qubits(-1)
+    best = {i: min(t, wide.get(i, t)) for i, t in one_wg.items()}
+    chose_stream = sum(1 for i in wide if wide[i] < one_wg[i])
+    scale = B / len(rows)
+    total_ms = sum(best.values()) * scale
+    one_ms = sum(one_wg.values()) * scale
+    log(f"plain (best single-circuit path, {len(rows)} of {B} circuits timed): {total_ms:.1f} ms "
+        f"({B / total_ms * 1e3:.0f} circuits/s); one-workgroup plans alone {one_ms:.1f} ms; "
+        f"streaming faster for {chose_stream} of {len(wide)} circuits of 14-16 qubits")
+    return {"ms": round(total_ms, 2), "circuits_per_s": round(B / total_ms * 1e3, 1),
+            "one_workgroup_plans_ms": round(one_ms, 2), "circuits_timed": len(rows),
+            "wide_circuits_timed": len(wide), "wide_streaming_faster": chose_stream,
+            "note": f"every {stride}th circuit in its own plan and launch on its faster "
+                    f"single-circuit path (batched kernel, or streaming passes for 14-16 "
+                    f"qubits), summed device time scaled to the batch; host planning excluded"}
+
+
 def measured_traffic(circuits, flops):
     """HBM bytes per launch from the committed rocprofv3 counter summary, if it
     was taken on this same workload (same circuit count and flop count)."""
@@ -671,7 +736,8 @@ def main():
                     help="skip the autotuner (the --only profiling runs never tune: they use "
                          "the planner defaults)")
     ap.add_argument("--only", choices=["all", "batch", "hbm", "copy", "ipgrad", "noisy", "ops",
-                                       "c2", "c3", "c5", "vqe", "opcalls"],
+                                       "c2", "c3", "c5", "vqe", "opcalls",
+                                       "plain"],
                     default="all",
                     help="profiling runs: 'batch' launches only the batched kernel (graph "
                          "replays, no plain launches, no CPU baseline); 'hbm' only the 30-qubit "
@@ -679,7 +745,8 @@ def main():
                          "'ipgrad' the inner-product gradient batch; 'c2' the 30-qubit samples "
                          "and sampled expectation; 'c3' the 26-qubit adjoint; 'c5' the "
                          "surrogate network's kernels; 'vqe' the surrogate-assisted VQE; "
-                         "'opcalls' the end-to-end op call rate on the uneven batch")
+                         "'opcalls' the end-to-end op call rate on the uneven batch; 'plain' the "
+                         "per-circuit comparator")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines of the C2/C3 "
                     "legs")
     args = ap.parse_args()
@@ -739,6 +806,10 @@ def main():
         print(json.dumps({"op_calls": op_call_rate(progs, names, values, ps, weights)}),
               flush=True)
         return
+    if args.only == "plain":
+        print(json.dumps({"plain_best": plain_best_measure(progs, names, values, ps, weights,
+                                                           stream)}), flush=True)
+        return
     tuned = None
     if args.only == "all" and not args.no_autotune:
         from quantum_amd import autotune as AT
@@ -779,6 +850,7 @@ def main():
     plain_diff = float((out_plain - out).abs().max())
     log(f"plain (one launch per circuit): {t_plain * 1e3:.1f} ms ({plain_value:.0f} circuits/s),"
         f" max |plain - batched| = {plain_diff:.2e}")
+    plain_best = plain_best_measure(progs, names, values, ps, weights, stream)
 
     per_launch = elapsed / args.steps
     tflops = flops / per_launch / 1e12
@@ -841,6 +913,10 @@ def main():
         "autotune": tuned,
         "plain_gpu_circuits_per_s": round(plain_value, 1),
         "batched_vs_plain": round(value / plain_value, 2),
+        "plain_note": "plain = the same plan launched one circuit at a time, one workgroup "
+                      "each; plain_best = each circuit alone on its faster single-circuit path",
+        "plain_best": plain_best,
+        "batched_vs_plain_best": round(value / plain_best["circuits_per_s"], 2),
         "max_abs_diff_plain_vs_batched": plain_diff,
         "max_abs_err_vs_oracle": err,
         "oracle_rows_checked": checked,

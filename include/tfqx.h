@@ -354,7 +354,7 @@ int tfqx_gemm_bf16(int32_t M, int32_t N, int32_t K, const void* A, int32_t lda, 
 /* Planner tuning for later plans: register engine off (0), on for the
  * batched kernel's tiles of 12-13 qubits (1), or on for those and the
  * streaming path's passes (2, the default); and the streaming path's tile
- * qubits for one-state and two-state passes (0 = defaults 13 and 11; at most
+ * qubits for one-state and two-state passes (0 = default 13 for both; at most
  * 14 and 13).  No reference counterpart (qsim's CPU path has no tiles). */
 void tfqx_set_tuning(int32_t register_engine, int32_t tile_single, int32_t tile_two);
 
@@ -364,6 +364,15 @@ void tfqx_set_tuning(int32_t register_engine, int32_t tile_single, int32_t tile_
  * the default, 1024 (four wavefronts per CU); 0 always uses wavefronts.
  * No reference counterpart. */
 void tfqx_set_wave_min_circuits(int32_t count);
+
+/* Width from which expectation, adjoint-gradient and state plans run on the
+ * HBM-streaming path (every pass spread over 2^(n-13) workgroups) rather than
+ * inside the batched kernel (one workgroup per circuit, tiles of 14-18 qubits
+ * through per-workgroup scratch).  Values below 14 or above 19, and negative
+ * values, restore the default 19.  The analogue of the reference's switch to
+ * its state-parallel schedule (tfq_simulate_expectation_op.cc:120-126,
+ * tfq_adj_grad_op.cc:161-169), by width only. */
+void tfqx_set_streaming_min_qubits(int32_t num_qubits);
 
 /* Fusion width (qubits per fused block) the planner uses; 0 = default (2).
  * The batched kernel fuses at most 4 qubits.  On the HBM-streaming path, 5 or

@@ -106,6 +106,8 @@ def _declare(lib):
     lib.tfqx_set_tuning.restype = None
     lib.tfqx_set_wave_min_circuits.argtypes = [i32]
     lib.tfqx_set_wave_min_circuits.restype = None
+    lib.tfqx_set_streaming_min_qubits.argtypes = [i32]
+    lib.tfqx_set_streaming_min_qubits.restype = None
     lib.tfqx_set_fusion_width.argtypes = [i32]
     lib.tfqx_set_fusion_width.restype = None
     lib.tfqx_version.restype = c.c_char_p
@@ -150,7 +152,7 @@ EXPORTED_SYMBOLS = (
     "tfqx_plan_output_bytes", "tfqx_plan_upload", "tfqx_plan_launch", "tfqx_plan_launch_plain",
     "tfqx_plan_launch_noalloc",
     "tfqx_graph_capture", "tfqx_graph_launch", "tfqx_graph_destroy", "tfqx_plan_destroy",
-    "tfqx_set_fusion_width", "tfqx_set_wave_min_circuits", "tfqx_set_tuning", "tfqx_version", "tfqx_mlp_layout", "tfqx_mlp_set_params",
+    "tfqx_set_fusion_width", "tfqx_set_wave_min_circuits", "tfqx_set_streaming_min_qubits", "tfqx_set_tuning", "tfqx_version", "tfqx_mlp_layout", "tfqx_mlp_set_params",
     "tfqx_mlp_predict", "tfqx_mlp_loss_grad", "tfqx_mlp_train_step", "tfqx_gemm_bf16")
 
 
@@ -331,6 +333,14 @@ def set_wave_min_circuits(count=-1):
     """Smallest count of 4-10 qubit circuits run one per wavefront (negative:
     the default 1024; 0: always)."""
     lib().tfqx_set_wave_min_circuits(int(count))
+    _settings_changed()
+
+
+def set_streaming_min_qubits(num_qubits=-1):
+    """Width from which expectation, adjoint and state plans take the
+    HBM-streaming path instead of the batched kernel's tiles (14-19; anything
+    else restores the default 19)."""
+    lib().tfqx_set_streaming_min_qubits(int(num_qubits))
     _settings_changed()
 
 

@@ -262,7 +262,10 @@ void Layout(tfqx_plan* p) {
   if (p->mode == kModeInnerProductGrad)
     off = Align(off + 2 * size_t(p->B) * size_t(p->S) * sizeof(double));
   p->off_scratch = off;
-  if (g.any_tiled) off = Align(off + size_t(kTiledGridCap) * 2 * (size_t(1) << p->max_nq) * sizeof(float2));
+  // One psi + lambda scratch pair per launched workgroup; the grid never
+  // exceeds the number of circuits (tfqx_plan_launch, tfqx_graph_capture).
+  const size_t tiled_wgs = std::min<size_t>(kTiledGridCap, std::max<size_t>(1, g.circuits.size()));
+  if (g.any_tiled) off = Align(off + tiled_wgs * 2 * (size_t(1) << p->max_nq) * sizeof(float2));
   p->total = off;
   p->staging.assign(image, 0);
   auto put = [&](size_t o, const void* src, size_t n) {
@@ -279,6 +282,7 @@ void Layout(tfqx_plan* p) {
 }
 
 std::atomic<int> g_wave_min{kWaveMinCircuits};
+std::atomic<int> g_stream_min{kTiledMaxQubits + 1};
 
 int BuildPlan(int mode, Inputs& in, int64_t B, int64_t S, int64_t O, int64_t K,
               const float* downstream, bool tfq_fd, tfqx_plan** out, char* err, int64_t len,
@@ -299,7 +303,7 @@ int BuildPlan(int mode, Inputs& in, int64_t B, int64_t S, int64_t O, int64_t K,
   opt.fuse.max_fuse = AutoFusionWidth(p->max_nq);
   const int resident = LdsLimit(mode);
   const bool tiled_ok = (mode == kModeExpectation || mode == kModeAdjoint || mode == kModeState) &&
-                        p->max_nq <= kTiledMaxQubits;
+                        p->max_nq <= kTiledMaxQubits && p->max_nq < g_stream_min.load();
   opt.tile_log2 = resident;
   // Wave mode gives each small circuit one wavefront.  It pays when there
   // are enough such circuits to occupy the chip (256 CUs x 4 SIMDs); fewer
@@ -507,6 +511,11 @@ extern "C" {
 const char* tfqx_version(void) { return "tfqx 0.1 (gfx950)"; }
 
 void tfqx_set_fusion_width(int32_t width) { g_fusion_width.store(width); }
+
+void tfqx_set_streaming_min_qubits(int32_t num_qubits) {
+  const bool valid = num_qubits >= kLdsMaxQubitsOneBuffer && num_qubits <= kTiledMaxQubits + 1;
+  g_stream_min.store(valid ? int(num_qubits) : kTiledMaxQubits + 1);
+}
 
 void tfqx_set_wave_min_circuits(int32_t count) {
   g_wave_min.store(count < 0 ? kWaveMinCircuits : int(count));
@@ -766,17 +775,18 @@ int tfqx_debug_philox_uniform(uint64_t seed, uint64_t start, int64_t count, uint
 
 int tfqx_plan_rebind(tfqx_plan* p, const float* values, char* err, int64_t len) {
   if (!p || (!values && p->S > 0)) return Fail(err, len, TFQX_INVALID_ARGUMENT, "null argument");
+  // The matrices are written straight into the upload image (prog.mats keeps
+  // the values the plan was built with; only Layout reads it).
+  cfloat* mats = reinterpret_cast<cfloat*>(p->staging.data() + p->off_mats);
   if (p->shared) {
     RebindSharedBatchProgram(p->tmpl, p->recipe, values, int(p->S), p->tfq_fd, HostThreads(),
-                             &p->prog);
+                             p->prog, mats);
   } else if (p->rebindable) {
-    RebindBatchProgram(p->rows_rebind, values, int(p->S), p->tfq_fd, HostThreads(), &p->prog);
+    RebindBatchProgram(p->rows_rebind, values, int(p->S), p->tfq_fd, HostThreads(), p->prog, mats);
   } else {
     return Fail(err, len, TFQX_INVALID_ARGUMENT,
                 "Plan cannot be rebound: it runs on the HBM-streaming path or is a noisy plan.");
   }
-  if (!p->prog.mats.empty())
-    memcpy(p->staging.data() + p->off_mats, p->prog.mats.data(), p->prog.mats.size() * sizeof(cfloat));
   return TFQX_OK;
 }
 

@@ -824,8 +824,8 @@ RowRebind CompactRecipe(const Circuit& c, const SharedRecipe& rec) {
 }
 
 void RebindBatchProgram(const std::vector<RowRebind>& rows, const float* values, int num_symbols,
-                        bool tfq_fd, int threads, BatchProgram* prog) {
-  const int B = prog->num_rows;
+                        bool tfq_fd, int threads, const BatchProgram& prog, cfloat* mats) {
+  const int B = prog.num_rows;
   RunParallel(B, threads, [&](int row) {
     const RowRebind& rr = rows[size_t(row)];
     if (rr.recipe.ops.empty()) return;
@@ -849,7 +849,7 @@ void RebindBatchProgram(const std::vector<RowRebind>& rows, const float* values,
       for (int j = 0; j < g.nsym; ++j) g.params[g.sym_pidx[j]] = v[g.sym_col[j]];
       GateUnitary(g.kind, g.params, g.U);
     }
-    RebindRow(gs, rr.recipe, tfq_fd, prog->mats.data() + prog->row_mat_base[size_t(row)]);
+    RebindRow(gs, rr.recipe, tfq_fd, mats + prog.row_mat_base[size_t(row)]);
   });
 }
 
@@ -884,12 +884,13 @@ void BuildSharedBatchProgram(const Circuit& tmpl, const std::vector<CircuitJob>&
 }
 
 void RebindSharedBatchProgram(const Circuit& tmpl, const SharedRecipe& recipe, const float* values,
-                              int num_symbols, bool tfq_fd, int threads, BatchProgram* prog) {
-  const int B = prog->num_rows;
+                              int num_symbols, bool tfq_fd, int threads, const BatchProgram& prog,
+                              cfloat* mats) {
+  const int B = prog.num_rows;
   RunParallel(B, threads, [&](int row) {
     std::vector<Gate> gs;
     RowGates(tmpl, values + size_t(row) * num_symbols, &gs);
-    RebindRow(gs, recipe, tfq_fd, prog->mats.data() + prog->row_mat_base[size_t(row)]);
+    RebindRow(gs, recipe, tfq_fd, mats + prog.row_mat_base[size_t(row)]);
   });
 }
 

@@ -164,9 +164,11 @@ struct RowRebind {
 };
 // Keep the symbolic ops of `rec` (built from circuit `c`) with their gates.
 RowRebind CompactRecipe(const Circuit& c, const SharedRecipe& rec);
-// Re-evaluate every row's symbolic matrices for new values [rows, num_symbols].
+// Re-evaluate every row's symbolic matrices for new values [rows, num_symbols],
+// writing them into `mats` (an array laid out as prog.mats, e.g. the upload
+// image's copy of it).
 void RebindBatchProgram(const std::vector<RowRebind>& rows, const float* values, int num_symbols,
-                        bool tfq_fd, int threads, BatchProgram* prog);
+                        bool tfq_fd, int threads, const BatchProgram& prog, cfloat* mats);
 
 // Rows whose programs are byte-identical: `tmpl` is the program resolved with
 // row 0's values; the op list is built once and each row re-evaluates only the
@@ -176,9 +178,11 @@ void BuildSharedBatchProgram(const Circuit& tmpl, const std::vector<CircuitJob>&
                              const float* values, int num_symbols, int num_slots,
                              const PlanOptions& opt, int threads, BatchProgram* out,
                              SharedRecipe* recipe);
-// Re-evaluate every row's matrices of a shared program for new values.
+// Re-evaluate every row's matrices of a shared program for new values, into
+// `mats` (laid out as prog.mats).
 void RebindSharedBatchProgram(const Circuit& tmpl, const SharedRecipe& recipe, const float* values,
-                              int num_symbols, bool tfq_fd, int threads, BatchProgram* prog);
+                              int num_symbols, bool tfq_fd, int threads, const BatchProgram& prog,
+                              cfloat* mats);
 
 // A fused block as a device op (matrices appended to `mats`), and a Pauli term
 // in XOR-mask form with weight w * i^{#Y}.

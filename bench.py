@@ -747,6 +747,9 @@ def main():
                          "surrogate network's kernels; 'vqe' the surrogate-assisted VQE; "
                          "'opcalls' the end-to-end op call rate on the uneven batch; 'plain' the "
                          "per-circuit comparator")
+    ap.add_argument("--fusion-width", type=int, default=None,
+                    help="planner fusion width for the run (0 = default; 5 or 6 puts dense "
+                         "MFMA passes on the streaming path, e.g. with --only hbm)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines of the C2/C3 "
                     "legs")
     args = ap.parse_args()
@@ -764,6 +767,9 @@ def main():
     torch.cuda.set_stream(side)
     stream = side.cuda_stream
 
+    if args.fusion_width is not None:
+        from quantum_amd import _lib
+        _lib.set_fusion_width(args.fusion_width)
     if args.only == "copy":
         src = torch.ones(1 << 27, dtype=torch.float64, device=dev)
         dst = torch.empty_like(src)

@@ -217,3 +217,24 @@ def test_network_gradient_beats_the_stale_exact_gradient(trained):
     print(f"[surrogate] mean |g_anchored - g| = {err_net:.4f} vs stale {err_stale:.4f}",
           file=sys.stderr)
     assert err_net < 0.95 * err_stale
+
+
+@pytest.mark.parametrize("loop", ["surrogate", "candidate", "ablation"])
+def test_surrogate_assisted_vqe_loops(loop):
+    """The C5 loops run end to end (150 steps from theta0 ~ U(-0.1, 0.1)):
+    the energy they report is the exact energy at the parameters they return
+    (every accepted point is checked by the simulator), it lies below the
+    start energy, and the exact simulator was called."""
+    from quantum_amd import surrogate as S
+    exact, names = _c5()
+    theta0 = np.random.default_rng(1005).uniform(-0.1, 0.1, len(names))
+    e0, _ = exact(theta0.astype(np.float32))
+    model = None if loop == "ablation" else S.MLPSurrogate(len(names), 512, max_batch=1024, seed=0)
+    vqe = S.CandidateVQE(model, exact) if loop == "candidate" else S.SurrogateVQE(model, exact)
+    _say(f"C5 {loop} VQE, 150 steps")
+    r = vqe.run(theta0, steps=150)
+    assert np.all(np.isfinite(r["theta"]))
+    e_check, _ = exact(np.asarray(r["theta"], dtype=np.float32))
+    assert abs(r["energy"] - e_check) < 1e-4 * max(1.0, abs(e_check))
+    assert r["energy"] < e0 - 1.0
+    assert r["exact_calls"] > 0

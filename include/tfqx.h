@@ -283,7 +283,12 @@ int tfqx_ps_weights_from_symbols(const uint8_t* programs, const int64_t* program
  * [3] max qubits, [4] adjoint ops, [5] adjoint HBM passes, [6] 1 if the
  * HBM-streaming path is used, [7] workspace bytes, [8] algorithmic real flops
  * of one launch (batch path; 0 on the streaming path), [9] the part of those
- * flops that runs on FP32 MFMA (both paths).  Noisy plans: [0] non-empty
+ * flops that runs on FP32 MFMA (both paths).  Batch-kernel plans, n > 10:
+ * the op mix in amplitude-op units (2^n per op of an n-qubit circuit, /1024):
+ * [10] forward ops, [11] reverse remaps, [12] reverse diagonal ops, [13]
+ * reverse dense one-qubit ops, [14] reverse dense wider ops, [15] reverse
+ * parameterised one-qubit ops, [16] reverse parameterised two-qubit ops,
+ * [17] reverse parameterised ops with a diagonal matrix.  Noisy plans: [0] non-empty
  * circuits, [1] ops (gates and channels), [2] trajectories per launch, [8]
  * expected algorithmic real flops of one launch.  No reference counterpart. */
 int tfqx_plan_stats(const tfqx_plan* plan, int64_t* out, int64_t n);
@@ -373,6 +378,17 @@ void tfqx_set_wave_min_circuits(int32_t count);
  * its state-parallel schedule (tfq_simulate_expectation_op.cc:120-126,
  * tfq_adj_grad_op.cc:161-169), by width only. */
 void tfqx_set_streaming_min_qubits(int32_t num_qubits);
+
+/* Adjoint plans: fuse the reverse sweep across parameterised gates (1, the
+ * default) or split it at every parameterised gate (0).  With fusion on, a
+ * parameterised gate without controls joins its neighbours in a block of at
+ * most two qubits and two derivatives; the block's derivative matrices are
+ * its product with one gate replaced by that gate's derivative, so the
+ * gradient 2 Re<lambda|dB|psi> of each symbol is unchanged while the sweep
+ * applies about half as many ops.  Off reproduces the reference's gradient
+ * circuit (core/src/adj_util.cc:155-172: fused segments between
+ * parameterised gates). */
+void tfqx_set_reverse_fusion(int32_t on);
 
 /* Fusion width (qubits per fused block) the planner uses; 0 = default (2).
  * The batched kernel fuses at most 4 qubits.  On the HBM-streaming path, 5 or

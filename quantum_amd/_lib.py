@@ -108,6 +108,8 @@ def _declare(lib):
     lib.tfqx_set_wave_min_circuits.restype = None
     lib.tfqx_set_streaming_min_qubits.argtypes = [i32]
     lib.tfqx_set_streaming_min_qubits.restype = None
+    lib.tfqx_set_reverse_fusion.argtypes = [i32]
+    lib.tfqx_set_reverse_fusion.restype = None
     lib.tfqx_set_fusion_width.argtypes = [i32]
     lib.tfqx_set_fusion_width.restype = None
     lib.tfqx_version.restype = c.c_char_p
@@ -152,7 +154,7 @@ EXPORTED_SYMBOLS = (
     "tfqx_plan_output_bytes", "tfqx_plan_upload", "tfqx_plan_launch", "tfqx_plan_launch_plain",
     "tfqx_plan_launch_noalloc",
     "tfqx_graph_capture", "tfqx_graph_launch", "tfqx_graph_destroy", "tfqx_plan_destroy",
-    "tfqx_set_fusion_width", "tfqx_set_wave_min_circuits", "tfqx_set_streaming_min_qubits", "tfqx_set_tuning", "tfqx_version", "tfqx_mlp_layout", "tfqx_mlp_set_params",
+    "tfqx_set_fusion_width", "tfqx_set_wave_min_circuits", "tfqx_set_streaming_min_qubits", "tfqx_set_reverse_fusion", "tfqx_set_tuning", "tfqx_version", "tfqx_mlp_layout", "tfqx_mlp_set_params",
     "tfqx_mlp_predict", "tfqx_mlp_loss_grad", "tfqx_mlp_train_step", "tfqx_gemm_bf16")
 
 
@@ -209,6 +211,15 @@ class Plan:
         keys = ("circuits", "forward_ops", "forward_passes", "max_qubits", "adjoint_ops",
                 "adjoint_passes", "streaming", "workspace_bytes", "flops", "mfma_flops")
         return dict(zip(keys, [int(x) for x in out]))
+
+    def op_mix(self):
+        """Op mix of a batch-kernel plan in amplitude-op units (2^n per op of
+        an n-qubit circuit, / 1024); zeros for streaming and noisy plans."""
+        out = (ctypes.c_int64 * 18)()
+        self._lib.tfqx_plan_stats(self.handle, out, 18)
+        keys = ("forward", "rev_remap", "rev_diag", "rev_dense1", "rev_dense2", "rev_param1",
+                "rev_param2", "rev_param_diag")
+        return dict(zip(keys, [int(x) for x in out[10:]]))
 
     def rebind(self, values):
         """New symbol values (float32 [batch, n_symbols]) for a batch-kernel plan
@@ -341,6 +352,14 @@ def set_streaming_min_qubits(num_qubits=-1):
     HBM-streaming path instead of the batched kernel's tiles (14-19; anything
     else restores the default 19)."""
     lib().tfqx_set_streaming_min_qubits(int(num_qubits))
+    _settings_changed()
+
+
+def set_reverse_fusion(on=True):
+    """Adjoint plans fuse the reverse sweep across parameterised gates (on,
+    the default) or split it at each of them (off, the reference's gradient
+    circuit)."""
+    lib().tfqx_set_reverse_fusion(1 if on else 0)
     _settings_changed()
 
 

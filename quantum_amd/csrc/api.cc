@@ -423,6 +423,39 @@ double MfmaFlopsOfRange(const std::vector<DevOp>& ops, int b, int e, double A, b
   return f;
 }
 
+// Op mix of a batch program in amplitude-op units (2^n per op of an n-qubit
+// circuit, / 1024): [0] forward ops, [1] reverse remaps, [2] reverse diagonal
+// ops, [3] reverse dense one-qubit ops, [4] reverse dense two-qubit (or
+// wider) ops, [5] reverse parameterised one-qubit ops, [6] reverse
+// parameterised two-qubit ops, [7] reverse parameterised ops whose matrix is
+// diagonal.
+void OpMix(const BatchProgram& g, double (&m)[8]) {
+  for (double& x : m) x = 0;
+  auto diag_matrix = [&](const DevOp& o) {
+    const int D = 1 << o.k;
+    for (int r = 0; r < D; ++r)
+      for (int c = 0; c < D; ++c)
+        if (r != c && g.mats[size_t(o.mat) + size_t(r * D + c)] != cfloat(0.f, 0.f)) return false;
+    return true;
+  };
+  for (const auto& c : g.circuits) {
+    if (c.flags & kFlagEmpty) continue;
+    const double A = std::ldexp(1.0, c.n) / 1024.0;
+    for (int i = c.fwd_begin; i < c.fwd_end; ++i) m[0] += A * (g.ops[size_t(i)].type != kOpRemap);
+    for (int i = c.rev_begin; i < c.rev_end; ++i) {
+      const DevOp& o = g.ops[size_t(i)];
+      if (o.type == kOpRemap) m[1] += A;
+      else if (o.type == kOpDiag) m[2] += A;
+      else if (o.type == kOpParam) {
+        m[o.k == 1 ? 5 : 6] += A;
+        if (diag_matrix(o)) m[7] += A;
+      } else {
+        m[o.k == 1 ? 3 : 4] += A;
+      }
+    }
+  }
+}
+
 double MfmaFlops(const BatchProgram& g) {
   double f = 0;
   for (const auto& c : g.circuits) {
@@ -511,6 +544,8 @@ extern "C" {
 const char* tfqx_version(void) { return "tfqx 0.1 (gfx950)"; }
 
 void tfqx_set_fusion_width(int32_t width) { g_fusion_width.store(width); }
+
+void tfqx_set_reverse_fusion(int32_t on) { SetReverseFusion(on != 0); }
 
 void tfqx_set_streaming_min_qubits(int32_t num_qubits) {
   const bool valid = num_qubits >= kLdsMaxQubitsOneBuffer && num_qubits <= kTiledMaxQubits + 1;
@@ -823,7 +858,12 @@ int tfqx_plan_rebind_downstream(tfqx_plan* p, const float* downstream, char* err
 
 int tfqx_plan_stats(const tfqx_plan* p, int64_t* out, int64_t n) {
   if (!p || !out) return TFQX_INVALID_ARGUMENT;
-  int64_t v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  int64_t v[18] = {0};
+  if (!p->large && !p->noisy && n > 10) {
+    double mix[8];
+    OpMix(p->prog, mix);
+    for (int i = 0; i < 8; ++i) v[10 + i] = int64_t(mix[i]);
+  }
   v[3] = p->max_nq;
   if (p->large) {
     const LargePlan& lp = *p->large_plan;
@@ -891,7 +931,7 @@ int tfqx_plan_stats(const tfqx_plan* p, int64_t* out, int64_t n) {
     v[8] = int64_t(total);
   }
   v[7] = int64_t(p->total);
-  for (int64_t i = 0; i < n && i < 10; ++i) out[i] = v[i];
+  for (int64_t i = 0; i < n && i < 18; ++i) out[i] = v[i];
   return TFQX_OK;
 }
 

@@ -352,34 +352,9 @@ Status BuildLargePlan(int mode, const std::vector<Circuit>& circuits,
       lc.lam_end = int(lp->terms.size());
     }
     if (grad) {
-      // Reverse program (as the LDS planner), param ops numbered by partial slot.
-      std::vector<int> pg;
-      for (int g = 0; g < int(c.gates.size()); ++g)
-        if (c.gates[g].nsym > 0) pg.push_back(g);
+      // Reverse program (as the batch planner), param ops numbered by partial slot.
       std::vector<FusedOp> rev;
-      int seg_end = int(c.gates.size());
-      for (int p = int(pg.size()); p >= 0; --p) {
-        const int seg_begin = p == 0 ? 0 : pg[p - 1] + 1;
-        std::vector<FusedOp> seg;
-        FuseGates(c, seg_begin, seg_end, fo, &seg);
-        for (int s = int(seg.size()) - 1; s >= 0; --s) rev.push_back(seg[s]);
-        if (p == 0) break;
-        const Gate& g = c.gates[pg[p - 1]];
-        FusedOp f = GateToOp(g, true);
-        f.param = true;
-        f.nderiv = g.nsym;
-        for (int d = 0; d < g.nsym; ++d) {
-          cdouble dU[16];
-          GateDerivative(g.kind, g.params, g.sym_pidx[d], tfq_fd, dU);
-          // Embed the derivative with the same layout as the unitary.
-          Gate dg = g;
-          for (int e = 0; e < 16; ++e) dg.U[e] = dU[e];
-          f.dM[d] = GateToOp(dg, true).M;
-          f.col[d] = g.sym_col[d];  // symbol column; replaced by a partial slot below
-        }
-        rev.push_back(f);
-        seg_end = pg[p - 1];
-      }
+      BuildReverseOps(c, fo, tfq_fd, &rev);
       const size_t op0 = lp->ops.size();
       AddPasses(lp.get(), rev, lc.n, lc.L, &lc.rev);
       // Number gradient partial slots in pass order so each pass owns a

@@ -378,9 +378,175 @@ void FuseGates(const Circuit& c, int begin, int end, const FuseOptions& opt,
   for (size_t bi = 0; bi < blocks.size(); ++bi) close(int(bi));
 }
 
+namespace {
+
+std::atomic<int> g_rev_fuse(1);
+
+// A parameterised gate as an op of its own, controls outside the matrix.
+FusedOp ParamGateOp(const Circuit& c, int gi, const FuseOptions& opt, bool tfq_fd) {
+  const Gate& g = c.gates[size_t(gi)];
+  FusedOp f = GateToOp(g, true, opt.structural_diag);
+  f.gates.assign(1, gi);
+  f.param = true;
+  f.nderiv = g.nsym;
+  const int dim = 1 << f.k;
+  for (int d = 0; d < g.nsym; ++d) {
+    cdouble dU[16];
+    GateDerivative(g.kind, g.params, g.sym_pidx[d], tfq_fd, dU);
+    f.dM[d].assign(size_t(dim) * dim, 0.0);
+    Embed(dU, g.bits, g.nt, 0, 0, f.bits, f.k, f.dM[d].data());
+    f.col[d] = g.sym_col[d];
+    f.dgate[d] = 0;
+    f.dsym[d] = d;
+  }
+  return f;
+}
+
+// Derivative matrices of a fused block that holds parameterised gates (none
+// of them controlled): for each symbol of each such gate, the block's product
+// with that gate's matrix replaced by its derivative.
+void MakeParamBlock(const Circuit& c, bool tfq_fd, FusedOp* f) {
+  const int dim = 1 << f->k;
+  std::vector<Gate> loc;
+  loc.reserve(f->gates.size());
+  for (int gi : f->gates) loc.push_back(c.gates[size_t(gi)]);
+  std::vector<int> idx(loc.size());
+  std::iota(idx.begin(), idx.end(), 0);
+  f->param = true;
+  f->fused_param = true;
+  f->diag = false;
+  f->nderiv = 0;
+  for (size_t j = 0; j < loc.size(); ++j) {
+    const Gate& g = c.gates[size_t(f->gates[j])];
+    for (int d = 0; d < g.nsym; ++d) {
+      const int s = f->nderiv++;
+      GateDerivative(g.kind, g.params, g.sym_pidx[d], tfq_fd, loc[j].U);
+      f->dM[s].assign(size_t(dim) * dim, 0.0);
+      BlockMatrix(loc, idx.data(), int(idx.size()), f->bits, f->k, f->dM[s].data());
+      for (int e = 0; e < 16; ++e) loc[j].U[e] = g.U[e];
+      f->col[s] = g.sym_col[d];
+      f->dgate[s] = int(j);
+      f->dsym[s] = d;
+    }
+  }
+}
+
+// The frontier fuser of FuseGates with parameterised gates inside blocks:
+// blocks holding them stay within two qubits and kMaxDeriv derivatives.
+// Blocks come out in a valid application order; the caller reverses them.
+void FuseWithParams(const Circuit& c, const FuseOptions& opt, bool tfq_fd,
+                    std::vector<FusedOp>* out) {
+  struct Block {
+    std::vector<int> gates;
+    uint64_t qmask;
+    int nderiv;
+    bool open;
+  };
+  const int pmax = std::min(opt.max_fuse, 2);
+  std::vector<Block> blocks;
+  std::vector<int> owner(64, -1);
+  auto close = [&](int bi) {
+    Block& b = blocks[size_t(bi)];
+    if (!b.open) return;
+    b.open = false;
+    for (int q = 0; q < 64; ++q)
+      if (b.qmask >> q & 1) owner[size_t(q)] = -1;
+    out->push_back(BlockToOp(c, b.gates, b.qmask, opt.structural_diag));
+    if (b.nderiv > 0) MakeParamBlock(c, tfq_fd, &out->back());
+  };
+  auto claim = [&](int bi) {
+    for (int q = 0; q < 64; ++q)
+      if (blocks[size_t(bi)].qmask >> q & 1) owner[size_t(q)] = bi;
+  };
+  for (int gi = 0; gi < int(c.gates.size()); ++gi) {
+    const Gate& g = c.gates[size_t(gi)];
+    const uint64_t Q = GateMask(g);
+    std::vector<int> touching;
+    for (int q = 0; q < 64; ++q)
+      if ((Q >> q & 1) && owner[size_t(q)] >= 0 &&
+          std::find(touching.begin(), touching.end(), owner[size_t(q)]) == touching.end())
+        touching.push_back(owner[size_t(q)]);
+    std::sort(touching.begin(), touching.end());
+    const bool param = g.nsym > 0;
+    if ((param && (g.cmask != 0 || Popc(Q) > pmax)) || Popc(Q) > opt.max_fuse) {
+      for (int bi : touching) close(bi);
+      if (param) {
+        out->push_back(ParamGateOp(c, gi, opt, tfq_fd));
+      } else {
+        out->push_back(GateToOp(g, true, opt.structural_diag));
+        out->back().gates.assign(1, gi);
+      }
+      continue;
+    }
+    uint64_t U = Q;
+    int nd = g.nsym;
+    for (int bi : touching) {
+      U |= blocks[size_t(bi)].qmask;
+      nd += blocks[size_t(bi)].nderiv;
+    }
+    if (Popc(U) <= (nd > 0 ? pmax : opt.max_fuse) && nd <= kMaxDeriv) {
+      int keep;
+      if (touching.empty()) {
+        blocks.push_back(Block{{}, 0, 0, true});
+        keep = int(blocks.size()) - 1;
+      } else {
+        keep = touching[0];
+        for (size_t t = 1; t < touching.size(); ++t) {
+          Block& o = blocks[size_t(touching[t])];
+          Block& k = blocks[size_t(keep)];
+          k.gates.insert(k.gates.end(), o.gates.begin(), o.gates.end());
+          k.qmask |= o.qmask;
+          k.nderiv += o.nderiv;
+          o.open = false;
+          o.gates.clear();
+        }
+      }
+      Block& k = blocks[size_t(keep)];
+      k.gates.push_back(gi);
+      k.qmask |= Q;
+      k.nderiv += g.nsym;
+      claim(keep);
+    } else {
+      for (int bi : touching) close(bi);
+      blocks.push_back(Block{{gi}, Q, g.nsym, true});
+      claim(int(blocks.size()) - 1);
+    }
+  }
+  for (size_t bi = 0; bi < blocks.size(); ++bi) close(int(bi));
+}
+
+}  // namespace
+
+void SetReverseFusion(int on) { g_rev_fuse.store(on ? 1 : 0); }
+int GetReverseFusion() { return g_rev_fuse.load(); }
+
+void BuildReverseOps(const Circuit& c, const FuseOptions& opt, bool tfq_fd,
+                     std::vector<FusedOp>* rev) {
+  rev->clear();
+  if (g_rev_fuse.load()) {
+    std::vector<FusedOp> fwd;
+    FuseWithParams(c, opt, tfq_fd, &fwd);
+    for (size_t i = fwd.size(); i-- > 0;) rev->push_back(std::move(fwd[i]));
+    return;
+  }
+  std::vector<int> pg;
+  for (int i = 0; i < int(c.gates.size()); ++i)
+    if (c.gates[size_t(i)].nsym > 0) pg.push_back(i);
+  int seg_end = int(c.gates.size());
+  for (int p = int(pg.size()); p >= 0; --p) {
+    const int seg_begin = p == 0 ? 0 : pg[size_t(p) - 1] + 1;
+    std::vector<FusedOp> seg;
+    FuseGates(c, seg_begin, seg_end, opt, &seg);
+    for (int s = int(seg.size()) - 1; s >= 0; --s) rev->push_back(std::move(seg[size_t(s)]));
+    if (p == 0) break;
+    rev->push_back(ParamGateOp(c, pg[size_t(p) - 1], opt, tfq_fd));
+    seg_end = pg[size_t(p) - 1];
+  }
+}
+
 void AppendRecipe(const FusedOp& f, const Circuit& c, const DevOp& d, SharedRecipe* rec) {
   OpRecipe r;
-  r.kind = f.param ? 2 : (f.cmask ? 1 : 0);
+  r.kind = f.fused_param ? 3 : f.param ? 2 : (f.cmask ? 1 : 0);
   r.k = uint8_t(f.k);
   r.diag = !f.param && f.diag;
   for (int j = 0; j < f.k; ++j) r.bits[j] = f.bits[j];
@@ -392,7 +558,11 @@ void AppendRecipe(const FusedOp& f, const Circuit& c, const DevOp& d, SharedReci
   r.gate_end = int(rec->gates.size());
   r.mat = d.mat;
   r.nderiv = f.param ? f.nderiv : 0;
-  for (int i = 0; i < r.nderiv; ++i) r.dmat[i] = d.dmat[i];
+  for (int i = 0; i < r.nderiv; ++i) {
+    r.dmat[i] = d.dmat[i];
+    r.dgate[i] = int8_t(f.dgate[i]);
+    r.dsym[i] = int8_t(f.dsym[i]);
+  }
   rec->ops.push_back(r);
 }
 
@@ -563,38 +733,14 @@ void BuildPiece(const CircuitJob& job, int num_slots, const PlanOptions& opt, Pi
   const double amps = std::ldexp(1.0, std::max(dc.n, 0));
   for (const auto& f : fused) cost += amps * (f.diag ? 1 : (1 << f.k)) + 2048.0;
   if (opt.adjoint && !empty) {
-    // Reverse program: segments between parameterised gates, last first.
-    std::vector<int> pg;
-    for (int i = 0; i < int(c.gates.size()); ++i)
-      if (c.gates[i].nsym > 0) pg.push_back(i);
+    // Reverse program (BuildReverseOps), last op first.
     dc.rev_begin = int(pc->ops.size());
     dc.rev_pass_begin = int(pc->passes.size());
     std::vector<FusedOp> rev_ops;
-    int seg_end = int(c.gates.size());
-    for (int p = int(pg.size()); p >= 0; --p) {
-      const int seg_begin = p == 0 ? 0 : pg[p - 1] + 1;
-      std::vector<FusedOp> seg;
-      FuseGates(c, seg_begin, seg_end, opt.fuse, &seg);
-      for (int s = int(seg.size()) - 1; s >= 0; --s) {
-        rev_ops.push_back(seg[s]);
-        cost += 2 * amps * (seg[s].diag ? 1 : (1 << seg[s].k)) + 2048.0;
-      }
-      if (p == 0) break;
-      const Gate& g = c.gates[pg[p - 1]];
-      FusedOp f = GateToOp(g, true, opt.fuse.structural_diag);
-      f.gates.assign(1, pg[p - 1]);
-      f.param = true;
-      f.nderiv = g.nsym;
-      for (int d = 0; d < g.nsym; ++d) {
-        cdouble dU[16];
-        GateDerivative(g.kind, g.params, g.sym_pidx[d], opt.tfq_fd, dU);
-        f.dM[d].assign((1 << f.k) * (1 << f.k), 0.0);
-        Embed(dU, g.bits, g.nt, 0, 0, f.bits, f.k, f.dM[d].data());
-        f.col[d] = g.sym_col[d];
-      }
-      rev_ops.push_back(f);
-      cost += amps * (1 << f.k) * (3 + f.nderiv) + 4096.0;
-      seg_end = pg[p - 1];
+    BuildReverseOps(c, opt.fuse, opt.tfq_fd, &rev_ops);
+    for (const auto& f : rev_ops) {
+      if (f.param) cost += amps * (1 << f.k) * (3 + f.nderiv) + 4096.0;
+      else cost += 2 * amps * (f.diag ? 1 : (1 << f.k)) + 2048.0;
     }
     if (tiled) {
       SchedulePasses(rev_ops, dc.n, opt.tile_log2, &pc->passes, &pc->ops, &pc->mats, true, rc,
@@ -645,6 +791,8 @@ void RowGates(const Circuit& tmpl, const float* values, std::vector<Gate>* gates
 // Rewrite the matrices of symbolic ops for one row (others keep `base`'s).
 void RebindRow(const std::vector<Gate>& gs, const SharedRecipe& rec, bool tfq_fd, cfloat* mats) {
   cdouble M[256];
+  std::vector<Gate> loc;
+  std::vector<int> idx;
   for (const auto& r : rec.ops) {
     if (!r.symbolic) continue;
     const int dim = 1 << r.k;
@@ -653,6 +801,24 @@ void RebindRow(const std::vector<Gate>& gs, const SharedRecipe& rec, bool tfq_fd
     if (r.kind == 0) {
       BlockMatrix(gs, gl, ng, r.bits, r.k, M);
       WriteMatrix(mats + r.mat, M, r.diag, dim);
+      continue;
+    }
+    if (r.kind == 3) {
+      BlockMatrix(gs, gl, ng, r.bits, r.k, M);
+      WriteMatrix(mats + r.mat, M, false, dim);
+      loc.assign(size_t(ng), Gate{});
+      idx.resize(size_t(ng));
+      for (int j = 0; j < ng; ++j) {
+        loc[size_t(j)] = gs[size_t(gl[j])];
+        idx[size_t(j)] = j;
+      }
+      for (int d = 0; d < r.nderiv; ++d) {
+        Gate& g = loc[size_t(r.dgate[d])];
+        GateDerivative(g.kind, g.params, g.sym_pidx[r.dsym[d]], tfq_fd, g.U);
+        BlockMatrix(loc, idx.data(), ng, r.bits, r.k, M);
+        WriteMatrix(mats + r.dmat[d], M, false, dim);
+        g = gs[size_t(gl[r.dgate[d]])];
+      }
       continue;
     }
     const Gate& g = gs[gl[0]];

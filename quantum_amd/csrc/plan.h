@@ -32,6 +32,12 @@ struct FusedOp {
   int nderiv = 0;
   std::vector<cdouble> dM[kMaxDeriv];
   int col[kMaxDeriv];
+  // A parameterised block fused with its neighbours (controls inside the
+  // matrix): dM[d] is the block's product with gate gates[dgate[d]] replaced
+  // by its derivative for that gate's symbol dsym[d].
+  bool fused_param = false;
+  int dgate[kMaxDeriv] = {0, 0};
+  int dsym[kMaxDeriv] = {0, 1};
   // Gates (circuit indices, application order) the matrix is the product of.
   std::vector<int> gates;
 };
@@ -46,7 +52,9 @@ struct FuseOptions {
 // How one device op's matrices are rebuilt from its gates for another row
 // that shares the program (only symbol values differ).
 struct OpRecipe {
-  uint8_t kind = 0;      // 0 fused block, 1 single gate (controls outside), 2 parameterised
+  // 0 fused block, 1 single gate (controls outside), 2 parameterised gate
+  // (controls outside), 3 fused block containing parameterised gates
+  uint8_t kind = 0;
   uint8_t k = 0;
   bool diag = false;
   bool symbolic = false; // contains a symbolic gate (else the matrix is value-independent)
@@ -55,6 +63,8 @@ struct OpRecipe {
   uint32_t mat = 0;                  // offsets into the row's matrices
   uint32_t dmat[kMaxDeriv];
   int nderiv = 0;
+  int8_t dgate[kMaxDeriv] = {0, 0};  // kind 3: block position of each derivative's gate
+  int8_t dsym[kMaxDeriv] = {0, 1};   // ... and which of its symbols
 };
 
 struct SharedRecipe {
@@ -74,6 +84,22 @@ void AppendRecipe(const FusedOp& f, const Circuit& c, const DevOp& d, SharedReci
 // unfused with a control mask.
 void FuseGates(const Circuit& c, int begin, int end, const FuseOptions& opt,
                std::vector<FusedOp>* out);
+
+// The adjoint sweep's op list for circuit `c`, in reverse application order.
+// Reverse fusion on (the default): the whole circuit is fused as by
+// FuseGates, except that a block holding parameterised gates spans at most
+// two qubits and kMaxDeriv derivatives, and becomes a kOpParam op whose
+// derivative matrices are the block's product with one gate replaced by its
+// derivative (gradient = 2 Re <lambda_after| U_a dG U_b |psi_before>).
+// Parameterised gates with controls stay alone with the controls outside.
+// Reverse fusion off: fused segments between parameterised gates, each
+// parameterised gate an op of its own (the reference's gradient circuit,
+// core/src/adj_util.cc:155-172).
+void BuildReverseOps(const Circuit& c, const FuseOptions& opt, bool tfq_fd,
+                     std::vector<FusedOp>* rev);
+// Process-wide switch for later plans (1 on, 0 off).
+void SetReverseFusion(int on);
+int GetReverseFusion();
 
 // Register-engine segments: rewrite ops[begin, end) of an op list that runs on
 // a tile of L qubits so that it starts with a kOpRemap and every non-diagonal

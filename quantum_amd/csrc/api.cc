@@ -428,8 +428,9 @@ double MfmaFlopsOfRange(const std::vector<DevOp>& ops, int b, int e, double A, b
 // ops, [3] reverse dense one-qubit ops, [4] reverse dense two-qubit (or
 // wider) ops, [5] reverse parameterised one-qubit ops, [6] reverse
 // parameterised two-qubit ops, [7] reverse parameterised ops whose matrix is
-// diagonal.
-void OpMix(const BatchProgram& g, double (&m)[8]) {
+// diagonal, [8] forward remaps, [9] forward diagonal ops, [10] forward dense
+// one-qubit ops, [11] forward dense wider ops.
+void OpMix(const BatchProgram& g, double (&m)[12]) {
   for (double& x : m) x = 0;
   auto diag_matrix = [&](const DevOp& o) {
     const int D = 1 << o.k;
@@ -441,7 +442,13 @@ void OpMix(const BatchProgram& g, double (&m)[8]) {
   for (const auto& c : g.circuits) {
     if (c.flags & kFlagEmpty) continue;
     const double A = std::ldexp(1.0, c.n) / 1024.0;
-    for (int i = c.fwd_begin; i < c.fwd_end; ++i) m[0] += A * (g.ops[size_t(i)].type != kOpRemap);
+    for (int i = c.fwd_begin; i < c.fwd_end; ++i) {
+      const DevOp& o = g.ops[size_t(i)];
+      m[0] += A * (o.type != kOpRemap);
+      if (o.type == kOpRemap) m[8] += A;
+      else if (o.type == kOpDiag) m[9] += A;
+      else m[o.k == 1 ? 10 : 11] += A;
+    }
     for (int i = c.rev_begin; i < c.rev_end; ++i) {
       const DevOp& o = g.ops[size_t(i)];
       if (o.type == kOpRemap) m[1] += A;
@@ -858,11 +865,11 @@ int tfqx_plan_rebind_downstream(tfqx_plan* p, const float* downstream, char* err
 
 int tfqx_plan_stats(const tfqx_plan* p, int64_t* out, int64_t n) {
   if (!p || !out) return TFQX_INVALID_ARGUMENT;
-  int64_t v[18] = {0};
+  int64_t v[22] = {0};
   if (!p->large && !p->noisy && n > 10) {
-    double mix[8];
+    double mix[12];
     OpMix(p->prog, mix);
-    for (int i = 0; i < 8; ++i) v[10 + i] = int64_t(mix[i]);
+    for (int i = 0; i < 12; ++i) v[10 + i] = int64_t(mix[i]);
   }
   v[3] = p->max_nq;
   if (p->large) {
@@ -931,7 +938,7 @@ int tfqx_plan_stats(const tfqx_plan* p, int64_t* out, int64_t n) {
     v[8] = int64_t(total);
   }
   v[7] = int64_t(p->total);
-  for (int64_t i = 0; i < n && i < 18; ++i) out[i] = v[i];
+  for (int64_t i = 0; i < n && i < 22; ++i) out[i] = v[i];
   return TFQX_OK;
 }
 

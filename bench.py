@@ -529,13 +529,31 @@ def surrogate_measure(stream, train_batch=1024, steps=20):
            "roofline": {"bound": "mfma", "achieved": round(flops / t / 1e12, 3), "peak": 2500.0,
                         "unit": "TFLOP/s", "frac": round(flops / t / 1e12 / 2500.0, 5),
                         "flops_per_step": flops}}
-    for b in (1, 32):
+    import ctypes
+    err = ctypes.create_string_buffer(4096)
+    for b in (1, 32, 64):
         xb = X[:b].contiguous()
-        model.predict(xb)
-        tp = time_launch(lambda: model.predict(xb), stream, reps=steps)
-        res[f"predict_with_grad_ms_b{b}"] = round(tp * 1e3, 4)
-    log(f"C5 training step {t * 1e3:.3f} ms ({flops / t / 1e12:.1f} TFLOP/s); predict "
-        f"b1 {res['predict_with_grad_ms_b1']} ms, b32 {res['predict_with_grad_ms_b32']} ms")
+        Eb = torch.empty(b, dtype=torch.float32, device=dev)
+        Gb = torch.empty((b, P), dtype=torch.float32, device=dev)
+        args = (model._ws(), P, H, train_batch, ctypes.c_void_p(xb.data_ptr()), b,
+                ctypes.c_void_p(Eb.data_ptr()), ctypes.c_void_p(Gb.data_ptr()),
+                ctypes.c_void_p(stream), err, 4096)
+        model.lib.tfqx_mlp_predict(*args)
+        # Back-to-back C-ABI calls on preallocated buffers: the device time of
+        # one predict (B <= 32: the fused single-launch kernel; B = 64: layer
+        # by layer, 14 launches).
+        td = time_launch(lambda: model.lib.tfqx_mlp_predict(*args), stream, reps=200)
+        res[f"predict_with_grad_device_ms_b{b}"] = round(td * 1e3, 4)
+        if b <= 32:
+            model.predict(xb)
+            tp = time_launch(lambda: model.predict(xb), stream, reps=steps)
+            res[f"predict_with_grad_call_ms_b{b}"] = round(tp * 1e3, 4)
+    res["predict_note"] = ("device_ms: back-to-back C-ABI calls with preallocated outputs; "
+                           "call_ms: MLPSurrogate.predict from Python (allocates the outputs)")
+    log(f"C5 training step {t * 1e3:.3f} ms ({flops / t / 1e12:.1f} TFLOP/s); predict device "
+        f"b1 {res['predict_with_grad_device_ms_b1']} ms, b32 "
+        f"{res['predict_with_grad_device_ms_b32']} ms, b64 (layer by layer) "
+        f"{res['predict_with_grad_device_ms_b64']} ms")
     return res
 
 

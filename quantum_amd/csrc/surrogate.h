@@ -14,6 +14,9 @@ struct MlpLayout {
   size_t oW1, ob1, oW2, ob2, oW3, ob3, ow4, ob4, nparams;
   // Byte offsets into the workspace.
   size_t w, g, m, v, wb1, wb2, wb3, xb;
+  // Transposed BF16 weights ([K x H] row-major) for the fused small-batch
+  // predict's backward products.
+  size_t wt1, wt2, wt3;
   size_t z[3], a[3], h[3], hb[3], u[3], ub[3], vv[3], t1[3], t1b[3], t2[3], t2b[3], dag[3];
   size_t E, gin, dE, dg, dgb, loss;
   size_t bytes;

@@ -112,6 +112,169 @@ __global__ __launch_bounds__(kGThreads) void gemm_bf16_kernel(
 // ------------------------------------------------------------------ elementwise
 __device__ __forceinline__ float sigm(float a) { return 1.f / (1.f + __expf(-a)); }
 
+// BF16 copies of W1..W3 ([H x K] row-major), as they are and transposed.
+struct RefreshArgs {
+  const float* W[3];
+  __bf16* wb[3];
+  __bf16* wt[3];
+  int cols[3];
+  size_t end[3];  // cumulative element counts
+};
+__global__ void refresh_bf16_kernel(RefreshArgs r) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < r.end[2];
+       i += size_t(gridDim.x) * blockDim.x) {
+    const int l = i < r.end[0] ? 0 : i < r.end[1] ? 1 : 2;
+    const size_t j = i - (l == 0 ? 0 : r.end[l - 1]);
+    const size_t cols = size_t(r.cols[l]), row = j / cols, col = j % cols;
+    const size_t rows = (r.end[l] - (l == 0 ? 0 : r.end[l - 1])) / cols;
+    const __bf16 v = to_bf16(r.W[l][j]);
+    r.wb[l][j] = v;
+    r.wt[l][col * rows + row] = v;
+  }
+}
+
+// ------------------------------------------------------------------ fused small-batch predict
+// Energies and input gradients of up to 32 rows in one launch: one workgroup
+// of 16 wavefronts, wavefront w owning output columns 32w..32w+31 of every
+// product (a 32 x 32 tile on v_mfma_f32_32x32x16_bf16, rows = batch).
+// Activations pass between layers through LDS as BF16 (the rounding points of
+// the layer-by-layer path, whose products run K in the same order); each
+// thread keeps silu'(a) of its 16 (row, column) positions in registers for
+// the backward products, which touch exactly those positions again.  The
+// backward products read the transposed BF16 weights, so every product streams
+// k-contiguous 16-byte weight rows.
+constexpr int kSmallRows = 32;
+constexpr int kSmallThreads = 1024;
+constexpr int kSmallMaxDim = 16 * 32;
+
+struct SmallArgs {
+  const float* X;
+  int B, P, H, ld;
+  const __bf16* W[3];   // [H x K]
+  const __bf16* WT[3];  // [K x H]
+  const float* bias[3];
+  const float* w4;
+  const float* b4;
+  float* E;
+  float* G;
+};
+
+// 32 x 32 tile: act[32 x K] (LDS, row stride ld) times rows n0..n0+31 of
+// Wm [N x K] (row-major); columns >= N contribute zero.
+__device__ __forceinline__ f32x16 small_tile(const __bf16* act, int ld, const __bf16* __restrict__ Wm,
+                                             int K, int n0, int N, int lane) {
+  f32x16 acc = {};
+  const int n = n0 + (lane & 31);
+  const bool ok = n < N;
+  const __bf16* wrow = Wm + size_t(ok ? n : 0) * size_t(K);
+  const __bf16* arow = act + (lane & 31) * ld;
+  const int half = 8 * (lane >> 5);
+#pragma unroll 8
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    const bf16x8 av = *reinterpret_cast<const bf16x8*>(arow + k0 + half);
+    bf16x8 bv = *reinterpret_cast<const bf16x8*>(wrow + k0 + half);
+    if (!ok) bv = bf16x8{};
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
+  }
+  return acc;
+}
+
+// D layout of the tile: column lane % 32, row 8 (r / 4) + 4 (lane / 32) + r % 4.
+__device__ __forceinline__ int tile_row(int r, int lane) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
+
+__global__ __launch_bounds__(kSmallThreads) void mlp_small_kernel(SmallArgs a) {
+  extern __shared__ __align__(16) __bf16 sbuf[];
+  __shared__ float epart[kSmallThreads / 64][kSmallRows];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int ld = a.ld, H = a.H, P = a.P, B = a.B;
+  __bf16* in = sbuf;
+  __bf16* out = sbuf + kSmallRows * ld;
+  for (int i = tid; i < kSmallRows * P; i += kSmallThreads) {
+    const int b = i / P, k = i - b * P;
+    in[b * ld + k] = to_bf16(b < B ? a.X[size_t(b) * P + k] : 0.f);
+  }
+  __syncthreads();
+  const bool owner = wave * 32 < H;
+  const int n = wave * 32 + (lane & 31);
+  float d[3][16];
+  float e[16];
+  int K = P;
+  for (int l = 0; l < 3; ++l) {
+    if (owner) {
+      const f32x16 acc = small_tile(in, ld, a.W[l], K, wave * 32, H, lane);
+      const float bn = a.bias[l][n];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int b = tile_row(r, lane);
+        const float x = acc[r] + bn, sg = sigm(x), h = x * sg;
+        d[l][r] = sg * (1.f + x * (1.f - sg));
+        if (l == 2) e[r] = h * a.w4[n];
+        out[b * ld + n] = to_bf16(b < B ? h : 0.f);
+      }
+    }
+    __syncthreads();
+    __bf16* t = in;
+    in = out;
+    out = t;
+    K = H;
+  }
+  // Energies: row sums of h3 * w4 over the 32 columns of each wavefront, then
+  // over the wavefronts.
+  if (owner) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      float v = e[r];
+      for (int off = 1; off < 32; off <<= 1) v += __shfl_xor(v, off, 64);
+      if ((lane & 31) == 0) epart[wave][tile_row(r, lane)] = v;
+    }
+  }
+  // u3 = w4 * silu'(a3) into the free buffer.
+  if (owner) {
+    const float w4n = a.w4[n];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int b = tile_row(r, lane);
+      out[b * ld + n] = to_bf16(b < B ? w4n * d[2][r] : 0.f);
+    }
+  }
+  __syncthreads();
+  if (tid < B) {
+    float s = 0.f;
+    for (int w = 0; w * 32 < H; ++w) s += epart[w][tid];
+    a.E[tid] = s + a.b4[0];
+  }
+  {
+    __bf16* t = in;
+    in = out;
+    out = t;
+  }
+  // v_{l-1} = u_l W_l = u_l (W_l^T)^T;  u_{l-1} = v_{l-1} * silu'(a_{l-1}).
+  for (int l = 2; l >= 1; --l) {
+    if (owner) {
+      const f32x16 acc = small_tile(in, ld, a.WT[l], H, wave * 32, H, lane);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int b = tile_row(r, lane);
+        out[b * ld + n] = to_bf16(b < B ? acc[r] * d[l - 1][r] : 0.f);
+      }
+    }
+    __syncthreads();
+    __bf16* t = in;
+    in = out;
+    out = t;
+  }
+  // dE/dx = u1 W1 = u1 (W1^T)^T: columns p < P.
+  if (a.G && wave * 32 < P) {
+    const f32x16 acc = small_tile(in, ld, a.WT[0], H, wave * 32, P, lane);
+    const int p = wave * 32 + (lane & 31);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int b = tile_row(r, lane);
+      if (b < B && p < P) a.G[size_t(b) * P + p] = acc[r];
+    }
+  }
+}
+
 // a = z + b (row broadcast); h = silu(a); writes a, h (fp32) and h (bf16).
 __global__ void bias_silu_kernel(const float* __restrict__ z, const float* __restrict__ bias, int rows,
                                  int cols, float* __restrict__ a_out, float* __restrict__ h_out,
@@ -358,6 +521,9 @@ MlpLayout MakeMlpLayout(int P, int H, int maxB) {
   L.wb1 = take(size_t(H) * P * 2);
   L.wb2 = take(size_t(H) * H * 2);
   L.wb3 = take(size_t(H) * H * 2);
+  L.wt1 = take(size_t(H) * P * 2);
+  L.wt2 = take(size_t(H) * H * 2);
+  L.wt3 = take(size_t(H) * H * 2);
   const size_t BH = size_t(maxB) * H, BP = size_t(maxB) * P;
   L.xb = take(BP * 2);
   for (int i = 0; i < 3; ++i) {
@@ -395,12 +561,18 @@ int Grid(size_t n) { return int(std::min<size_t>((n + 255) / 256, 4096)); }
 
 void RefreshBf16(const MlpLayout& L, void* ws, hipStream_t s) {
   const float* w = at<float>(ws, L.w);
-  hipLaunchKernelGGL(to_bf16_kernel, dim3(Grid(size_t(L.H) * L.P)), dim3(256), 0, s, w + L.oW1,
-                     at<__bf16>(ws, L.wb1), size_t(L.H) * L.P);
-  hipLaunchKernelGGL(to_bf16_kernel, dim3(Grid(size_t(L.H) * L.H)), dim3(256), 0, s, w + L.oW2,
-                     at<__bf16>(ws, L.wb2), size_t(L.H) * L.H);
-  hipLaunchKernelGGL(to_bf16_kernel, dim3(Grid(size_t(L.H) * L.H)), dim3(256), 0, s, w + L.oW3,
-                     at<__bf16>(ws, L.wb3), size_t(L.H) * L.H);
+  RefreshArgs r{};
+  const size_t oW[3] = {L.oW1, L.oW2, L.oW3}, wb[3] = {L.wb1, L.wb2, L.wb3},
+               wt[3] = {L.wt1, L.wt2, L.wt3};
+  size_t total = 0;
+  for (int l = 0; l < 3; ++l) {
+    r.W[l] = w + oW[l];
+    r.wb[l] = at<__bf16>(ws, wb[l]);
+    r.wt[l] = at<__bf16>(ws, wt[l]);
+    r.cols[l] = l == 0 ? L.P : L.H;
+    r.end[l] = total += size_t(L.H) * size_t(r.cols[l]);
+  }
+  hipLaunchKernelGGL(refresh_bf16_kernel, dim3(Grid(total)), dim3(256), 0, s, r);
 }
 
 // Forward + input gradient for B rows of X (device fp32 [B x P]).
@@ -520,6 +692,31 @@ int MlpSetParams(const MlpLayout& L, void* ws, const float* host_params, hipStre
 int MlpPredict(const MlpLayout& L, void* ws, const float* X, int B, float* E, float* G, hipStream_t s,
                char* err, int64_t len) {
   if (B > L.maxB) return Err(err, len, "batch larger than the workspace was sized for");
+  if (B <= kSmallRows && L.H % 32 == 0 && L.H <= kSmallMaxDim && L.P % 16 == 0 &&
+      L.P <= kSmallMaxDim) {
+    SmallArgs a{};
+    a.X = X;
+    a.B = B;
+    a.P = L.P;
+    a.H = L.H;
+    a.ld = std::max(L.P, L.H) + 8;
+    const float* w = at<float>(ws, L.w);
+    const size_t wb[3] = {L.wb1, L.wb2, L.wb3}, wt[3] = {L.wt1, L.wt2, L.wt3},
+                 ob[3] = {L.ob1, L.ob2, L.ob3};
+    for (int l = 0; l < 3; ++l) {
+      a.W[l] = at<__bf16>(ws, wb[l]);
+      a.WT[l] = at<__bf16>(ws, wt[l]);
+      a.bias[l] = w + ob[l];
+    }
+    a.w4 = w + L.ow4;
+    a.b4 = w + L.ob4;
+    a.E = E;
+    a.G = G;
+    const size_t lds = size_t(2) * kSmallRows * size_t(a.ld) * sizeof(__bf16);
+    hipLaunchKernelGGL(mlp_small_kernel, dim3(1), dim3(kSmallThreads), lds, s, a);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : Err(err, len, std::string("predict: ") + hipGetErrorString(e));
+  }
   Forward(L, ws, X, B, s);
   hipError_t e = hipMemcpyAsync(E, at<float>(ws, L.E), size_t(B) * 4, hipMemcpyDeviceToDevice, s);
   if (e == hipSuccess && G)

@@ -92,6 +92,29 @@ def test_forward_and_input_gradient_match_oracle():
     assert np.abs(E - Ef).max() <= 5e-2 * np.abs(Ef).max()
 
 
+@pytest.mark.parametrize("B", [1, 5, 32])
+def test_fused_small_batch_predict(B):
+    """Batches of up to 32 rows run as one fused launch; B = 37 runs layer by
+    layer.  Both match the oracle at the device's rounding points, and each
+    other: the products run K in the same order, so only the energy's final
+    sum order differs."""
+    import torch
+    model, theta, x, _, _ = _model_and_batch()
+    xs = torch.tensor(x).cuda()
+    _say(f"fused predict B={B}")
+    E, G = model.predict(xs[:B])
+    E_ref, G_ref = model.predict(xs)  # 37 rows: the layer-by-layer path
+    E, G = E.cpu().numpy(), G.cpu().numpy()
+    E_ref, G_ref = E_ref.cpu().numpy()[:B], G_ref.cpu().numpy()[:B]
+    Eo, Go, _ = O.forward(theta, x[:B].astype(np.float64), 144, 512, bf16=True)
+    assert np.abs(E - Eo).max() <= 2e-3 * np.abs(Eo).max()
+    assert np.abs(G - Go).max() <= 2e-3 * np.abs(Go).max()
+    np.testing.assert_allclose(E, E_ref, rtol=1e-5, atol=1e-5 * np.abs(E_ref).max())
+    np.testing.assert_allclose(G, G_ref, rtol=1e-5, atol=1e-6 * np.abs(G_ref).max())
+    E1, G1 = model.predict(xs[:B], with_grad=False)
+    np.testing.assert_array_equal(E1.cpu().numpy(), E)
+
+
 @pytest.mark.parametrize("lam", [0.0, 1.0])
 def test_sobolev_loss_gradient_matches_oracle(lam):
     import torch

@@ -169,12 +169,24 @@ __device__ __forceinline__ f32x16 small_tile(const __bf16* act, int ld, const __
   const __bf16* wrow = Wm + size_t(ok ? n : 0) * size_t(K);
   const __bf16* arow = act + (lane & 31) * ld;
   const int half = 8 * (lane >> 5);
-#pragma unroll 8
-  for (int k0 = 0; k0 < K; k0 += 16) {
-    const bf16x8 av = *reinterpret_cast<const bf16x8*>(arow + k0 + half);
-    bf16x8 bv = *reinterpret_cast<const bf16x8*>(wrow + k0 + half);
-    if (!ok) bv = bf16x8{};
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
+  // Eight k-steps' weight operands are loaded before their MFMAs, so a chunk
+  // costs one memory round trip instead of eight.
+  constexpr int kDepth = 8;
+  for (int c0 = 0; c0 < K; c0 += 16 * kDepth) {
+    bf16x8 bv[kDepth];
+#pragma unroll
+    for (int u = 0; u < kDepth; ++u) {
+      const int k0 = c0 + 16 * u;
+      bv[u] = (ok && k0 < K) ? *reinterpret_cast<const bf16x8*>(wrow + k0 + half) : bf16x8{};
+    }
+#pragma unroll
+    for (int u = 0; u < kDepth; ++u) {
+      const int k0 = c0 + 16 * u;
+      if (k0 < K) {
+        const bf16x8 av = *reinterpret_cast<const bf16x8*>(arow + k0 + half);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv[u], acc, 0, 0, 0);
+      }
+    }
   }
   return acc;
 }

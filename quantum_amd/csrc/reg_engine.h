@@ -149,29 +149,50 @@ __device__ __forceinline__ void make_map(RegMap<R>& m, uint32_t mask, int L) {
   m.tbase = tb;
 }
 
+// Byte offsets in LDS of the 2^R registers of this thread.  Register i's
+// index is tbase | rdep(i) = tbase ^ (XOR of bit(rpos[j]) over the set bits j
+// of i), and the swizzle is XOR-linear (swz(a ^ b) = swz(a) ^ swz(b)), so
+// offset(i) = offset(i without its highest bit h) ^ sb[h]: one XOR per
+// register after R per-map terms.
+template <int R>
+__device__ __forceinline__ void reg_offsets(const RegMap<R>& m, bool swizzled,
+                                            uint32_t (&off)[1 << R]) {
+  uint32_t sb[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const uint32_t b = 1u << m.rpos[j];
+    sb[j] = (swizzled ? swz(b) : b) << 3;
+  }
+  off[0] = (swizzled ? swz(m.tbase) : m.tbase) << 3;
+#pragma unroll
+  for (int i = 1; i < (1 << R); ++i) {
+    const int h = 31 - __builtin_clz(uint32_t(i));
+    off[i] = off[i ^ (1 << h)] ^ sb[h];
+  }
+}
+
 template <int R>
 __device__ __forceinline__ void reg_load(cf (&a)[1 << R], const float2* __restrict__ S,
                                          const RegMap<R>& m, bool swizzled) {
-  const float* Sf = reinterpret_cast<const float*>(S);
+  uint32_t off[1 << R];
+  reg_offsets<R>(m, swizzled, off);
+  const char* base = reinterpret_cast<const char*>(S);
 #pragma unroll
   for (int i = 0; i < (1 << R); ++i) {
-    const uint32_t x = m.tbase | rdep(m, i);
-    const uint32_t p = 2u * (swizzled ? swz(x) : x);
-    a[i] = cf{Sf[p], Sf[p + 1]};
+    const float2 v = *reinterpret_cast<const float2*>(base + off[i]);
+    a[i] = cf{v.x, v.y};
   }
 }
 
 template <int R>
 __device__ __forceinline__ void reg_store(const cf (&a)[1 << R], float2* __restrict__ S,
                                           const RegMap<R>& m, bool swizzled) {
-  float* Sf = reinterpret_cast<float*>(S);
+  uint32_t off[1 << R];
+  reg_offsets<R>(m, swizzled, off);
+  char* base = reinterpret_cast<char*>(S);
 #pragma unroll
-  for (int i = 0; i < (1 << R); ++i) {
-    const uint32_t x = m.tbase | rdep(m, i);
-    const uint32_t p = 2u * (swizzled ? swz(x) : x);
-    Sf[p] = a[i].x;
-    Sf[p + 1] = a[i].y;
-  }
+  for (int i = 0; i < (1 << R); ++i)
+    *reinterpret_cast<float2*>(base + off[i]) = make_float2(a[i].x, a[i].y);
 }
 
 // ------------------------------------------------------------------ dense ops

@@ -10,7 +10,9 @@ from its specification, on qubit masks only:
   matrix, controls kept outside), after closing the open blocks it touches;
 * otherwise, if the union of the gate's qubits and the open blocks it touches
   fits the width, the touched blocks merge (into the oldest) and the gate
-  joins; if not, the touched blocks close and the gate opens a new block;
+  joins; if not, the gate opens a new block, into which the touched blocks
+  that lie within the gate's own qubits move (their gates first, in order of
+  the blocks' indices), while the other touched blocks close;
 * closing a block emits it; at the end the still-open blocks are emitted in
   creation order.
 
@@ -66,9 +68,15 @@ def fuse(gate_masks, width):
             for q in _bits(blocks[keep][1]):
                 owner[q] = keep
         else:
+            gates = []
             for t in touching:
-                close(t)
-            blocks.append([[gi], Q, True])
+                if blocks[t][1] & ~Q == 0:
+                    gates += blocks[t][0]
+                    blocks[t][0] = []
+                    blocks[t][2] = False
+                else:
+                    close(t)
+            blocks.append([gates + [gi], Q, True])
             for q in _bits(Q):
                 owner[q] = len(blocks) - 1
     for bi in range(len(blocks)):

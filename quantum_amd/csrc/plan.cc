@@ -369,8 +369,22 @@ void FuseGates(const Circuit& c, int begin, int end, const FuseOptions& opt,
       for (int q = 0; q < 64; ++q)
         if (blocks[keep].qmask >> q & 1) owner[q] = keep;
     } else {
-      for (int bi : touching) close(bi);
-      blocks.push_back(Block{{gi}, Q, int(blocks.size()), true});
+      // Open blocks within the gate's own qubits join the new block (their
+      // gates first); the others close.  Open blocks are disjoint, so the
+      // absorbed gates commute with those of the closed blocks.
+      Block nb{{}, Q, int(blocks.size()), true};
+      for (int bi : touching) {
+        Block& o = blocks[bi];
+        if ((o.qmask & ~Q) == 0) {
+          nb.gates.insert(nb.gates.end(), o.gates.begin(), o.gates.end());
+          o.open = false;
+          o.gates.clear();
+        } else {
+          close(bi);
+        }
+      }
+      nb.gates.push_back(gi);
+      blocks.push_back(nb);
       for (int q = 0; q < 64; ++q)
         if (Q >> q & 1) owner[q] = int(blocks.size()) - 1;
     }
@@ -507,8 +521,23 @@ void FuseWithParams(const Circuit& c, const FuseOptions& opt, bool tfq_fd,
       k.nderiv += g.nsym;
       claim(keep);
     } else {
-      for (int bi : touching) close(bi);
-      blocks.push_back(Block{{gi}, Q, g.nsym, true});
+      // As FuseGates: open blocks within the gate's qubits join the new block
+      // while the derivative count allows; the others close.
+      Block nb{{}, Q, g.nsym, true};
+      for (int bi : touching) {
+        Block& o = blocks[size_t(bi)];
+        if ((o.qmask & ~Q) == 0 && nb.nderiv + o.nderiv <= kMaxDeriv &&
+            Popc(Q) <= (nb.nderiv + o.nderiv > 0 ? pmax : opt.max_fuse)) {
+          nb.gates.insert(nb.gates.end(), o.gates.begin(), o.gates.end());
+          nb.nderiv += o.nderiv;
+          o.open = false;
+          o.gates.clear();
+        } else {
+          close(bi);
+        }
+      }
+      nb.gates.push_back(gi);
+      blocks.push_back(nb);
       claim(int(blocks.size()) - 1);
     }
   }

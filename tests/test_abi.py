@@ -178,9 +178,10 @@ def test_fuser_replica_pinned_by_hand_example():
                                    [C.CNOT(q[1], q[2])], [C.X(q[3])]])
     data = S.serialize_circuit(circ)
     # qubit q -> bit 3 - q.  H0 opens {3}; CZ joins -> {2,3}; H2 opens {1};
-    # CNOT(1,2) touches {2,3} and {1}: union 3 qubits > 2 -> both close, new
-    # block {1,2}; X3 opens {0}; end: {1,2} then {0}.
-    want = [((2, 3), False, (0, 1)), ((1,), False, (2,)), ((1, 2), False, (3,)), ((0,), False, (4,))]
+    # CNOT(1,2) touches {2,3} and {1}: union 3 qubits > 2 -> CNOT opens block
+    # {1,2}, into which {1} (inside the gate's qubits) moves while {2,3}
+    # closes; X3 opens {0}; end: {1,2} = (H2, CNOT) then {0}.
+    want = [((2, 3), False, (0, 1)), ((1, 2), False, (2, 3)), ((0,), False, (4,))]
     assert fuser.fuse(fuser.program_masks(data), 2) == want
     assert _lib.fusion_decisions(data, width=2) == want
 

@@ -290,7 +290,9 @@ int tfqx_ps_weights_from_symbols(const uint8_t* programs, const int64_t* program
  * parameterised one-qubit ops, [16] reverse parameterised two-qubit ops,
  * [17] reverse parameterised ops with a diagonal matrix, [18] forward
  * remaps, [19] forward diagonal ops, [20] forward dense one-qubit ops, [21]
- * forward dense wider ops.  Noisy plans: [0] non-empty
+ * forward dense wider ops, [22] register-engine remaps that keep every
+ * amplitude in its wavefront (run without workgroup barriers), [23] all
+ * register-engine remaps of workgroup-run circuits.  Noisy plans: [0] non-empty
  * circuits, [1] ops (gates and channels), [2] trajectories per launch, [8]
  * expected algorithmic real flops of one launch.  No reference counterpart. */
 int tfqx_plan_stats(const tfqx_plan* plan, int64_t* out, int64_t n);

@@ -215,11 +215,11 @@ class Plan:
     def op_mix(self):
         """Op mix of a batch-kernel plan in amplitude-op units (2^n per op of
         an n-qubit circuit, / 1024); zeros for streaming and noisy plans."""
-        out = (ctypes.c_int64 * 22)()
-        self._lib.tfqx_plan_stats(self.handle, out, 22)
+        out = (ctypes.c_int64 * 24)()
+        self._lib.tfqx_plan_stats(self.handle, out, 24)
         keys = ("forward", "rev_remap", "rev_diag", "rev_dense1", "rev_dense2", "rev_param1",
                 "rev_param2", "rev_param_diag", "fwd_remap", "fwd_diag", "fwd_dense1",
-                "fwd_dense2")
+                "fwd_dense2", "remap_wave_local", "remap_engine")
         return dict(zip(keys, [int(x) for x in out[10:]]))
 
     def rebind(self, values):

@@ -430,8 +430,45 @@ double MfmaFlopsOfRange(const std::vector<DevOp>& ops, int b, int e, double A, b
 // parameterised two-qubit ops, [7] reverse parameterised ops whose matrix is
 // diagonal, [8] forward remaps, [9] forward diagonal ops, [10] forward dense
 // one-qubit ops, [11] forward dense wider ops.
-void OpMix(const BatchProgram& g, double (&m)[12]) {
+// Remaps of op ranges [b, e) on an L-qubit tile (register engine) that keep
+// the wavefront-index positions of the previous map (the engine runs them
+// without workgroup barriers), in amplitude-op units; [13] counts all
+// remaps of those ranges.
+void LocalRemaps(const std::vector<DevOp>& ops, int b, int e, int L, double A, double& local,
+                 double& all) {
+  if (b >= e || ops[size_t(b)].type != kOpRemap || RegBitsForTile(L) == 0) return;
+  auto wp = [L](uint32_t mask) {
+    uint32_t rest = ((1u << L) - 1u) & ~mask;
+    for (int k = 0; k < 6; ++k) rest &= rest - 1u;
+    return rest;
+  };
+  int seen = 0;
+  uint32_t prev = 0;
+  for (int i = b; i < e; ++i) {
+    const DevOp& o = ops[size_t(i)];
+    if (o.type != kOpRemap) continue;
+    all += A;
+    if (seen >= 2 && wp(o.cmask) == wp(prev)) local += A;
+    prev = o.cmask;
+    ++seen;
+  }
+}
+
+void OpMix(const BatchProgram& g, double (&m)[14]) {
   for (double& x : m) x = 0;
+  for (const auto& c : g.circuits) {
+    if ((c.flags & kFlagEmpty) || (c.flags & kFlagWave)) continue;
+    const double A = std::ldexp(1.0, c.n) / 1024.0;
+    if (c.flags & kFlagTiled) {
+      for (int p = c.fwd_pass_begin; p < c.rev_pass_end; ++p) {
+        const DevPass& ps = g.passes[size_t(p)];
+        LocalRemaps(g.ops, ps.op_begin, ps.op_end, ps.L, A, m[12], m[13]);
+      }
+    } else {
+      LocalRemaps(g.ops, c.fwd_begin, c.fwd_end, c.n, A, m[12], m[13]);
+      LocalRemaps(g.ops, c.rev_begin, c.rev_end, c.n, A, m[12], m[13]);
+    }
+  }
   auto diag_matrix = [&](const DevOp& o) {
     const int D = 1 << o.k;
     for (int r = 0; r < D; ++r)
@@ -865,11 +902,11 @@ int tfqx_plan_rebind_downstream(tfqx_plan* p, const float* downstream, char* err
 
 int tfqx_plan_stats(const tfqx_plan* p, int64_t* out, int64_t n) {
   if (!p || !out) return TFQX_INVALID_ARGUMENT;
-  int64_t v[22] = {0};
+  int64_t v[24] = {0};
   if (!p->large && !p->noisy && n > 10) {
-    double mix[12];
+    double mix[14];
     OpMix(p->prog, mix);
-    for (int i = 0; i < 12; ++i) v[10 + i] = int64_t(mix[i]);
+    for (int i = 0; i < 14; ++i) v[10 + i] = int64_t(mix[i]);
   }
   v[3] = p->max_nq;
   if (p->large) {
@@ -938,7 +975,7 @@ int tfqx_plan_stats(const tfqx_plan* p, int64_t* out, int64_t n) {
     v[8] = int64_t(total);
   }
   v[7] = int64_t(p->total);
-  for (int64_t i = 0; i < n && i < 22; ++i) out[i] = v[i];
+  for (int64_t i = 0; i < n && i < 24; ++i) out[i] = v[i];
   return TFQX_OK;
 }
 

@@ -332,6 +332,17 @@ __device__ __forceinline__ void dispatch2(int j0, int j1, cf (&a)[1 << R], cf (&
   }
 }
 
+// Tile positions of the wavefront-index bits of the thread index (bits
+// 6..8) under register mask `mask`: make_map gives thread bit k the k-th
+// lowest non-register position, so they are the non-register positions left
+// after the six lowest (the lane bits).
+__device__ __forceinline__ uint32_t wave_positions(uint32_t mask, int L) {
+  uint32_t rest = ((1u << L) - 1u) & ~mask;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) rest &= rest - 1u;
+  return rest;
+}
+
 template <int R>
 __device__ __forceinline__ int reg_slot(const RegMap<R>& m, uint32_t q) {
   return __builtin_popcount(m.mask & ((1u << q) - 1u));
@@ -849,7 +860,7 @@ __device__ __forceinline__ void reg_run(float2* __restrict__ A, float2* __restri
   cf ra[1 << R];
   cf rb[1 << R];  // unused when !REV
   RegMap<R> m;
-  bool have = false, swizzled = false;
+  bool have = false, swizzled = false, first_store = true;
   int slot = 0;
   for (int i = b; i < e; ++i) {
     DevOp op = load_op(ops + i);
@@ -864,14 +875,25 @@ __device__ __forceinline__ void reg_run(float2* __restrict__ A, float2* __restri
           continue;
         }
       }
+      // A remap that leaves the wavefront-index qubits where they are moves
+      // every amplitude within its own wavefront: wavefront w stores and
+      // reloads exactly the (swizzled) LDS entries it read at the previous
+      // remap, so it needs no workgroup barrier.  The first store of a run
+      // still waits for the whole workgroup (entries last read in another
+      // layout, or by other wavefronts on the previous tile).
+      const uint32_t nmask = uint32_t(uniform(int(op.cmask)));
+      const bool local = !WAVE && have && !first_store &&
+                         wave_positions(m.mask, L) == wave_positions(nmask, L);
       if (have) {
-        esync<WAVE>();
+        if (!local) esync<WAVE>();
         reg_store<R>(ra, A, m, true);
         if (REV) reg_store<R>(rb, B, m, true);
         swizzled = true;
+        first_store = false;
       }
-      esync<WAVE>();
-      make_map<R, WAVE>(m, op.cmask, L);
+      if (local) esync<true>();
+      else esync<WAVE>();
+      make_map<R, WAVE>(m, nmask, L);
       reg_load<R>(ra, A, m, swizzled);
       if (REV) reg_load<R>(rb, B, m, swizzled);
       have = true;

@@ -60,6 +60,75 @@ __device__ __forceinline__ uint32_t cdf_first_above(const double* cdf, uint32_t 
   return lo;
 }
 
+// The first x whose inclusive FP64 prefix of |psi|^2 exceeds u * total
+// (the last index if none does), for one draw from a workgroup's state (LDS
+// or global; N >= kThreads).  The state is split into kThreads contiguous
+// chunks; wavefronts read each chunk with consecutive lanes on consecutive
+// amplitudes (coalesced) and reduce its mass, a block scan of the chunk masses
+// finds the chunk, and one wavefront scans that chunk 64 amplitudes at a time.
+// Nothing is written to the state.
+__device__ __forceinline__ double wave_inclusive_scan(double v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const double t = __shfl_up(v, d, 64);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint32_t sample_one(const float2* __restrict__ s, uint32_t N, double u,
+                                               double* red) {
+  __shared__ double mass[kThreads];
+  __shared__ int s_chunk;
+  __shared__ uint32_t s_x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t C = N / kThreads;
+  for (int k = wave; k < kThreads; k += kWaves) {
+    double m = 0.0;
+    for (uint32_t i = uint32_t(lane); i < C; i += 64) {
+      const float2 v = s[size_t(k) * C + i];
+      m += double(v.x) * double(v.x) + double(v.y) * double(v.y);
+    }
+    m = wave_sum(m);
+    if (lane == 0) mass[k] = m;
+  }
+  if (tid == 0) s_chunk = kThreads - 1;
+  __syncthreads();
+  const double mk = mass[tid];
+  double total;
+  const double off = block_exclusive_scan(mk, red, &total);
+  const double target = u * total;
+  if (off + mk > target) atomicMin(&s_chunk, tid);
+  __syncthreads();
+  const int k = s_chunk;
+  if (tid == k) red[0] = off;
+  __syncthreads();
+  if (wave == 0) {
+    const double goal = target - red[0];
+    double run = 0.0;
+    uint32_t x = uint32_t(k + 1) * C - 1u;
+    for (uint32_t i0 = 0; i0 < C; i0 += 64) {
+      const uint32_t i = i0 + uint32_t(lane);
+      double p = 0.0;
+      if (i < C) {
+        const float2 v = s[size_t(k) * C + i];
+        p = double(v.x) * double(v.x) + double(v.y) * double(v.y);
+      }
+      const double pre = run + wave_inclusive_scan(p);
+      const unsigned long long hit = __ballot(i < C && pre > goal);
+      if (hit) {
+        x = uint32_t(k) * C + i0 + uint32_t(__ffsll((long long)hit) - 1);
+        break;
+      }
+      run = __shfl(pre, 63, 64);
+    }
+    if (lane == 0) s_x = x;
+  }
+  __syncthreads();
+  return s_x;
+}
+
 // ------------------------------------------------------------------ workgroup trajectories
 // One channel on an LDS state (all threads; selection rule as reg_channel_j).
 __device__ __forceinline__ void lds_channel(float2* s, int n, const DevOp& op, const float2* mats,
@@ -135,25 +204,8 @@ __device__ __forceinline__ void wg_trajectory(const NoisyArgs& a, const DevChunk
             }
           });
   if (MODE == kModeNoisySamples) {
-    const double total = [&] {
-      // FP64 inclusive prefix of |psi|^2 in place.
-      double* cdf = reinterpret_cast<double*>(psi);
-      const uint32_t chunk = (N + kThreads - 1) / kThreads;
-      const uint32_t b = tid * chunk, e = min(N, b + chunk);
-      double run = 0.0;
-      for (uint32_t x = b; x < e; ++x) {
-        const float2 v = psi[x];
-        run += double(v.x) * double(v.x) + double(v.y) * double(v.y);
-        cdf[x] = run;
-      }
-      double tot;
-      const double off = block_exclusive_scan(run, red, &tot);
-      for (uint32_t x = b; x < e; ++x) cdf[x] += off;
-      __syncthreads();
-      return tot;
-    }();
     const double u = philox_uniform(a.seed, traj, uint32_t(row), kStreamSample);
-    const uint32_t x = cdf_first_above(reinterpret_cast<const double*>(psi), N, u * total);
+    const uint32_t x = sample_one(psi, N, u, red);
     signed char* o = a.samples_out + (size_t(row) * a.nshots + size_t(traj)) * a.max_nq;
     for (int q = tid; q < a.max_nq; q += kThreads)
       o[a.max_nq - 1 - q] = q < ch.nq ? (signed char)((x >> q) & 1u) : (signed char)-2;

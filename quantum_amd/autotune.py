@@ -39,7 +39,7 @@ STREAM_CANDIDATES = (
     {"fusion_width": 2, "register_engine": 2, "tile_single": 13},
     {"fusion_width": 2, "register_engine": 1, "tile_single": 13},
     {"fusion_width": 2, "register_engine": 2, "tile_single": 12},
-    {"fusion_width": 2, "register_engine": 1, "tile_single": 14},
+    {"fusion_width": 2, "register_engine": 2, "tile_single": 14},
     {"fusion_width": 5, "register_engine": 2, "tile_single": 13},
     {"fusion_width": 6, "register_engine": 2, "tile_single": 13},
 )

@@ -50,7 +50,7 @@ DevTermWide WideTerm(const Pauli& p, float w, int slot, bool phase_y, int index)
 
 void SchedulePasses(const std::vector<FusedOp>& ops, int n, int L, std::vector<DevPass>* passes,
                     std::vector<DevOp>* dev_ops, std::vector<cfloat>* mats,
-                    bool register_engine, const Circuit* c, SharedRecipe* rec) {
+                    int max_r, const Circuit* c, SharedRecipe* rec) {
   L = std::min(L, n);
   const int low_run = std::min(kMinLowRun, L);
   const uint64_t lowmask = (uint64_t(1) << low_run) - 1;
@@ -176,7 +176,7 @@ void SchedulePasses(const std::vector<FusedOp>& ops, int n, int L, std::vector<D
       if (rec && c) AppendRecipe(f, *c, d, rec);
       dev_ops->push_back(d);
     }
-    if (register_engine) InsertRemaps(dev_ops, size_t(p.op_begin), L);
+    if (max_r) InsertRemaps(dev_ops, size_t(p.op_begin), L, max_r);
     p.op_end = int(dev_ops->size());
     passes->push_back(p);
     remaining.swap(deferred);
@@ -238,8 +238,9 @@ void AddPasses(LargePlan* lp, const std::vector<FusedOp>& ops, int n, int L, Pas
   if (lp->too_wide) return;
   std::vector<DevPass> passes;
   // The streaming passes keep the LDS path unless tuning level 2 asks for the
-  // register engine there too.
-  SchedulePasses(ops, n, L, &passes, &lp->ops, &lp->mats, GetTuning().reg_engine >= 2);
+  // register engine there too; one-state 14-qubit tiles get five register
+  // qubits (pass_kernel's wide instantiation).
+  SchedulePasses(ops, n, L, &passes, &lp->ops, &lp->mats, GetTuning().reg_engine >= 2 ? 5 : 0);
   pp->passes.insert(pp->passes.end(), passes.begin(), passes.end());
 }
 

@@ -88,11 +88,13 @@ Status BuildLargePlan(int mode, const std::vector<Circuit>& circuits,
 constexpr int kDenseTile = 12;
 
 // Group fused ops into tile passes (exposed for tests of the scheduler).
+// max_r: largest register-engine register count for the passes (0: LDS path
+// only; 4: tiles of 12-13 qubits; 5: also 14-qubit one-state tiles).
 // With `rec` (and the circuit `c` the ops come from), each op's matrix recipe
 // is recorded for rebinding.
 void SchedulePasses(const std::vector<FusedOp>& ops, int n, int L, std::vector<DevPass>* passes,
                     std::vector<DevOp>* dev_ops, std::vector<cfloat>* mats,
-                    bool register_engine, const Circuit* c = nullptr,
+                    int max_r, const Circuit* c = nullptr,
                     SharedRecipe* rec = nullptr);
 
 size_t LargeWorkspaceBytes(LargePlan* p);

@@ -64,8 +64,11 @@ struct PassArgs {
 // CPLX (two-buffer passes only): each derivative also accumulates the
 // imaginary part of <dG psi|lambda>, written to the partial slot after the
 // real part's.
-template <bool TWO, bool CPLX = false>
-__global__ __launch_bounds__(kThreads, TWO ? 2 : 4) void pass_kernel(PassArgs a) {
+// WIDE: one-state passes on 14-qubit tiles (128 KiB of LDS, one workgroup
+// per CU) with the five-register-qubit engine and the VGPR budget of two
+// waves per SIMD.
+template <bool TWO, bool CPLX = false, bool WIDE = false>
+__global__ __launch_bounds__(kThreads, (TWO || WIDE) ? 2 : 4) void pass_kernel(PassArgs a) {
   constexpr int kPacc = TWO ? (CPLX ? 2 : 1) * kMaxPassDerivs : 1;
   extern __shared__ float2 lds[];
   __shared__ double red[kWaves];
@@ -121,7 +124,8 @@ __global__ __launch_bounds__(kThreads, TWO ? 2 : 4) void pass_kernel(PassArgs a)
             a.dst[g] = make_float2(va.x, va.y);
           }
         });
-    if (reg_run_any<TWO>(A, Bf, L, a.ops, ps.op_begin, ps.op_end, mc, acc, io)) continue;
+    if (reg_run_any<TWO, WIDE ? 5 : 4>(A, Bf, L, a.ops, ps.op_begin, ps.op_end, mc, acc, io))
+      continue;
     if (TWO) {
       tile_gather2(A, Bf, a.psi, a.lam, T, gidx);
     } else if (a.init) {
@@ -800,7 +804,8 @@ int LargeLaunch(LargePlan* p, void* ws, void* output, hipStream_t stream, char* 
       const uint64_t ntiles = uint64_t(1) << (n - ps.L);
       const int g = int(std::min<uint64_t>(ntiles, uint64_t(grid)));
       const size_t lds = (size_t(1) << ps.L) * sizeof(float2);
-      hipLaunchKernelGGL(pass_kernel<false>, dim3(g), dim3(kThreads), lds, stream, a);
+      if (ps.L > 13) hipLaunchKernelGGL((pass_kernel<false, false, true>), dim3(g), dim3(kThreads), lds, stream, a);
+      else hipLaunchKernelGGL(pass_kernel<false>, dim3(g), dim3(kThreads), lds, stream, a);
       first = false;
     }
     if (first) {  // no ops at all: still need |0>

@@ -189,9 +189,9 @@ void SetTuning(const Tuning& t) {
   g_tile_two.store(t.tile_two);
 }
 
-void InsertRemaps(std::vector<DevOp>* ops, size_t begin, int L) {
+void InsertRemaps(std::vector<DevOp>* ops, size_t begin, int L, int max_r) {
   if (!g_reg_engine.load()) return;
-  InsertRemapsR(ops, begin, L, RegBitsForTile(L));
+  InsertRemapsR(ops, begin, L, RegBitsForTile(L, max_r));
 }
 
 void InsertRemapsR(std::vector<DevOp>* ops, size_t begin, int L, int R) {
@@ -750,7 +750,7 @@ void BuildPiece(const CircuitJob& job, int num_slots, const PlanOptions& opt, Pi
   dc.fwd_begin = 0;
   dc.fwd_pass_begin = 0;
   if (tiled) {
-    SchedulePasses(fused, dc.n, opt.tile_log2, &pc->passes, &pc->ops, &pc->mats, true, rc, rec);
+    SchedulePasses(fused, dc.n, opt.tile_log2, &pc->passes, &pc->ops, &pc->mats, 4, rc, rec);
   } else {
     for (const auto& f : fused) pc->ops.push_back(MakeDevOp(f, &pc->mats, rc, rec));
     remaps(size_t(dc.fwd_begin));
@@ -772,7 +772,7 @@ void BuildPiece(const CircuitJob& job, int num_slots, const PlanOptions& opt, Pi
       else cost += 2 * amps * (f.diag ? 1 : (1 << f.k)) + 2048.0;
     }
     if (tiled) {
-      SchedulePasses(rev_ops, dc.n, opt.tile_log2, &pc->passes, &pc->ops, &pc->mats, true, rc,
+      SchedulePasses(rev_ops, dc.n, opt.tile_log2, &pc->passes, &pc->ops, &pc->mats, 4, rc,
                      rec);
     } else {
       for (const auto& f : rev_ops) pc->ops.push_back(MakeDevOp(f, &pc->mats, rc, rec));

@@ -104,11 +104,11 @@ int GetReverseFusion();
 // Register-engine segments: rewrite ops[begin, end) of an op list that runs on
 // a tile of L qubits so that it starts with a kOpRemap and every non-diagonal
 // op's targets lie in the register qubits of the latest remap (segments of at
-// most RegBitsForTile(L) target qubits).  A segment may take an op ahead of
+// most RegBitsForTile(L, max_r) target qubits).  A segment may take an op ahead of
 // earlier ops that commute with it (disjoint supports, or both diagonal), so
 // the op order changes only between commuting ops.  No-op when the tile size
 // has no register engine or an op is wider than 2 qubits.
-void InsertRemaps(std::vector<DevOp>* ops, size_t begin, int L);
+void InsertRemaps(std::vector<DevOp>* ops, size_t begin, int L, int max_r = 4);
 // The same with an explicit register-qubit count R (wave mode).
 void InsertRemapsR(std::vector<DevOp>* ops, size_t begin, int L, int R);
 

@@ -937,14 +937,22 @@ __device__ __forceinline__ void reg_run(float2* __restrict__ A, float2* __restri
 // Run ops [b, e) on a tile of L qubits with the register engine when the op
 // list was planned for it (starts with a remap); returns false otherwise,
 // before touching the tile.
-template <bool REV, typename AccFn, typename IO>
+// RMAX = 5 adds 14-qubit one-state tiles (32 registers per thread); only the
+// streaming pass kernel's wide instantiation, whose plans use them, asks for it.
+template <bool REV, int RMAX = 4, typename AccFn, typename IO>
 __device__ __forceinline__ bool reg_run_any(float2* A, float2* B, int L, const DevOp* ops, int b,
                                             int e, MatCache& mc, AccFn&& acc_fn, IO& io) {
   if (b >= e) return false;
   if (uniform(int(ops[b].type)) != kOpRemap) return false;
-  switch (RegBitsForTile(L)) {
+  switch (RegBitsForTile(L, RMAX)) {
     case 3: reg_run<3, REV, false>(A, B, L, ops, b, e, mc, acc_fn, io); return true;
     case 4: reg_run<4, REV, false>(A, B, L, ops, b, e, mc, acc_fn, io); return true;
+    case 5:
+      if constexpr (RMAX >= 5 && !REV) {
+        reg_run<5, REV, false>(A, B, L, ops, b, e, mc, acc_fn, io);
+        return true;
+      }
+      return false;
     default: return false;
   }
 }

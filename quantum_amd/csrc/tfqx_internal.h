@@ -48,7 +48,12 @@ constexpr int kRegThreadBits = 9;
 // complex entries over the plan's matrix array; the device copy of that array
 // is padded by the same count so a window never reads past it.
 constexpr int kMatCache = 768;
-constexpr int RegBitsForTile(int L) { return (L >= 12 && L <= 13) ? L - kRegThreadBits : 0; }
+// Register qubits of the register engine for an L-qubit tile (0: no engine):
+// L - 9 for tiles of 12..(9 + max_r) qubits.  Two-state runs and the batched
+// kernel use at most 4; one-state streaming passes may use 5 (14-qubit tiles).
+constexpr int RegBitsForTile(int L, int max_r = 4) {
+  return (L >= 12 && L - kRegThreadBits <= max_r) ? L - kRegThreadBits : 0;
+}
 
 // Wave mode of the batched kernel: after the workgroup-sized circuits, each
 // wavefront runs one small circuit (kWaveRegBits..kWaveMaxQubits qubits) in

@@ -221,7 +221,8 @@ void ExercisePlan(const Circuit& c) {
       std::vector<DevPass> passes;
       std::vector<DevOp> ops;
       std::vector<cfloat> mats;
-      SchedulePasses(fused, c.num_qubits, std::min(13, c.num_qubits), &passes, &ops, &mats, w == 2);
+      SchedulePasses(fused, c.num_qubits, std::min(13 + (w == 2), c.num_qubits), &passes, &ops,
+                     &mats, w == 2 ? 5 : 0);
     }
   }
 }

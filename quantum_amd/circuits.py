@@ -317,6 +317,30 @@ class IdentityGate(Gate):
     gate_id = "I"
 
 
+class MeasurementGate(Gate):
+    """Computational-basis measurement of ``num_qubits`` qubits.  It has no
+    serialized id: ``serialize_circuit`` removes measurements (and requires
+    each qubit to be measured at most once), as the reference serializer does
+    (``serializer.py:826-860``)."""
+
+    gate_id = ""
+
+    def __init__(self, num_qubits: int = 1, key: str = ""):
+        if num_qubits < 1:
+            raise ValueError("a measurement acts on at least one qubit")
+        self.num_qubits = int(num_qubits)
+        self.key = str(key)
+
+    def __repr__(self):
+        return f"MeasurementGate({self.num_qubits}, key={self.key!r})"
+
+
+def measure(*qubits, key: str = "") -> "Operation":
+    """Measure the given qubits (``cirq.measure``)."""
+    qs = list(_flatten_qubits(qubits))
+    return MeasurementGate(len(qs), key).on(*qs)
+
+
 # --------------------------------------------------------------------------- channels
 class Channel(Gate):
     """A single-qubit noise channel.  Channels carry plain float arguments (no

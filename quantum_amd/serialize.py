@@ -167,18 +167,35 @@ def _operation(op: C.Operation) -> bytes:
 
 # --------------------------------------------------------------------------- public API
 def serialize_circuit(circuit: C.Circuit) -> bytes:
-    """``serializer.serialize_circuit`` -> ``Program`` bytes."""
+    """``serializer.serialize_circuit`` -> ``Program`` bytes.
+
+    Measurements are removed, as the reference does (``serializer.py:826-860``):
+    a qubit measured a second time raises ``ValueError``; every moment that
+    holds a measurement loses the operations on the qubits measured in it."""
     if not isinstance(circuit, C.Circuit):
         raise TypeError("serialize requires Circuit objects. Given: " + str(type(circuit)))
+    measured_all = set()
+    measured = []
     for m in circuit.moments:
+        here = set()
         for op in m.operations:
             for q in op.qubits:
                 if not isinstance(q, (C.GridQubit, C.LineQubit)):
                     raise ValueError("Attempted to serialize circuit that don't use "
                                      "only GridQubits or LineQubits.")
+            if isinstance(op.gate, C.MeasurementGate):
+                for q in op.qubits:
+                    if q in measured_all:
+                        raise ValueError("Serialization of circuit failed. Circuits with "
+                                         "non-terminal measurement operations are not "
+                                         "supported.")
+                    here.add(q)
+                    measured_all.add(q)
+        measured.append(here)
     circ = _key(1, 0) + _varint(1)  # MOMENT_BY_MOMENT
-    for m in circuit.moments:
-        mom = b"".join(_len_field(1, _operation(op)) for op in m.operations)
+    for m, here in zip(circuit.moments, measured):
+        ops = [op for op in m.operations if not any(q in here for q in op.qubits)]
+        mom = b"".join(_len_field(1, _operation(op)) for op in ops)
         circ += _len_field(2, mom)
     lang = _str_field(1, GATE_SET_NAME)
     return _len_field(1, lang) + _len_field(2, circ)

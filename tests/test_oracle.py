@@ -472,3 +472,28 @@ def test_serializer_rounds_to_six_decimals():
     dec = wire.decode_program(S.serialize_circuit(C.Circuit(C.rx(0.5).on(q))))
     e = dec[0][0]["args"]["exponent"]["float"]
     assert e == np.float32(0.159155)
+
+
+def test_serializer_removes_measurements_as_the_reference():
+    """``serializer.py:826-860``: measurements are removed (their moment
+    stays, without the ops on the qubits measured in it); a qubit measured
+    twice is an error; a gate after a measurement is kept."""
+    q = C.LineQubit.range(3)
+    with_m = C.Circuit.from_moments([[C.H(q[0])], [C.CNOT(q[0], q[1]), C.X(q[2])],
+                                     [C.measure(q[0], q[1], key="m"), C.Y(q[2])]])
+    without = C.Circuit.from_moments([[C.H(q[0])], [C.CNOT(q[0], q[1]), C.X(q[2])],
+                                      [C.Y(q[2])]])
+    assert S.serialize_circuit(with_m) == S.serialize_circuit(without)
+    later = C.Circuit.from_moments([[C.measure(q[0])], [C.X(q[0])]])
+    assert S.serialize_circuit(later) == S.serialize_circuit(
+        C.Circuit.from_moments([[], [C.X(q[0])]]))
+    twice = C.Circuit.from_moments([[C.measure(q[0])], [C.measure(q[0], q[1])]])
+    with pytest.raises(ValueError, match="non-terminal measurement"):
+        S.serialize_circuit(twice)
+    # The measured program simulates like the unmeasured one.
+    ps = [[C.PauliSum([C.pauli_z(q[1]) * 0.5, C.pauli_x(q[2])])]]
+    e1 = O.simulate_expectation([S.serialize_circuit(with_m)], [], np.zeros((1, 0)),
+                                S.convert_to_tensor(ps))
+    e0 = O.simulate_expectation([S.serialize_circuit(without)], [], np.zeros((1, 0)),
+                                S.convert_to_tensor(ps))
+    np.testing.assert_allclose(e1, e0, atol=1e-12)

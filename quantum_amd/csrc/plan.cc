@@ -157,6 +157,39 @@ void BlockMatrix(const std::vector<Gate>& gs, const int* gates, int ngates, cons
   for (int i = 0; i < dim * dim; ++i) out[i] = cdouble(acc[2 * i], acc[2 * i + 1]);
 }
 
+// A parameterised block's product and derivative products in one sweep: M =
+// U_{m-1} ... U_0 and, for derivative s of gate dgate[s] with derivative
+// matrix dU[s] (the gate's layout), U_{m-1} ... dU[s] ... U_0.  Each
+// derivative product copies the running prefix at its gate and then sees the
+// same multiplications as a product built from the identity, so the results
+// equal BlockMatrix's bit for bit.
+void ParamBlockMatrices(const std::vector<Gate>& gs, const int* gates, int ngates, const int* bits,
+                        int k, int nderiv, const int* dgate, const cdouble (*dU)[16],
+                        cdouble* M, cdouble* const* dM) {
+  const int dim = 1 << k, n2 = 2 * dim * dim;
+  double acc[2 * 16], dacc[kMaxDeriv][2 * 16];
+  for (int i = 0; i < n2; ++i) acc[i] = 0;
+  for (int i = 0; i < dim; ++i) acc[2 * (i * dim + i)] = 1.0;
+  bool live[kMaxDeriv] = {false, false};
+  for (int t = 0; t < ngates; ++t) {
+    const Gate& g = gs[size_t(gates[t])];
+    for (int s = 0; s < nderiv; ++s) {
+      if (dgate[s] != t) continue;
+      for (int i = 0; i < n2; ++i) dacc[s][i] = acc[i];
+      Gate dg = g;
+      for (int e = 0; e < 16; ++e) dg.U[e] = dU[s][e];
+      ApplyLeft(dg, bits, k, dacc[s]);
+      live[s] = true;
+    }
+    for (int s = 0; s < nderiv; ++s)
+      if (live[s] && dgate[s] != t) ApplyLeft(g, bits, k, dacc[s]);
+    ApplyLeft(g, bits, k, acc);
+  }
+  for (int i = 0; i < dim * dim; ++i) M[i] = cdouble(acc[2 * i], acc[2 * i + 1]);
+  for (int s = 0; s < nderiv; ++s)
+    for (int i = 0; i < dim * dim; ++i) dM[s][i] = cdouble(dacc[s][2 * i], dacc[s][2 * i + 1]);
+}
+
 FusedOp BlockToOp(const Circuit& c, const std::vector<int>& gates, uint64_t qmask,
                   bool structural_diag) {
   FusedOp op;
@@ -421,28 +454,32 @@ FusedOp ParamGateOp(const Circuit& c, int gi, const FuseOptions& opt, bool tfq_f
 // with that gate's matrix replaced by its derivative.
 void MakeParamBlock(const Circuit& c, bool tfq_fd, FusedOp* f) {
   const int dim = 1 << f->k;
-  std::vector<Gate> loc;
-  loc.reserve(f->gates.size());
-  for (int gi : f->gates) loc.push_back(c.gates[size_t(gi)]);
-  std::vector<int> idx(loc.size());
-  std::iota(idx.begin(), idx.end(), 0);
   f->param = true;
   f->fused_param = true;
   f->diag = false;
   f->nderiv = 0;
-  for (size_t j = 0; j < loc.size(); ++j) {
+  int dgate[kMaxDeriv];
+  cdouble dU[kMaxDeriv][16];
+  for (size_t j = 0; j < f->gates.size(); ++j) {
     const Gate& g = c.gates[size_t(f->gates[j])];
     for (int d = 0; d < g.nsym; ++d) {
       const int s = f->nderiv++;
-      GateDerivative(g.kind, g.params, g.sym_pidx[d], tfq_fd, loc[j].U);
-      f->dM[s].assign(size_t(dim) * dim, 0.0);
-      BlockMatrix(loc, idx.data(), int(idx.size()), f->bits, f->k, f->dM[s].data());
-      for (int e = 0; e < 16; ++e) loc[j].U[e] = g.U[e];
+      GateDerivative(g.kind, g.params, g.sym_pidx[d], tfq_fd, dU[s]);
+      dgate[s] = int(j);
       f->col[s] = g.sym_col[d];
       f->dgate[s] = int(j);
       f->dsym[s] = d;
     }
   }
+  std::vector<cdouble> M(size_t(dim) * dim);
+  cdouble* dM[kMaxDeriv];
+  for (int s = 0; s < f->nderiv; ++s) {
+    f->dM[s].assign(size_t(dim) * dim, 0.0);
+    dM[s] = f->dM[s].data();
+  }
+  ParamBlockMatrices(c.gates, f->gates.data(), int(f->gates.size()), f->bits, f->k, f->nderiv,
+                     dgate, dU, M.data(), dM);
+  f->M.swap(M);
 }
 
 // The frontier fuser of FuseGates with parameterised gates inside blocks:
@@ -820,8 +857,6 @@ void RowGates(const Circuit& tmpl, const float* values, std::vector<Gate>* gates
 // Rewrite the matrices of symbolic ops for one row (others keep `base`'s).
 void RebindRow(const std::vector<Gate>& gs, const SharedRecipe& rec, bool tfq_fd, cfloat* mats) {
   cdouble M[256];
-  std::vector<Gate> loc;
-  std::vector<int> idx;
   for (const auto& r : rec.ops) {
     if (!r.symbolic) continue;
     const int dim = 1 << r.k;
@@ -833,21 +868,17 @@ void RebindRow(const std::vector<Gate>& gs, const SharedRecipe& rec, bool tfq_fd
       continue;
     }
     if (r.kind == 3) {
-      BlockMatrix(gs, gl, ng, r.bits, r.k, M);
-      WriteMatrix(mats + r.mat, M, false, dim);
-      loc.assign(size_t(ng), Gate{});
-      idx.resize(size_t(ng));
-      for (int j = 0; j < ng; ++j) {
-        loc[size_t(j)] = gs[size_t(gl[j])];
-        idx[size_t(j)] = j;
-      }
+      int dgate[kMaxDeriv];
+      cdouble dU[kMaxDeriv][16], D[kMaxDeriv][16];
+      cdouble* dM[kMaxDeriv] = {D[0], D[1]};
       for (int d = 0; d < r.nderiv; ++d) {
-        Gate& g = loc[size_t(r.dgate[d])];
-        GateDerivative(g.kind, g.params, g.sym_pidx[r.dsym[d]], tfq_fd, g.U);
-        BlockMatrix(loc, idx.data(), ng, r.bits, r.k, M);
-        WriteMatrix(mats + r.dmat[d], M, false, dim);
-        g = gs[size_t(gl[r.dgate[d]])];
+        const Gate& g = gs[size_t(gl[r.dgate[d]])];
+        dgate[d] = r.dgate[d];
+        GateDerivative(g.kind, g.params, g.sym_pidx[r.dsym[d]], tfq_fd, dU[d]);
       }
+      ParamBlockMatrices(gs, gl, ng, r.bits, r.k, r.nderiv, dgate, dU, M, dM);
+      WriteMatrix(mats + r.mat, M, false, dim);
+      for (int d = 0; d < r.nderiv; ++d) WriteMatrix(mats + r.dmat[d], D[d], false, dim);
       continue;
     }
     const Gate& g = gs[gl[0]];

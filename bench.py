@@ -882,7 +882,8 @@ def main():
     roof = {"bound": "mfma", "achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": round(tflops / FP32_PEAK_TFLOPS, 4),
             "traffic": measured_traffic(B, flops),
-            "kernel": "batch_kernel<adjoint> (LDS-resident persistent queue, FP32 VALU)",
+            "kernel": "batch_kernel<adjoint> (persistent queue over LDS-resident, tiled and "
+                      "wave-mode circuits; FP32 MFMA and VALU)",
             "flops_per_launch": flops, "ms_per_launch": round(per_launch * 1e3, 4),
             "peak_note": "MI355X FP32 peak (vector and matrix: 157.3 TFLOP/s)",
             # The two-qubit products that run on v_mfma_f32_4x4x1f32: their share of

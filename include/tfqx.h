@@ -215,6 +215,22 @@ int tfqx_plan_rebind(tfqx_plan* plan, const float* symbol_values, char* err, int
 int tfqx_plan_rebind_downstream(tfqx_plan* plan, const float* downstream_grads, char* err,
                                 int64_t err_len);
 
+/* New symbol values [batch, num_symbols] evaluated on the device: the values
+ * are kept with the plan, and the next tfqx_plan_upload or
+ * tfqx_plan_upload_changed copies them (B x S floats) and enqueues one kernel
+ * that rebuilds the matrices of every op containing symbolic gates in the
+ * workspace, ahead of tfqx_plan_launch on the same stream.  The host matrices
+ * are not recomputed, so the upload carries the values instead of the
+ * matrices.  Results equal tfqx_plan_rebind's to float rounding (the same
+ * double-precision recipes; only the device's sin/cos may differ in the last
+ * bit).  The first call builds the device tables (gates and recipes of the
+ * symbolic ops), which the next upload copies once.  Same plans as
+ * tfqx_plan_rebind (streaming and noisy plans return TFQX_INVALID_ARGUMENT).
+ * No reference counterpart: the reference re-parses and re-evaluates every
+ * gate on the host every call (core/ops/parse_context.cc:60-92). */
+int tfqx_plan_stage_values(tfqx_plan* plan, const float* symbol_values, char* err,
+                           int64_t err_len);
+
 /* Fusion decisions of the planner for one program (host only): the fused ops
  * of the forward program at the given width (1..4).  ops_out[i][8] = {k,
  * bit0..bit3 (global bit positions, ascending, -1 unused), 1 if the op keeps
@@ -306,6 +322,15 @@ int64_t tfqx_plan_output_bytes(const tfqx_plan* plan);
 
 /* Copy the device program into the workspace (host -> device, on `stream`). */
 int tfqx_plan_upload(tfqx_plan* plan, void* workspace, void* stream, char* err, int64_t err_len);
+/* Bring a workspace that holds this plan's last upload up to date, copying
+ * only what changed since: matrices after tfqx_plan_rebind, Pauli weights
+ * after tfqx_plan_rebind_downstream, staged values after
+ * tfqx_plan_stage_values (then evaluated on the device).  When `workspace` is
+ * not the one the plan was last uploaded to, this is tfqx_plan_upload.  The
+ * caller keeps the workspace to this plan between the two uploads: another
+ * plan's upload into it, or any other write, requires tfqx_plan_upload. */
+int tfqx_plan_upload_changed(tfqx_plan* plan, void* workspace, void* stream, char* err,
+                             int64_t err_len);
 /* Run the plan's kernels on an uploaded workspace; writes `output`. */
 int tfqx_plan_launch(tfqx_plan* plan, void* workspace, void* output, void* stream,
                      char* err, int64_t err_len);

@@ -78,6 +78,10 @@ def _declare(lib):
     lib.tfqx_plan_rebind.restype = c.c_int
     lib.tfqx_plan_rebind_downstream.argtypes = [P, P, c.c_char_p, i64]
     lib.tfqx_plan_rebind_downstream.restype = c.c_int
+    lib.tfqx_plan_stage_values.argtypes = [P, P, c.c_char_p, i64]
+    lib.tfqx_plan_stage_values.restype = c.c_int
+    lib.tfqx_plan_upload_changed.argtypes = [P, P, P, c.c_char_p, i64]
+    lib.tfqx_plan_upload_changed.restype = c.c_int
     lib.tfqx_plan_stats.argtypes = [P, P, i64]
     lib.tfqx_plan_stats.restype = c.c_int
     for name in ("tfqx_plan_simulate_expectation", "tfqx_plan_simulate_sampled_expectation",
@@ -147,7 +151,7 @@ EXPORTED_SYMBOLS = (
     "tfqx_strings_data", "tfqx_strings_offsets", "tfqx_strings_dim", "tfqx_strings_destroy",
     "tfqx_plan_inner_product",
     "tfqx_plan_inner_product_grad", "tfqx_plan_adjoint_gradient", "tfqx_plan_expectation_and_gradient", "tfqx_plan_rebind",
-    "tfqx_plan_rebind_downstream",
+    "tfqx_plan_rebind_downstream", "tfqx_plan_stage_values", "tfqx_plan_upload_changed",
     "tfqx_debug_fuse", "tfqx_debug_philox_uniform",
     "tfqx_plan_stats",
     "tfqx_plan_max_num_qubits", "tfqx_plan_workspace_bytes",
@@ -200,6 +204,9 @@ class Plan:
         rc = builder(*args, ctypes.byref(self.handle), err, 4096)
         check(rc, err)
         self.workspace = None
+        # The workspace holds this plan's last upload: later uploads copy only
+        # what changed since (tfqx_plan_upload_changed).
+        self._resident = False
 
     @property
     def max_num_qubits(self):
@@ -230,6 +237,15 @@ class Plan:
         check(self._lib.tfqx_plan_rebind(self.handle, v.ctypes.data if v.size else None, err, 4096),
               err)
 
+    def stage_values(self, values):
+        """New symbol values (float32 [batch, n_symbols]) for a batch-kernel plan,
+        evaluated into the plan's matrices on the device at the next upload
+        (``rebind_kernel``); the host matrices are not recomputed."""
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        err = ctypes.create_string_buffer(4096)
+        check(self._lib.tfqx_plan_stage_values(self.handle, v.ctypes.data if v.size else None,
+                                               err, 4096), err)
+
     def rebind_downstream(self, grads):
         """New downstream gradients (float32 [batch, n_ops]) for an adjoint plan of
         the batch kernel; upload again before launching."""
@@ -247,9 +263,12 @@ class Plan:
         nbytes = max(256, self.workspace_bytes)
         if self.workspace is None or self.workspace.numel() < nbytes:
             self.workspace = torch.empty(nbytes, dtype=torch.uint8, device=device)
+            self._resident = False
         err = ctypes.create_string_buffer(4096)
-        check(self._lib.tfqx_plan_upload(self.handle, ctypes.c_void_p(self.workspace.data_ptr()),
-                                         ctypes.c_void_p(stream), err, 4096), err)
+        fn = self._lib.tfqx_plan_upload_changed if self._resident else self._lib.tfqx_plan_upload
+        check(fn(self.handle, ctypes.c_void_p(self.workspace.data_ptr()), ctypes.c_void_p(stream),
+                 err, 4096), err)
+        self._resident = True
 
     def launch(self, out, stream, plain=False):
         err = ctypes.create_string_buffer(4096)

@@ -22,7 +22,7 @@ HOST_SOURCES = ["wire.cc", "textproto.cc", "circuit.cc", "plan.cc", "large.cc", 
 # parallel (each instantiates the register engine many times).
 HIP_SOURCES = ["batch_mode_adjoint.hip", "batch_mode_expectation.hip", "batch_mode_other.hip",
                "batch_mode_ipgrad.hip", "batch_kernels.hip", "large_kernels.hip",
-               "surrogate_kernels.hip", "noisy_kernels.hip", "noalloc.hip"]
+               "surrogate_kernels.hip", "noisy_kernels.hip", "noalloc.hip", "rebind_kernels.hip"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
           # Packed FP32 (v_pk_*_f32) stays off for every device code object.

@@ -94,6 +94,21 @@ struct tfqx_plan {
          off_terms = 0, off_ident = 0, off_nsamp = 0, off_passes = 0, off_gacc = 0,
          off_scratch = 0, total = 0;
   std::vector<uint8_t> staging;
+  // Device re-evaluation of the symbolic matrices (tfqx_plan_stage_values):
+  // a region after the scratch holding the staged symbol values and the
+  // rebind tables, whose host image rb_image is built on first use.
+  RebindTableSizes rb_sizes;
+  size_t off_rb = 0, rb_off_eigen = 0, rb_off_rows = 0, rb_off_ops = 0, rb_off_gidx = 0,
+         rb_off_gates = 0, rb_bytes = 0;
+  std::vector<uint8_t> rb_image;
+  int rb_kmax = 1;
+  bool rb_built = false;
+  // Upload bookkeeping: the workspace of the last upload and what the host
+  // changed since (tfqx_plan_upload_changed copies only that).
+  void* last_ws = nullptr;
+  bool mats_dirty = false, terms_dirty = false;
+  bool rb_tables_dirty = false, values_staged = false;
+  bool device_mats = false;  // the device holds matrices newer than staging's
   int lds_log2 = 0;
   int kmax = 2;
   int grid = 0;
@@ -266,6 +281,20 @@ void Layout(tfqx_plan* p) {
   // exceeds the number of circuits (tfqx_plan_launch, tfqx_graph_capture).
   const size_t tiled_wgs = std::min<size_t>(kTiledGridCap, std::max<size_t>(1, g.circuits.size()));
   if (g.any_tiled) off = Align(off + tiled_wgs * 2 * (size_t(1) << p->max_nq) * sizeof(float2));
+  if (p->shared || p->rebindable) {
+    p->rb_sizes = p->shared ? SizeSharedRebindTables(p->tmpl, p->recipe)
+                            : SizeRebindTables(p->rows_rebind);
+    size_t r = 0;
+    r = Align(r + size_t(p->B) * size_t(p->S) * sizeof(float));
+    p->rb_off_eigen = r;  r = Align(r + kRbEigenKinds * sizeof(RbEigen));
+    p->rb_off_rows = r;   r = Align(r + size_t(p->B) * sizeof(RbRow));
+    p->rb_off_ops = r;    r = Align(r + p->rb_sizes.ops * sizeof(RbOp));
+    p->rb_off_gidx = r;   r = Align(r + p->rb_sizes.gidx * sizeof(int32_t));
+    p->rb_off_gates = r;  r = Align(r + p->rb_sizes.gates * sizeof(RbGate));
+    p->rb_bytes = r;
+    p->off_rb = off;
+    off += r;
+  }
   p->total = off;
   p->staging.assign(image, 0);
   auto put = [&](size_t o, const void* src, size_t n) {
@@ -373,6 +402,7 @@ int BuildPlan(int mode, Inputs& in, int64_t B, int64_t S, int64_t O, int64_t K,
 }
 
 hipStream_t AsStream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
 
 // Real floating-point operations one launch of a batch program performs, by
 // the op's algorithm (not the instruction count): a dense k-qubit op is a
@@ -579,6 +609,29 @@ BatchArgs MakeArgs(tfqx_plan* p, void* ws, void* output) {
 int CheckHip(hipError_t e, char* err, int64_t len, const char* what) {
   if (e == hipSuccess) return TFQX_OK;
   return Fail(err, len, TFQX_DEVICE_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Re-evaluate the plan's symbolic matrices in `ws` from the staged values
+// already copied there (rebind_kernel, one workgroup per row).
+int RunDeviceRebind(tfqx_plan* p, void* ws, hipStream_t s, char* err, int64_t len) {
+  p->device_mats = true;
+  if (p->rb_sizes.ops == 0) return TFQX_OK;
+  uint8_t* base = static_cast<uint8_t*>(ws);
+  const uint8_t* rb = base + p->off_rb;
+  RebindArgs a;
+  a.values = reinterpret_cast<const float*>(rb);
+  a.eigen = reinterpret_cast<const RbEigen*>(rb + p->rb_off_eigen);
+  a.rows = reinterpret_cast<const RbRow*>(rb + p->rb_off_rows);
+  a.ops = reinterpret_cast<const RbOp*>(rb + p->rb_off_ops);
+  a.gidx = reinterpret_cast<const int32_t*>(rb + p->rb_off_gidx);
+  a.gates = reinterpret_cast<const RbGate*>(rb + p->rb_off_gates);
+  a.mats = reinterpret_cast<float2*>(base + p->off_mats);
+  a.num_rows = int(p->B);
+  a.num_ops = int(p->rb_sizes.ops);
+  a.shared = p->shared ? 1 : 0;
+  a.num_symbols = int(p->S);
+  a.tfq_fd = p->tfq_fd ? 1 : 0;
+  return CheckHip(LaunchRebind(a, p->rb_kmax, s), err, len, "rebind kernel launch");
 }
 
 }  // namespace
@@ -852,6 +905,45 @@ int tfqx_debug_philox_uniform(uint64_t seed, uint64_t start, int64_t count, uint
   return rc ? TFQX_DEVICE_ERROR : TFQX_OK;
 }
 
+int tfqx_plan_stage_values(tfqx_plan* p, const float* values, char* err, int64_t len) {
+  if (!p || (!values && p->B * p->S > 0))
+    return Fail(err, len, TFQX_INVALID_ARGUMENT, "null argument");
+  if (!p->shared && !p->rebindable)
+    return Fail(err, len, TFQX_INVALID_ARGUMENT,
+                "Plan cannot be rebound: it runs on the HBM-streaming path or is a noisy plan.");
+  if (!p->rb_built) {
+    RebindTables t;
+    const bool ok = p->shared ? BuildSharedRebindTables(p->tmpl, p->recipe, p->prog, &t)
+                              : BuildRebindTables(p->rows_rebind, p->prog, &t);
+    if (!ok)
+      return Fail(err, len, TFQX_INVALID_ARGUMENT,
+                  "Plan cannot be rebound on the device: a parameterised block is wider than "
+                  "two qubits; use tfqx_plan_rebind.");
+    if (t.ops.size() != p->rb_sizes.ops || t.gidx.size() != p->rb_sizes.gidx ||
+        t.gates.size() != p->rb_sizes.gates || t.rows.size() != size_t(p->B))
+      return Fail(err, len, TFQX_INTERNAL, "rebind table sizes differ from the layout");
+    p->rb_image.assign(p->rb_bytes, 0);
+    uint8_t* img = p->rb_image.data();
+    double eig[kRbEigenKinds][100];
+    EigenTableForDevice(eig);
+    memcpy(img + p->rb_off_eigen, eig, sizeof(eig));
+    auto put = [&](size_t o, const void* src, size_t n) {
+      if (n) memcpy(img + o, src, n);
+    };
+    put(p->rb_off_rows, t.rows.data(), t.rows.size() * sizeof(RbRow));
+    put(p->rb_off_ops, t.ops.data(), t.ops.size() * sizeof(RbOp));
+    put(p->rb_off_gidx, t.gidx.data(), t.gidx.size() * sizeof(int32_t));
+    put(p->rb_off_gates, t.gates.data(), t.gates.size() * sizeof(RbGate));
+    p->rb_kmax = t.kmax;
+    p->rb_built = true;
+    p->rb_tables_dirty = true;
+  }
+  if (p->B * p->S > 0)
+    memcpy(p->rb_image.data(), values, size_t(p->B) * size_t(p->S) * sizeof(float));
+  p->values_staged = true;
+  return TFQX_OK;
+}
+
 int tfqx_plan_rebind(tfqx_plan* p, const float* values, char* err, int64_t len) {
   if (!p || (!values && p->S > 0)) return Fail(err, len, TFQX_INVALID_ARGUMENT, "null argument");
   // The matrices are written straight into the upload image (prog.mats keeps
@@ -866,6 +958,9 @@ int tfqx_plan_rebind(tfqx_plan* p, const float* values, char* err, int64_t len) 
     return Fail(err, len, TFQX_INVALID_ARGUMENT,
                 "Plan cannot be rebound: it runs on the HBM-streaming path or is a noisy plan.");
   }
+  p->mats_dirty = true;
+  p->device_mats = false;
+  p->values_staged = false;
   return TFQX_OK;
 }
 
@@ -897,6 +992,7 @@ int tfqx_plan_rebind_downstream(tfqx_plan* p, const float* downstream, char* err
       memcpy(p->staging.data() + p->off_terms + size_t(k) * sizeof(DevTerm), &t, sizeof(DevTerm));
     }
   }
+  p->terms_dirty = true;
   return TFQX_OK;
 }
 
@@ -1003,9 +1099,56 @@ int tfqx_plan_upload(tfqx_plan* p, void* ws, void* stream, char* err, int64_t le
   if (p->large) return LargeUpload(p->large_plan.get(), ws, AsStream(stream), err, len);
   if (p->noisy) return NoisyUpload(p->noisy_plan.get(), ws, AsStream(stream), err, len);
   if (p->staging.empty()) return TFQX_OK;
-  return CheckHip(hipMemcpyAsync(ws, p->staging.data(), p->staging.size(), hipMemcpyHostToDevice,
-                                 AsStream(stream)),
+  const hipStream_t s = AsStream(stream);
+  int rc = CheckHip(hipMemcpyAsync(ws, p->staging.data(), p->staging.size(), hipMemcpyHostToDevice, s),
+                    err, len, "hipMemcpyAsync");
+  if (rc) return rc;
+  uint8_t* base = static_cast<uint8_t*>(ws);
+  if (p->rb_built) {
+    rc = CheckHip(hipMemcpyAsync(base + p->off_rb, p->rb_image.data(), p->rb_image.size(),
+                                 hipMemcpyHostToDevice, s),
                   err, len, "hipMemcpyAsync");
+    if (rc) return rc;
+    p->rb_tables_dirty = false;
+    // The image's matrices predate values evaluated (or staged) for the device.
+    if (p->values_staged || p->device_mats) {
+      rc = RunDeviceRebind(p, ws, s, err, len);
+      if (rc) return rc;
+    }
+  }
+  p->last_ws = ws;
+  p->mats_dirty = p->terms_dirty = p->values_staged = false;
+  return TFQX_OK;
+}
+
+int tfqx_plan_upload_changed(tfqx_plan* p, void* ws, void* stream, char* err, int64_t len) {
+  if (!p || (!ws && p->total)) return Fail(err, len, TFQX_INVALID_ARGUMENT, "null plan or workspace");
+  if (p->large || p->noisy || p->staging.empty() || ws != p->last_ws)
+    return tfqx_plan_upload(p, ws, stream, err, len);
+  const hipStream_t s = AsStream(stream);
+  uint8_t* base = static_cast<uint8_t*>(ws);
+  auto copy = [&](size_t dst, const uint8_t* src, size_t n) -> int {
+    if (!n) return TFQX_OK;
+    return CheckHip(hipMemcpyAsync(base + dst, src, n, hipMemcpyHostToDevice, s), err, len,
+                    "hipMemcpyAsync");
+  };
+  int rc = TFQX_OK;
+  if (p->mats_dirty)
+    rc = copy(p->off_mats, p->staging.data() + p->off_mats, p->prog.mats.size() * sizeof(cfloat));
+  if (!rc && p->terms_dirty)
+    rc = copy(p->off_terms, p->staging.data() + p->off_terms,
+              p->prog.terms.size() * sizeof(DevTerm));
+  if (!rc && p->rb_built) {
+    if (p->rb_tables_dirty)
+      rc = copy(p->off_rb, p->rb_image.data(), p->rb_image.size());
+    else if (p->values_staged)
+      rc = copy(p->off_rb, p->rb_image.data(), size_t(p->B) * size_t(p->S) * sizeof(float));
+    if (!rc) p->rb_tables_dirty = false;
+    if (!rc && p->values_staged) rc = RunDeviceRebind(p, ws, s, err, len);
+  }
+  if (rc) return rc;
+  p->mats_dirty = p->terms_dirty = p->values_staged = false;
+  return TFQX_OK;
 }
 
 int tfqx_plan_launch(tfqx_plan* p, void* ws, void* output, void* stream, char* err, int64_t len) {

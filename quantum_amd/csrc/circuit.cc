@@ -396,6 +396,21 @@ Status CollectQubits(const Program& prog, std::vector<std::string>* sorted_ids) 
 
 }  // namespace
 
+void EigenTableForDevice(double (*out)[100]) {
+  for (int kind = 0; kind <= int(kGateISwap); ++kind) {
+    const Eigen& e = EigenOf(GateKind(kind));
+    double* o = out[kind];
+    for (int i = 0; i < 100; ++i) o[i] = 0;
+    o[0] = e.count;
+    for (int k = 0; k < 3; ++k) o[1 + k] = e.lam[k];
+    for (int k = 0; k < 3; ++k)
+      for (int i = 0; i < 16; ++i) {
+        o[4 + 32 * k + 2 * i] = e.P[k][i].real();
+        o[4 + 32 * k + 2 * i + 1] = e.P[k][i].imag();
+      }
+  }
+}
+
 int GateTargets(GateKind kind) {
   switch (kind) {
     case kGateI1: case kGateH: case kGateX: case kGateY: case kGateZ: case kGatePhasedX:

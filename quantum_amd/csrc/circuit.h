@@ -111,4 +111,8 @@ void GateUnitary(GateKind kind, const float* params, cdouble* U);
 void GateDerivative(GateKind kind, const float* params, int pidx, bool tfq_fd, cdouble* dU);
 int GateTargets(GateKind kind);
 
+// The eigen components of gate kinds 0..12 (I1 .. ISWAP) for the device
+// evaluator: out[kind] = {count, lam[3], P[3] as (re, im) row-major}.
+void EigenTableForDevice(double (*out)[100]);
+
 }  // namespace tfqx

@@ -1079,6 +1079,126 @@ void RebindBatchProgram(const std::vector<RowRebind>& rows, const float* values,
   });
 }
 
+namespace {
+
+RbGate ToRbGate(GateKind kind, int nt, int nsym, const int* sym_pidx, const int* sym_col,
+                const int* bits, uint64_t cmask, uint64_t cval, const float* params) {
+  RbGate g{};
+  g.kind = int32_t(kind);
+  g.nt = nt;
+  g.nsym = nsym;
+  for (int j = 0; j < 2; ++j) {
+    g.sym_pidx[j] = j < nsym ? sym_pidx[j] : 0;
+    g.sym_col[j] = j < nsym ? sym_col[j] : 0;
+    g.bits[j] = j < nt ? bits[j] : 0;
+  }
+  g.cmask = cmask;
+  g.cval = cval;
+  for (int j = 0; j < 5; ++j) g.params[j] = params[j];
+  return g;
+}
+
+bool ToRbOp(const OpRecipe& r, RbOp* o) {
+  if (r.kind == 3 && r.k > 2) return false;
+  *o = RbOp{};
+  o->kind = r.kind;
+  o->k = r.k;
+  o->diag = r.diag;
+  o->nderiv = r.nderiv;
+  for (int j = 0; j < r.k; ++j) o->bits[j] = r.bits[j];
+  o->gate_begin = r.gate_begin;
+  o->gate_end = r.gate_end;
+  o->mat = r.mat;
+  for (int d = 0; d < r.nderiv; ++d) {
+    o->dmat[d] = r.dmat[d];
+    o->dgate[d] = r.dgate[d];
+    o->dsym[d] = r.dsym[d];
+  }
+  return true;
+}
+
+}  // namespace
+
+RebindTableSizes SizeRebindTables(const std::vector<RowRebind>& rows) {
+  RebindTableSizes z;
+  for (const RowRebind& rr : rows) {
+    z.gates += rr.gates.size();
+    z.gidx += rr.recipe.gates.size();
+    z.ops += rr.recipe.ops.size();
+  }
+  return z;
+}
+
+RebindTableSizes SizeSharedRebindTables(const Circuit& tmpl, const SharedRecipe& rec) {
+  RebindTableSizes z;
+  z.gates = tmpl.gates.size();
+  z.gidx = rec.gates.size();
+  for (const OpRecipe& r : rec.ops) z.ops += r.symbolic;
+  return z;
+}
+
+bool BuildRebindTables(const std::vector<RowRebind>& rows, const BatchProgram& prog,
+                       RebindTables* t) {
+  *t = RebindTables();
+  const RebindTableSizes z = SizeRebindTables(rows);
+  t->gates.reserve(z.gates);
+  t->gidx.reserve(z.gidx);
+  t->ops.reserve(z.ops);
+  t->rows.resize(rows.size());
+  for (size_t i = 0; i < rows.size(); ++i) {
+    const RowRebind& rr = rows[i];
+    RbRow& row = t->rows[i];
+    row = RbRow{};
+    row.op_begin = int32_t(t->ops.size());
+    row.gidx_base = int32_t(t->gidx.size());
+    row.gate_base = int32_t(t->gates.size());
+    row.mat_base = uint32_t(prog.row_mat_base[i]);
+    row.vrow = int32_t(i);
+    for (const RebindGate& g : rr.gates) {
+      const int pidx[2] = {g.sym_pidx[0], g.sym_pidx[1]};
+      t->gates.push_back(ToRbGate(g.kind, g.nt, g.nsym, pidx, g.sym_col, g.bits, g.cmask,
+                                  g.cval, g.params));
+    }
+    t->gidx.insert(t->gidx.end(), rr.recipe.gates.begin(), rr.recipe.gates.end());
+    for (const OpRecipe& r : rr.recipe.ops) {
+      if (!r.symbolic) continue;
+      RbOp o;
+      if (!ToRbOp(r, &o)) return false;
+      o.row = int32_t(i);
+      t->kmax = std::max(t->kmax, int(r.k));
+      t->ops.push_back(o);
+    }
+    row.op_end = int32_t(t->ops.size());
+  }
+  return true;
+}
+
+bool BuildSharedRebindTables(const Circuit& tmpl, const SharedRecipe& rec,
+                             const BatchProgram& prog, RebindTables* t) {
+  *t = RebindTables();
+  for (const Gate& g : tmpl.gates)
+    t->gates.push_back(ToRbGate(g.kind, g.nt, g.nsym, g.sym_pidx, g.sym_col, g.bits, g.cmask,
+                                g.cval, g.params));
+  t->gidx.assign(rec.gates.begin(), rec.gates.end());
+  for (const OpRecipe& r : rec.ops) {
+    if (!r.symbolic) continue;
+    RbOp o;
+    if (!ToRbOp(r, &o)) return false;
+    t->kmax = std::max(t->kmax, int(r.k));
+    t->ops.push_back(o);
+  }
+  t->rows.resize(size_t(prog.num_rows));
+  for (int i = 0; i < prog.num_rows; ++i) {
+    RbRow& row = t->rows[size_t(i)];
+    row = RbRow{};
+    row.op_begin = 0;
+    row.op_end = int32_t(t->ops.size());
+    row.mat_base = uint32_t(prog.row_mat_base[size_t(i)]);
+    row.vrow = i;
+  }
+  return true;
+}
+
 void BuildSharedBatchProgram(const Circuit& tmpl, const std::vector<CircuitJob>& jobs,
                              const float* values, int num_symbols, int num_slots,
                              const PlanOptions& opt_in, int threads, BatchProgram* out,

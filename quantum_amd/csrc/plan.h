@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "circuit.h"
+#include "rebind_device.h"
 #include "tfqx_internal.h"
 
 namespace tfqx {
@@ -195,6 +196,28 @@ RowRebind CompactRecipe(const Circuit& c, const SharedRecipe& rec);
 // image's copy of it).
 void RebindBatchProgram(const std::vector<RowRebind>& rows, const float* values, int num_symbols,
                         bool tfq_fd, int threads, const BatchProgram& prog, cfloat* mats);
+
+// Flat tables for re-evaluating the symbolic matrices on the device
+// (rebind_device.h).  Per-row plans: each row's compacted gates and symbolic
+// recipes; shared plans: the template's gates and recipe, used by every row.
+struct RebindTables {
+  std::vector<RbGate> gates;
+  std::vector<int32_t> gidx;
+  std::vector<RbOp> ops;
+  std::vector<RbRow> rows;
+  int kmax = 1;
+};
+struct RebindTableSizes {
+  size_t gates = 0, gidx = 0, ops = 0;
+};
+RebindTableSizes SizeRebindTables(const std::vector<RowRebind>& rows);
+RebindTableSizes SizeSharedRebindTables(const Circuit& tmpl, const SharedRecipe& rec);
+// false if a recipe falls outside what the device evaluates (a parameterised
+// block wider than two qubits).
+bool BuildRebindTables(const std::vector<RowRebind>& rows, const BatchProgram& prog,
+                       RebindTables* t);
+bool BuildSharedRebindTables(const Circuit& tmpl, const SharedRecipe& rec,
+                             const BatchProgram& prog, RebindTables* t);
 
 // Rows whose programs are byte-identical: `tmpl` is the program resolved with
 // row 0's values; the op list is built once and each row re-evaluates only the

@@ -235,7 +235,10 @@ class ExactEnergy:
         thetas = np.ascontiguousarray(thetas, dtype=np.float32)
         n = thetas.shape[0]
         plan, out, dev = self._plan(n)
-        plan.rebind(thetas)
+        if n >= 64:
+            plan.stage_values(thetas)  # matrices evaluated on the device at upload
+        else:
+            plan.rebind(thetas)
         stream = torch.cuda.current_stream(dev).cuda_stream
         plan.upload(dev, stream)
         plan.launch(out, stream)

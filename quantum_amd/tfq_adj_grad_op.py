@@ -48,8 +48,9 @@ def plan_adj_grad(programs, symbol_names, symbol_values, pauli_sums, prev_grad,
 def _cached(programs, symbol_names, symbol_values, pauli_sums, prev_grad, derivative,
             with_expectation):
     """Launch an adjoint plan, reusing a recent call's plan when the programs,
-    names and Pauli sums are the same (new values and downstream weights are
-    rebound; see ``plan_cache``)."""
+    names and Pauli sums are the same (see ``plan_cache``): the new values are
+    staged and evaluated into the matrices on the device, the downstream
+    weights rewritten, and only those are uploaded."""
     from . import plan_cache
     from .tfq_simulate_ops import _device_and_stream
     programs, names, values = _common(programs, symbol_names, symbol_values)
@@ -66,7 +67,8 @@ def _cached(programs, symbol_names, symbol_values, pauli_sums, prev_grad, deriva
         return plan
 
     def rebind(plan):
-        plan.rebind(values)
+        # The symbolic matrices are re-evaluated on the device at upload.
+        plan.stage_values(values)
         plan.rebind_downstream(grads)
 
     def launch(plan):

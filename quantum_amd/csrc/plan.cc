@@ -4,6 +4,7 @@
 #include "large.h"
 
 #include <algorithm>
+#include <tuple>
 #include <atomic>
 #include <cmath>
 #include <numeric>
@@ -1170,6 +1171,15 @@ bool BuildRebindTables(const std::vector<RowRebind>& rows, const BatchProgram& p
     }
     row.op_end = int32_t(t->ops.size());
   }
+  // Neighbouring threads of rebind_kernel take neighbouring ops: group ops of
+  // one shape (width, kind, derivative passes, gate count) so a wavefront's
+  // threads follow the same path.  Each op names its row, so order is free.
+  std::stable_sort(t->ops.begin(), t->ops.end(), [](const RbOp& x, const RbOp& y) {
+    const auto key = [](const RbOp& o) {
+      return std::make_tuple(o.k, o.kind, o.nderiv, o.gate_end - o.gate_begin);
+    };
+    return key(x) < key(y);
+  });
   return true;
 }
 

@@ -42,7 +42,8 @@ struct RbOp {
   int32_t pad;
 };
 
-// One batch row: its ops, gate tables and matrix base in the plan's array.
+// One batch row: its gate tables and matrix base in the plan's array (and,
+// for shared tables, the op range every row evaluates).
 struct RbRow {
   int32_t op_begin, op_end;
   int32_t gidx_base, gate_base;

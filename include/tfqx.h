@@ -342,8 +342,9 @@ int tfqx_plan_launch_plain(tfqx_plan* plan, void* workspace, void* output, void*
  * of a caller buffer: allocates nothing.  No counterpart in the reference: it
  * is for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  Plans
- * whose workspace exceeds 32 MiB, or that take the HBM-streaming path, are
- * rejected with TFQX_INVALID_ARGUMENT.  Every call on one GPU shares that
+ * whose workspace exceeds 32 MiB (not counting the region of
+ * tfqx_plan_stage_values until values are staged), or that take the
+ * HBM-streaming path, are rejected with TFQX_INVALID_ARGUMENT.  Every call on one GPU shares that
  * GPU's array: calls that overlap on one GPU (different streams or unordered
  * host threads) race. */
 int tfqx_plan_launch_noalloc(tfqx_plan* plan, void* output, void* stream, char* err,

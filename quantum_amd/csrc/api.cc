@@ -606,6 +606,18 @@ BatchArgs MakeArgs(tfqx_plan* p, void* ws, void* output) {
   return a;
 }
 
+}  // namespace
+
+// Workspace bytes a launch touches: the device-evaluation region only after
+// values have been staged (it sits at the end of the workspace).
+namespace tfqx {
+int64_t PlanLaunchBytes(const tfqx_plan* p) {
+  return int64_t(p->rb_built ? p->total : p->total - p->rb_bytes);
+}
+}  // namespace tfqx
+
+namespace {
+
 int CheckHip(hipError_t e, char* err, int64_t len, const char* what) {
   if (e == hipSuccess) return TFQX_OK;
   return Fail(err, len, TFQX_DEVICE_ERROR, std::string(what) + ": " + hipGetErrorString(e));

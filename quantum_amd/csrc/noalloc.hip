@@ -27,6 +27,10 @@
 
 #include "tfqx.h"
 
+namespace tfqx {
+int64_t PlanLaunchBytes(const tfqx_plan* p);  // api.cc
+}  // namespace tfqx
+
 namespace {
 
 constexpr size_t kNoAllocBytes = size_t(32) << 20;
@@ -80,7 +84,7 @@ int tfqx_plan_launch_noalloc(tfqx_plan* p, void* output, void* stream, char* err
     return Fail(err, err_len, TFQX_INVALID_ARGUMENT,
                 "tfqx_plan_launch_noalloc: plans on the HBM-streaming path (circuits wider than "
                 "the batch kernel takes) are not supported; use tfqx_plan_launch");
-  const int64_t need = tfqx_plan_workspace_bytes(p);
+  const int64_t need = tfqx::PlanLaunchBytes(p);
   if (need < 0 || size_t(need) > kNoAllocBytes)
     return Fail(err, err_len, TFQX_INVALID_ARGUMENT,
                 "tfqx_plan_launch_noalloc: the plan needs " + std::to_string(need) +

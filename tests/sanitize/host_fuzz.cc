@@ -227,6 +227,75 @@ void ExercisePlan(const Circuit& c) {
   }
 }
 
+// The device re-evaluation tables of one-row plans (rebind_kernel indexes
+// them without bounds checks): every op's gates, matrices and qubits must lie
+// inside what the plan holds; then the host re-evaluation runs on the same
+// recipes.
+uint64_t g_rebind_ops = 0;
+
+void CheckRebindTables(const RebindTables& t, const BatchProgram& prog, int nq) {
+  g_rebind_ops += t.ops.size();
+  for (const RbOp& o : t.ops) {
+    const RbRow& row = t.rows[size_t(o.row)];
+    const int dim = 1 << o.k;
+    if (o.k < 1 || o.k > kMaxFuse || (o.kind == 3 && o.k > 2)) abort();
+    for (int j = 0; j < o.k; ++j)
+      if (o.bits[j] < 0 || o.bits[j] >= nq) abort();
+    if (o.gate_begin < 0 || o.gate_end <= o.gate_begin ||
+        size_t(row.gidx_base + o.gate_end) > t.gidx.size())
+      abort();
+    for (int g = o.gate_begin; g < o.gate_end; ++g) {
+      const int gi = row.gate_base + t.gidx[size_t(row.gidx_base + g)];
+      if (gi < 0 || size_t(gi) >= t.gates.size()) abort();
+      const RbGate& rg = t.gates[size_t(gi)];
+      if (rg.nt < 1 || rg.nt > 2 || rg.nsym < 0 || rg.nsym > 2) abort();
+      for (int j = 0; j < rg.nsym; ++j)
+        if (rg.sym_pidx[j] < 0 || rg.sym_pidx[j] > 4 || rg.sym_col[j] < 0 || rg.sym_col[j] > 2) abort();
+    }
+    const size_t m = size_t(o.diag && o.kind != 3 ? dim : dim * dim);
+    if (row.mat_base + size_t(o.mat) + m > prog.mats.size()) abort();
+    for (int d = 0; d < o.nderiv; ++d) {
+      if (o.kind == 3 && (o.dgate[d] < 0 || o.dgate[d] >= o.gate_end - o.gate_begin)) abort();
+      if (row.mat_base + size_t(o.dmat[d]) + size_t(dim * dim) > prog.mats.size()) abort();
+    }
+  }
+}
+
+void ExerciseRebind(const Circuit& c, const float* vals) {
+  if (c.gates.empty() || c.num_qubits > 13) return;
+  for (int adj = 0; adj < 2; ++adj) {
+    PlanOptions opt;
+    opt.mode = adj ? kModeAdjoint : kModeExpectation;
+    opt.adjoint = adj != 0;
+    opt.fuse.max_fuse = 2;
+    opt.fuse.structural_diag = true;
+    opt.keep_all_lambda = adj != 0;
+    std::vector<CircuitJob> jobs(1);
+    jobs[0] = CircuitJob{&c, nullptr, nullptr, nullptr, 0};
+    BatchProgram prog;
+    std::vector<SharedRecipe> recipes;
+    BuildBatchProgram(jobs, 1, 3, opt, 1, &prog, &recipes);
+    std::vector<RowRebind> rows{CompactRecipe(c, recipes[0])};
+    RebindTables t;
+    if (BuildRebindTables(rows, prog, &t)) {
+      const RebindTableSizes z = SizeRebindTables(rows);
+      if (t.ops.size() != z.ops || t.gidx.size() != z.gidx || t.gates.size() != z.gates) abort();
+      CheckRebindTables(t, prog, c.num_qubits);
+    }
+    std::vector<cfloat> mats(prog.mats);
+    RebindBatchProgram(rows, vals, 3, adj != 0, 1, prog, mats.data());
+    BatchProgram sprog;
+    SharedRecipe srec;
+    BuildSharedBatchProgram(c, jobs, vals, 3, 1, opt, 1, &sprog, &srec);
+    RebindTables st;
+    if (BuildSharedRebindTables(c, srec, sprog, &st)) {
+      if (st.ops.size() != SizeSharedRebindTables(c, srec).ops) abort();
+      for (RbOp& o : st.ops) o.row = 0;
+      CheckRebindTables(st, sprog, c.num_qubits);
+    }
+  }
+}
+
 void OneProgram(const std::string& bytes, const std::vector<std::string>& sums, uint64_t* ok) {
   Program prog;
   if (!DecodeProgram(reinterpret_cast<const uint8_t*>(bytes.data()), bytes.size(), &prog)) return;
@@ -239,6 +308,7 @@ void OneProgram(const std::string& bytes, const std::vector<std::string>& sums, 
   if (BuildCircuit(prog, sm, &c).ok()) {
     ++*ok;
     ExercisePlan(c);
+    ExerciseRebind(c, vals);
     Circuit r;
     RebindCircuit(c, vals, &r);
     for (const auto& sb : sums) {
@@ -295,7 +365,8 @@ int main(int argc, char** argv) {
   }
   // The unmutated seeds must build (the fuzz reaches the deep paths).
   for (const auto& p : progs) OneProgram(p, sums, &built);
-  printf("host_fuzz: %ld iterations, %llu circuits built, %llu text programs parsed\n", iters,
-         (unsigned long long)built, (unsigned long long)text_ok);
+  printf("host_fuzz: %ld iterations, %llu circuits built, %llu text programs parsed, %llu "
+         "device rebind ops checked\n", iters, (unsigned long long)built,
+         (unsigned long long)text_ok, (unsigned long long)g_rebind_ops);
   return built > 0 ? 0 : 2;
 }

@@ -1,5 +1,7 @@
 """The host C++ (proto decoders, text-format parser, qubit resolution, gate
-building, fuser, pass scheduler, parameter-shift rewriting) under
+building, fuser, pass scheduler, rebind recipes and the device re-evaluation
+tables, whose indices are checked against the plan, parameter-shift
+rewriting) under
 AddressSanitizer and UndefinedBehaviorSanitizer, driven by a mutation fuzz of
 valid programs and Pauli sums (tests/sanitize/host_fuzz.cc).  The analogue of
 the reference's sanitizer run over core/src (scripts/msan_test.sh:16-35).

@@ -263,13 +263,13 @@ void CheckRebindTables(const RebindTables& t, const BatchProgram& prog, int nq) 
 
 void ExerciseRebind(const Circuit& c, const float* vals) {
   if (c.gates.empty() || c.num_qubits > 13) return;
-  for (int adj = 0; adj < 2; ++adj) {
+  for (int adj = 0; adj < 6; ++adj) {
     PlanOptions opt;
-    opt.mode = adj ? kModeAdjoint : kModeExpectation;
-    opt.adjoint = adj != 0;
-    opt.fuse.max_fuse = 2;
+    opt.mode = adj & 1 ? kModeAdjoint : kModeExpectation;
+    opt.adjoint = (adj & 1) != 0;
+    opt.fuse.max_fuse = 2 + adj / 2;  // widths 2..4
     opt.fuse.structural_diag = true;
-    opt.keep_all_lambda = adj != 0;
+    opt.keep_all_lambda = opt.adjoint;
     std::vector<CircuitJob> jobs(1);
     jobs[0] = CircuitJob{&c, nullptr, nullptr, nullptr, 0};
     BatchProgram prog;
@@ -283,7 +283,7 @@ void ExerciseRebind(const Circuit& c, const float* vals) {
       CheckRebindTables(t, prog, c.num_qubits);
     }
     std::vector<cfloat> mats(prog.mats);
-    RebindBatchProgram(rows, vals, 3, adj != 0, 1, prog, mats.data());
+    RebindBatchProgram(rows, vals, 3, opt.adjoint, 1, prog, mats.data());
     BatchProgram sprog;
     SharedRecipe srec;
     BuildSharedBatchProgram(c, jobs, vals, 3, 1, opt, 1, &sprog, &srec);

@@ -127,6 +127,38 @@ def test_device_values_match_host_values_expectation(wave):
         _lib.set_wave_min_circuits(-1)
 
 
+@pytest.mark.parametrize("width", [3, 4])
+def test_device_values_wide_fused_blocks(width):
+    """Fusion width 3 and 4: forward blocks of 3-4 qubits take the kernel's
+    general path (matrices in private memory); the reverse sweep's
+    parameterised blocks stay within two qubits."""
+    from quantum_amd import tfq_simulate_ops as sim
+    _lib.set_fusion_width(width)
+    try:
+        progs, names, ps, rng = _batch(91 + width, [5, 9, 12, 13, 7, 15])
+        B = len(progs)
+        v1, v2 = _values(rng, B, 5), _values(rng, B, 5)
+        g = rng.uniform(-1, 1, size=(B, 2)).astype(np.float32)
+        for kind in ("expectation", "adjoint"):
+            if kind == "expectation":
+                host, shape = sim.plan_expectation(progs, names, v1, ps)
+                dev, _ = sim.plan_expectation(progs, names, v1, ps)
+            else:
+                host, shape = _adjoint_plan(progs, names, v1, ps, g)
+                dev, _ = _adjoint_plan(progs, names, v1, ps, g)
+            _launch(dev, shape)
+            host.rebind(v2)
+            want = _launch(host, shape)
+            dev.stage_values(v2)
+            _say(f"device-evaluated matrices, fusion width {width}, {kind} plan")
+            got = _launch(dev, shape)
+            np.testing.assert_allclose(got, want, atol=MATCH)
+            if kind == "adjoint":
+                _check_oracle(got, progs, names, v2, ps, g, "analytic")
+    finally:
+        _lib.set_fusion_width(0)
+
+
 def test_device_values_shared_program():
     progs1, names, ps1, rng = _batch(59, [11], n_sym=4, p_symbol=0.8)
     B = 70

@@ -306,9 +306,9 @@ __device__ __forceinline__ void tiled_forward(float2* G, int n, const BatchArgs&
         return use_tab ? tile_index_tab(base, j, ps.nlow, tab) : tile_index(base, j, ps);
       };
       // Register engine: the tile goes straight between G and the registers.
-      auto io = tile_io(
-          [&](uint32_t x, cf& va, cf&) {
-            const uint64_t g = gidx(x);
+      auto io = tile_io_indexed(
+          gidx,
+          [&](uint64_t g, cf& va, cf&) {
             if (first) {
               va = cf{g == 0 ? 1.f : 0.f, 0.f};
             } else {
@@ -316,7 +316,7 @@ __device__ __forceinline__ void tiled_forward(float2* G, int n, const BatchArgs&
               va = cf{v.x, v.y};
             }
           },
-          [&](uint32_t x, const cf& va, const cf&) { G[gidx(x)] = make_float2(va.x, va.y); });
+          [&](uint64_t g, const cf& va, const cf&) { G[g] = make_float2(va.x, va.y); });
       if (reg_run_any<false>(A, A, ps.L, a.ops, ps.op_begin, ps.op_end, mc, NoAcc(), io)) continue;
       if (first) {
         for (uint32_t j = tid; j < T; j += kThreads) A[j] = make_float2(gidx(j) == 0 ? 1.f : 0.f, 0.f);
@@ -357,15 +357,14 @@ __device__ __forceinline__ void tiled_reverse(float2* G, float2* Lg, int n, cons
       auto gidx = [&](uint32_t j) -> uint64_t {
         return use_tab ? tile_index_tab(base, j, ps.nlow, tab) : tile_index(base, j, ps);
       };
-      auto io = tile_io(
-          [&](uint32_t x, cf& va, cf& vb) {
-            const uint64_t g = gidx(x);
+      auto io = tile_io_indexed(
+          gidx,
+          [&](uint64_t g, cf& va, cf& vb) {
             const float2 u = G[g], v = Lg[g];
             va = cf{u.x, u.y};
             vb = cf{v.x, v.y};
           },
-          [&](uint32_t x, const cf& va, const cf& vb) {
-            const uint64_t g = gidx(x);
+          [&](uint64_t g, const cf& va, const cf& vb) {
             G[g] = make_float2(va.x, va.y);
             Lg[g] = make_float2(vb.x, vb.y);
           });

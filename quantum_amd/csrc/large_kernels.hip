@@ -101,9 +101,9 @@ __global__ __launch_bounds__(kThreads, (TWO || WIDE) ? 2 : 4) void pass_kernel(P
       if (CPLX) atomicAdd(&pacc[kMaxPassDerivs + slot], v);
     };
     // Register engine: the tile goes straight between memory and registers.
-    auto io = tile_io(
-        [&](uint32_t x, cf& va, cf& vb) {
-          const uint64_t g = gidx(x);
+    auto io = tile_io_indexed(
+        gidx,
+        [&](uint64_t g, cf& va, cf& vb) {
           if (TWO) {
             const float2 u = a.psi[g], v = a.lam[g];
             va = cf{u.x, u.y};
@@ -115,8 +115,7 @@ __global__ __launch_bounds__(kThreads, (TWO || WIDE) ? 2 : 4) void pass_kernel(P
             va = cf{u.x, u.y};
           }
         },
-        [&](uint32_t x, const cf& va, const cf& vb) {
-          const uint64_t g = gidx(x);
+        [&](uint64_t g, const cf& va, const cf& vb) {
           if (TWO) {
             a.psi[g] = make_float2(va.x, va.y);
             a.lam[g] = make_float2(vb.x, vb.y);

@@ -822,15 +822,45 @@ __device__ __forceinline__ void esync() {
 // storing them with store(x, a, b), x the tile index of each register (b only
 // in the reverse sweep); LDS is then used only by the remaps in between, and
 // no barrier surrounds the run (a remap's own barriers order its LDS use).
+// With an index function (tile_io_indexed), load and store receive index(x)
+// instead of x.  The index must be OR-linear over disjoint bits of x (a
+// tile's bits deposited into the global index, plus the tile's base), so the
+// engine computes it once per thread (its own bits) and once per register
+// qubit, and each register's index is one OR of those.
 struct LdsIO {};
-template <typename LoadFn, typename StoreFn>
+struct IdentityIndex {
+  __device__ __forceinline__ uint64_t operator()(uint32_t x) const { return x; }
+};
+template <typename LoadFn, typename StoreFn, typename IndexFn = IdentityIndex>
 struct TileIO {
   LoadFn load;
   StoreFn store;
+  IndexFn index;
 };
 template <typename LoadFn, typename StoreFn>
 __device__ __forceinline__ TileIO<LoadFn, StoreFn> tile_io(LoadFn l, StoreFn s) {
-  return TileIO<LoadFn, StoreFn>{l, s};
+  return TileIO<LoadFn, StoreFn>{l, s, IdentityIndex{}};
+}
+template <typename IndexFn, typename LoadFn, typename StoreFn>
+__device__ __forceinline__ TileIO<LoadFn, StoreFn, IndexFn> tile_io_indexed(IndexFn i, LoadFn l,
+                                                                            StoreFn s) {
+  return TileIO<LoadFn, StoreFn, IndexFn>{l, s, i};
+}
+
+// index(tbase | rdep(m, r)) for every register r: the thread's part once, one
+// term per register qubit, then one OR per register (f(r, index)).
+template <int R, typename IO, typename F>
+__device__ __forceinline__ void for_reg_index(const RegMap<R>& m, IO& io, F&& f) {
+  uint64_t g[1 << R];
+  g[0] = io.index(m.tbase);
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const uint64_t gj = io.index(1u << m.rpos[j]);
+#pragma unroll
+    for (int r = 0; r < (1 << j); ++r) g[r | (1 << j)] = g[r] | gj;
+  }
+#pragma unroll
+  for (int r = 0; r < (1 << R); ++r) f(r, g[r]);
 }
 template <typename IO>
 struct IsLdsIO {
@@ -869,8 +899,7 @@ __device__ __forceinline__ void reg_run(float2* __restrict__ A, float2* __restri
       if constexpr (!kLds) {
         if (!have) {
           make_map<R, WAVE>(m, op.cmask, L);
-#pragma unroll
-          for (int r = 0; r < (1 << R); ++r) io.load(m.tbase | rdep(m, r), ra[r], rb[r]);
+          for_reg_index<R>(m, io, [&](int r, uint64_t g) { io.load(g, ra[r], rb[r]); });
           have = true;
           continue;
         }
@@ -929,8 +958,7 @@ __device__ __forceinline__ void reg_run(float2* __restrict__ A, float2* __restri
     if (REV) reg_store<R>(rb, B, m, false);
     esync<WAVE>();
   } else {
-#pragma unroll
-    for (int r = 0; r < (1 << R); ++r) io.store(m.tbase | rdep(m, r), ra[r], rb[r]);
+    for_reg_index<R>(m, io, [&](int r, uint64_t g) { io.store(g, ra[r], rb[r]); });
   }
 }
 

@@ -266,7 +266,7 @@ void Layout(tfqx_plan* p) {
   p->off_circuits = off; off = Align(off + g.circuits.size() * sizeof(DevCircuit));
   p->off_others = off;   off = Align(off + g.others.size() * sizeof(DevOther));
   p->off_ops = off;      off = Align(off + g.ops.size() * sizeof(DevOp));
-  p->off_mats = off;     off = Align(off + (g.mats.size() + kMatCache) * sizeof(cfloat));
+  p->off_mats = off;     off = Align(off + (g.mats.size() + kMatCacheWide) * sizeof(cfloat));
   p->off_terms = off;    off = Align(off + g.terms.size() * sizeof(DevTerm));
   p->off_ident = off;    off = Align(off + g.ident.size() * sizeof(float));
   p->off_nsamp = off;    off = Align(off + p->num_samples.size() * sizeof(int32_t));

@@ -395,8 +395,9 @@ __global__ __launch_bounds__(kThreads, BatchMinWaves(MODE)) void batch_kernel(Ba
   __shared__ double red[kWaves];
   __shared__ int s_ci;
   __shared__ uint64_t tab[kTileTable];
-  __shared__ float2 mcache[kMatCache];
-  MatCache mc = mat_cache(mcache, a.mats);
+  constexpr int kWindow = MODE == kModeAdjoint ? kMatCacheWide : kMatCache;
+  __shared__ float2 mcache[kWindow];
+  MatCache mc = mat_cache(mcache, a.mats, kWindow);
   __shared__ float2 rotS[8];
   if (MODE == kModeSampledExpectation && threadIdx.x == 0) {
 #pragma unroll

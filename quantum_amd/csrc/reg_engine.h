@@ -761,21 +761,24 @@ __device__ __forceinline__ void reg_channel(int j, cf (&a)[1 << R], const DevOp&
 // the same op in every wave; windows stay valid for the whole launch because
 // the matrices do not change while it runs.
 struct MatCache {
-  float2* lds;  // kMatCache entries
+  float2* lds;  // n entries
   const float2* g;
   uint32_t lo, hi;
+  uint32_t n;
 };
-__device__ __forceinline__ MatCache mat_cache(float2* lds, const float2* g) {
-  return MatCache{lds, g, 0u, 0u};
+__device__ __forceinline__ MatCache mat_cache(float2* lds, const float2* g,
+                                              uint32_t n = kMatCache) {
+  return MatCache{lds, g, 0u, 0u, n};
 }
-// Make entries [off, off + len) resident in the window.
+// Make entries [off, off + len) resident in the window (every plan pads its
+// matrix array by at least its kernel's window, so a refill stays inside).
 __device__ __forceinline__ void cache_span(MatCache& c, uint32_t off, uint32_t len) {
   if (off >= c.lo && off + len <= c.hi) return;
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < uint32_t(kMatCache); i += kThreads) c.lds[i] = c.g[off + i];
+  for (uint32_t i = threadIdx.x; i < c.n; i += kThreads) c.lds[i] = c.g[off + i];
   __syncthreads();
   c.lo = off;
-  c.hi = off + uint32_t(kMatCache);
+  c.hi = off + c.n;
 }
 // Matrix entries an op reads: its unitary (or diagonal) and derivatives,
 // stored back to back.

@@ -48,6 +48,9 @@ constexpr int kRegThreadBits = 9;
 // complex entries over the plan's matrix array; the device copy of that array
 // is padded by the same count so a window never reads past it.
 constexpr int kMatCache = 768;
+// The adjoint batch kernel keeps two 13-qubit tiles (128 KiB) and one
+// workgroup per CU, so its window can take the LDS the others leave free.
+constexpr int kMatCacheWide = 3072;
 // Register qubits of the register engine for an L-qubit tile (0: no engine):
 // L - 9 for tiles of 12..(9 + max_r) qubits.  Two-state runs and the batched
 // kernel use at most 4; one-state streaming passes may use 5 (14-qubit tiles).

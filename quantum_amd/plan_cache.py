@@ -4,9 +4,11 @@ The reference parses, resolves and fuses every program on every call
 (``core/ops/parse_context.cc:60-92``).  Here an op call whose programs, symbol
 names and Pauli sums are byte-identical to a recent call's reuses that call's
 plan: only the symbolic gate matrices are re-evaluated for the new symbol
-values (``tfqx_plan_rebind``), and for the adjoint the lambda weights for the
-new downstream gradients (``tfqx_plan_rebind_downstream``); then the program is
-uploaded and launched.  A training loop therefore pays the host planner once.
+values (on the device, ``tfqx_plan_stage_values``), and for the adjoint the
+lambda weights for the new downstream gradients
+(``tfqx_plan_rebind_downstream``); the upload then carries only what changed
+(``tfqx_plan_upload_changed``).  A training loop therefore pays the host
+planner once.
 
 Keys hold a 128-bit xxh3 digest of the string tensors' bytes and offsets, the
 symbol names, the shapes, the op's other arguments, the caller's stream and
@@ -21,8 +23,10 @@ the key.
 """
 
 import collections
+import ctypes
 import threading
 
+import numpy as np
 import xxhash
 
 from .serialize import pack_strings
@@ -56,7 +60,17 @@ def bump_generation():
 
 
 _MEMO_SIZE = 16
-_MEMO = collections.OrderedDict()  # element ids -> (elements, digest)
+_MEMO = collections.OrderedDict()  # element ids -> (elements, digest, id array or None)
+
+
+def _element_ids(arr):
+    """The element ids of a C-contiguous object ndarray as uint64 (a copy of
+    the array's PyObject pointer buffer, read in one step), or None."""
+    if not (isinstance(arr, np.ndarray) and arr.dtype == object and arr.flags.c_contiguous
+            and arr.size > 0 and ctypes.sizeof(ctypes.c_void_p) == 8):
+        return None
+    buf = (ctypes.c_uint64 * arr.size).from_address(arr.ctypes.data)
+    return np.frombuffer(buf, dtype=np.uint64).copy()
 
 
 def _tensor_digest(arr):
@@ -65,14 +79,22 @@ def _tensor_digest(arr):
     4096-circuit batch); its digest is memoised on the identity of the
     elements.  The memo holds the elements themselves, so their ids cannot be
     reused by other objects while the entry lives, and ``bytes`` are
-    immutable: equal ids mean equal contents."""
-    elems = tuple(arr.reshape(-1)) if hasattr(arr, "reshape") else tuple(arr)
-    ident = (getattr(arr, "shape", None), tuple(map(id, elems)))
+    immutable: equal ids mean equal contents.  For object ndarrays the ids
+    are the array's pointer buffer, compared in bulk."""
+    ids = _element_ids(arr)
+    if ids is not None:
+        ident = (arr.shape, xxhash.xxh3_128_digest(ids.tobytes()))
+        elems = None
+    else:
+        elems = tuple(arr.reshape(-1)) if hasattr(arr, "reshape") else tuple(arr)
+        ident = (getattr(arr, "shape", None), tuple(map(id, elems)))
     with _LOCK:
         hit = _MEMO.get(ident)
-        if hit is not None:
+        if hit is not None and (ids is None or np.array_equal(hit[2], ids)):
             _MEMO.move_to_end(ident)
             return hit[1]
+    if elems is None:
+        elems = tuple(arr.reshape(-1))
     if not all(type(x) is bytes for x in elems):
         data, offsets = pack_strings(arr)
         h = xxhash.xxh3_128()
@@ -85,7 +107,7 @@ def _tensor_digest(arr):
         h.update(x)
     d = h.digest()
     with _LOCK:
-        _MEMO[ident] = (elems, d)
+        _MEMO[ident] = (elems, d, ids)
         while len(_MEMO) > _MEMO_SIZE:
             _MEMO.popitem(last=False)
     return d

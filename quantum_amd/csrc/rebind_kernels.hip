@@ -463,7 +463,7 @@ __device__ void EvalOpWide(const RebindArgs& a, const RbOp& op, const RbRow& row
 // One thread per (row, op): per-row tables list every row's ops (op.row
 // names the row); shared tables hold one program's ops for every row.
 template <int DIM>
-__global__ __launch_bounds__(kRbThreads) void rebind_kernel(RebindArgs a) {
+__global__ __launch_bounds__(kRbThreads, 2) void rebind_kernel(RebindArgs a) {
   const long long i = (long long)blockIdx.x * kRbThreads + threadIdx.x;
   const long long items = a.shared ? (long long)a.num_ops * a.num_rows : a.num_ops;
   if (i >= items) return;

@@ -42,8 +42,8 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "batch_kernel_uneven4096_traffic.j
 
 def host_threads():
     """Host threads for the CPU legs: every core this process may run on,
-    capped by OMP_NUM_THREADS when the machine sets it (the GPU boxes allot
-    16 host threads to each GPU)."""
+    capped by OMP_NUM_THREADS when the machine sets it (a machine that shares
+    its cores among GPUs sets it to one GPU's share, e.g. 16)."""
     cores = len(os.sched_getaffinity(0))
     omp = os.environ.get("OMP_NUM_THREADS", "")
     return max(1, min(cores, int(omp))) if omp.isdigit() and int(omp) > 0 else cores

@@ -240,3 +240,45 @@ def test_op_calls_with_new_values_match_host_plans_and_oracle():
         ex = sim.tfq_simulate_expectation(progs, names, v, ps).cpu().numpy()
         np.testing.assert_allclose(ex, eo, atol=TOL * max(1, np.abs(eo).max()))
     plan_cache.clear()
+
+
+def test_op_calls_without_symbols_and_with_empty_rows():
+    """Reused plans whose rows have no symbols (zero-width values) or are
+    empty programs: staging and the changed-region upload have nothing to
+    evaluate and the outputs stay equal to the oracle's (-2 for empty rows
+    in expectations, zero gradients)."""
+    from quantum_amd import plan_cache
+    from quantum_amd import tfq_adj_grad_op as A
+    from quantum_amd import tfq_simulate_ops as sim
+    rng = np.random.default_rng(101)
+    plan_cache.clear()
+    q = C.LineQubit.range(6)
+    fixed = [W.random_circuit(rng, q, 18, symbols=(), p_symbol=0.0), C.Circuit(),
+             W.random_circuit(rng, q[:4], 10, symbols=(), p_symbol=0.0)]
+    progs = S.convert_to_tensor(fixed)
+    sums = S.convert_to_tensor([[W.random_pauli_sum(rng, q, 2)], [C.PauliSum([])],
+                                [W.random_pauli_sum(rng, q[:4], 2)]])
+    none = np.zeros((3, 0), np.float32)
+    eo = O.simulate_expectation(list(progs), [], none, sums)
+    for call in range(3):
+        _say(f"symbol-free op call {call}")
+        ex = sim.tfq_simulate_expectation(progs, [], none, sums).cpu().numpy()
+        np.testing.assert_allclose(ex, eo, atol=TOL)
+        g = rng.uniform(-1, 1, size=(3, 1)).astype(np.float32)
+        e2, gr = A.expectation_and_gradient(progs, [], none, sums, g)
+        np.testing.assert_allclose(e2.cpu().numpy(), eo, atol=TOL)
+        assert gr.shape == (3, 0)
+    names = ["a"]
+    circ = [W.random_circuit(rng, q, 18, symbols=names, p_symbol=0.5), C.Circuit()]
+    progs2 = S.convert_to_tensor(circ)
+    sums2 = S.convert_to_tensor([[W.random_pauli_sum(rng, q, 2)], [C.PauliSum([])]])
+    for call in range(3):
+        v = rng.uniform(-2, 2, size=(2, 1)).astype(np.float32)
+        g = rng.uniform(-1, 1, size=(2, 1)).astype(np.float32)
+        _say(f"op call {call} with an empty row")
+        e2, gr = A.expectation_and_gradient(progs2, names, v, sums2, g, derivative="analytic")
+        eo2 = O.simulate_expectation(list(progs2), names, v, sums2)
+        go2 = O.adjoint_gradient(list(progs2), names, v, sums2, g, mode="analytic")
+        np.testing.assert_allclose(e2.cpu().numpy(), eo2, atol=TOL)
+        np.testing.assert_allclose(gr.cpu().numpy(), go2, atol=TOL * max(1, np.abs(go2).max()))
+    plan_cache.clear()

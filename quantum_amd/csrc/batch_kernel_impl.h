@@ -288,7 +288,7 @@ __device__ __forceinline__ void wave_circuit(const BatchArgs& a, int ci, float2*
 
 // Tiled forward: the state lives in global memory G; each pass streams its
 // tiles through LDS buffer A.  The first pass starts from |0>.
-template <int KMAX>
+template <int KMAX, int RMAX = 4>
 __device__ __forceinline__ void tiled_forward(float2* G, int n, const BatchArgs& a, int pb, int pe, float2* A,
                               uint64_t* tab, MatCache& mc) {
   const int tid = threadIdx.x;
@@ -317,7 +317,8 @@ __device__ __forceinline__ void tiled_forward(float2* G, int n, const BatchArgs&
             }
           },
           [&](uint64_t g, const cf& va, const cf&) { G[g] = make_float2(va.x, va.y); });
-      if (reg_run_any<false>(A, A, ps.L, a.ops, ps.op_begin, ps.op_end, mc, NoAcc(), io)) continue;
+      if (reg_run_any<false, RMAX>(A, A, ps.L, a.ops, ps.op_begin, ps.op_end, mc, NoAcc(), io))
+        continue;
       if (first) {
         for (uint32_t j = tid; j < T; j += kThreads) A[j] = make_float2(gidx(j) == 0 ? 1.f : 0.f, 0.f);
       } else {
@@ -461,8 +462,11 @@ __global__ __launch_bounds__(kThreads, BatchMinWaves(MODE)) void batch_kernel(Ba
       }
     }
     if (tiled) {
-      tiled_forward<KMAX>(gpsi, n, a, uniform(c.fwd_pass_begin), uniform(c.fwd_pass_end), psi,
-                    tab, mc);
+      // The adjoint's forward carries one state: its passes may use 14-qubit
+      // tiles (the LDS of the two 13-qubit reverse tiles) and five register
+      // qubits.
+      tiled_forward<KMAX, MODE == kModeAdjoint ? 5 : 4>(gpsi, n, a, uniform(c.fwd_pass_begin),
+                                                        uniform(c.fwd_pass_end), psi, tab, mc);
     } else {
       set_zero_state(psi, N);
       run_ops<KMAX>(psi, n, a.ops, uniform(c.fwd_begin), uniform(c.fwd_end), mc);

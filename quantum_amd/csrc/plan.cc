@@ -788,7 +788,12 @@ void BuildPiece(const CircuitJob& job, int num_slots, const PlanOptions& opt, Pi
   dc.fwd_begin = 0;
   dc.fwd_pass_begin = 0;
   if (tiled) {
-    SchedulePasses(fused, dc.n, opt.tile_log2, &pc->passes, &pc->ops, &pc->mats, 4, rc, rec);
+    // Adjoint plans: the forward holds one state, so its tiles may take the
+    // LDS of the reverse sweep's two tiles (one more qubit, five register
+    // qubits).
+    const bool wide = opt.mode == kModeAdjoint && g_reg_engine.load() >= 2;
+    const int fwd_L = wide ? std::min(opt.tile_log2 + 1, dc.n) : opt.tile_log2;
+    SchedulePasses(fused, dc.n, fwd_L, &pc->passes, &pc->ops, &pc->mats, wide ? 5 : 4, rc, rec);
   } else {
     for (const auto& f : fused) pc->ops.push_back(MakeDevOp(f, &pc->mats, rc, rec));
     remaps(size_t(dc.fwd_begin));
